@@ -1,0 +1,187 @@
+/*
+ * pirgpu — MI355X (gfx950) implementation of ReinsPIRe's online PIR answer
+ * path, exposed as a plain C ABI.
+ *
+ * This header is the drop-in boundary. It replaces the server API of
+ * google/InsPIRe's PirServer<uint16_t|uint8_t, uint64_t, int32_t|int16_t>
+ * (crypto/pir/server/pir_server.h:41-111). Every entry point names the
+ * reference declaration it stands for. No absl, C++ or torch types appear
+ * here: only plain pointers, sizes and int status codes.
+ *
+ * Status codes are numerically equal to absl::StatusCode:
+ *   0 OK, 3 INVALID_ARGUMENT, 8 RESOURCE_EXHAUSTED, 9 FAILED_PRECONDITION,
+ *   13 INTERNAL, 14 UNAVAILABLE (no usable GPU).
+ * Every function that can fail takes (char* err, size_t errlen) and writes the
+ * same message text the reference returns for the same condition.
+ *
+ * Threading: a pirg_server is externally synchronised (like the reference,
+ * whose ProcessResponse is not safe to call concurrently), except for the
+ * serving-loop calls pirg_serve_submit / pirg_serve_poll, which are
+ * thread-safe.
+ */
+#ifndef PIRGPU_H_
+#define PIRGPU_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIRG_OK 0
+#define PIRG_INVALID_ARGUMENT 3
+#define PIRG_RESOURCE_EXHAUSTED 8
+#define PIRG_FAILED_PRECONDITION 9
+#define PIRG_INTERNAL 13
+#define PIRG_UNAVAILABLE 14
+
+/* PirParams<uint64_t> + RlweParams + the two GadgetParams
+ * (crypto/pir/pir_params.h:36-105, crypto/encryption.h:29-98), with the
+ * compile-time types of the reference's template instantiation made runtime:
+ * db_bytes 2 = uint16_t DB, 1 = uint8_t DB; mat_bytes 4 = int32_t pack
+ * matrix (condensed), 2 = int16_t pack matrix (plain). CoeffType is uint64_t.
+ * log_q is LogModulus() (64 means modulus 0, i.e. Z_{2^64}). */
+typedef struct {
+  int log_degree;
+  int log_q;
+  int log_q1;
+  int log_q2;
+  int variance;
+  uint64_t plaintext_modulus;
+  int pack_log_digit, pack_num_digits;
+  int eval_log_digit, eval_num_digits;
+  int num_entries;
+  int interp_degree;
+  int entry_size_multiple;
+  int db_bytes;
+  int mat_bytes;
+} pirg_params;
+
+/* View of PirPreprocessedData<Db, uint64_t, Mat>
+ * (crypto/pir/server/pir_preprocessed_data.h:38-50): the per-shard DB
+ * matrices (condensed words, crypto/matrix.cc:76-160), the [shard][t]
+ * PreprocessMatrixPackOutput entries (crypto/lwes_to_rlwe.h:42-47) and
+ * second_dim_a. All pointers are host pointers, borrowed for the call. */
+typedef struct {
+  int num_db_matrices;               /* db_matrices.size() */
+  const uint64_t* const* db_cdata;   /* [num_db_matrices] rows x words */
+  const int* db_rows;                /* [num_db_matrices] Rows() */
+  const int* db_cols;                /* [num_db_matrices] Cols() (original) */
+  int num_outputs;                   /* preprocessed_outputs.size() */
+  const int* outputs_len;            /* [num_outputs] preprocessed_outputs[k].size() */
+  /* [num_outputs][t] (row-major over shard then i): */
+  const void* const* pack;           /* d x k*d: int32 condensed words or int16 */
+  const uint64_t* const* a_tilde;    /* d coefficients */
+  const uint64_t* const* t_vec_h;    /* k x d coefficients */
+  int second_dim_a_len;              /* second_dim_a.size() */
+  const uint64_t* second_dim_a;      /* second_dim_a_len x d */
+} pirg_preprocessed_view;
+
+typedef struct pirg_server pirg_server;
+
+/* PirServer::Create (crypto/pir/server/pir_server.cc:52-124). Validates with
+ * the reference's messages, then moves the material into HBM in the
+ * layouts the kernels stream. */
+int pirg_server_create(const pirg_params* params,
+                       const pirg_preprocessed_view* data,
+                       pirg_server** out, char* err, size_t errlen);
+
+/* The same, from the already-concatenated form PirServer keeps after Create
+ * (combined_db_matrix_ / combined_pack_matrix_, pir_server.cc:100-118):
+ *   db_cdata: M x (db_cols/4) words (uint16 DB) or M x (db_cols/8) (uint8),
+ *             M = ell * t * d, shard-major;
+ *   pack:     M rows, (shard, i)-major, each d x k*d (int32 condensed words
+ *             or int16 values);
+ *   a_tilde:  (ell*t) x d;  t_vec_h: (ell*t) x k x d;
+ *   second_dim_a: 2e x d (may be NULL when t == 1). */
+int pirg_server_create_flat(const pirg_params* params, const uint64_t* db_cdata,
+                            int db_cols, const void* pack,
+                            const uint64_t* a_tilde, const uint64_t* t_vec_h,
+                            const uint64_t* second_dim_a, pirg_server** out,
+                            char* err, size_t errlen);
+
+void pirg_server_destroy(pirg_server* server);
+
+/* PirServer::ProcessResponse (crypto/pir/server/pir_server.cc:152-306).
+ * Request = PirRequest<uint64_t> (crypto/pir/pir_params.h:196-208):
+ *   first_dim[first_dim_len], second_dim[second_dim_polys x d],
+ *   packing_key[packing_key_polys x d].
+ * Response = PirResponse<uint64_t> (pir_params.h:211-216) written to
+ *   out[ell][2][d] as (a_0, b_0, a_1, b_1, ...). */
+int pirg_process(pirg_server* server, const uint64_t* first_dim,
+                 size_t first_dim_len, const uint64_t* second_dim,
+                 size_t second_dim_polys, const uint64_t* packing_key,
+                 size_t packing_key_polys, uint64_t* out, char* err,
+                 size_t errlen);
+
+/* One PirRequest, borrowed. */
+typedef struct {
+  const uint64_t* first_dim;
+  size_t first_dim_len;
+  const uint64_t* second_dim;
+  size_t second_dim_polys;
+  const uint64_t* packing_key;
+  size_t packing_key_polys;
+} pirg_request_view;
+
+/* Many PirServer::ProcessResponse calls answered by one batched pass: the DB
+ * and pack products become int8-limb MFMA matrix products over the batch.
+ * statuses[i] receives request i's status (its validation mirrors
+ * ProcessResponse); outs[i] (ell x 2 x d) is written when statuses[i] == 0.
+ * No reference counterpart: the reference answers one request per call. */
+int pirg_process_batch(pirg_server* server, int nreq,
+                       const pirg_request_view* reqs, uint64_t* const* outs,
+                       int* statuses, char* err, size_t errlen);
+
+/* Device-resident batch, for callers that keep requests in HBM:
+ *   pirg_batch_stage copies nreq requests into the server's HBM workspace;
+ *   pirg_batch_run answers the staged batch (all inputs already in HBM);
+ *   pirg_batch_fetch copies the nreq responses back.
+ * All requests of a staged batch must have first_dim_len == r (InterpolatedRows)
+ * and the full second_dim / packing_key; pirg_batch_stage validates them. */
+int pirg_batch_stage(pirg_server* server, int nreq,
+                     const pirg_request_view* reqs, char* err, size_t errlen);
+int pirg_batch_run(pirg_server* server, int nreq, char* err, size_t errlen);
+int pirg_batch_fetch(pirg_server* server, int nreq, uint64_t* const* outs,
+                     char* err, size_t errlen);
+
+/* PirServer::GetDbMultiplyTime / GetPackingTime / GetPolyEvalTime
+ * (pir_server.h:70-72): cumulative GPU time, in milliseconds, of the DB
+ * product, the packing (pack product + FinalizeMatrixPack) and EvalPoly
+ * stages over every call so far. */
+int pirg_server_stage_times(pirg_server* server, double* db_mult_ms,
+                            double* packing_ms, double* poly_eval_ms);
+
+/* Largest |x - round(x)| over every value the FFT products rounded since the
+ * last reset, the precision margin of the double-precision products. */
+double pirg_server_fft_margin(pirg_server* server, int reset);
+
+/* Matrix<uint16_t>::Multiply<uint64_t> on a condensed matrix
+ * (crypto/matrix.h:165-195 -> crypto/matrix.cc:261-318), full 64-bit result,
+ * computed by the same int8-limb kernel as the DB product. Host pointers. */
+int pirg_matrix_multiply_u16(const uint64_t* cdata, int rows, int cols,
+                             const uint64_t* vec, uint64_t* out, char* err,
+                             size_t errlen);
+/* Matrix<int32_t>::Multiply<uint64_t> on a condensed matrix
+ * (crypto/matrix.cc:320-374). */
+int pirg_matrix_multiply_i32(const uint64_t* cdata, int rows, int cols,
+                             const uint64_t* vec, uint64_t* out, char* err,
+                             size_t errlen);
+
+/* Polynomial::InnerProductFft (crypto/polynomial_fft.cc:68-197): n
+ * polynomials u, v of length d (n x d each), on the GPU with the reference's
+ * FFT arithmetic. */
+int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
+                           int u_bits, int v_bits, int chunk_bits,
+                           int u_is_signed, uint64_t* out, char* err,
+                           size_t errlen);
+
+/* Version / device info string (for logs). */
+const char* pirg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIRGPU_H_ */

@@ -1,0 +1,415 @@
+"""inspire_amd: MI355X (gfx950) ReinsPIRe online PIR answer path.
+
+Python view of ``libpirgpu.so`` (the C ABI declared in include/pirgpu.h).
+``PirServer`` mirrors the reference's ``PirServer<Db, uint64_t, Mat>``
+(crypto/pir/server/pir_server.h:41-111): ``create`` takes the preprocessed
+material, ``process_response`` answers a ``PirRequest`` with a ``PirResponse``,
+and the stage-time getters report the DB / packing / poly-eval split. Errors
+raise ``PirError`` carrying the reference's status code and message.
+
+The library is built in-tree by ``inspire_amd.build``; importing this package
+never falls back to a CPU implementation: without the library or a GPU, calls
+fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpirgpu.so")
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_lib = None
+
+OK = 0
+INVALID_ARGUMENT = 3
+RESOURCE_EXHAUSTED = 8
+FAILED_PRECONDITION = 9
+INTERNAL = 13
+UNAVAILABLE = 14
+
+
+class PirError(RuntimeError):
+    """A non-OK status from libpirgpu (code = absl::StatusCode number)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libpirgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m inspire_amd.build`")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.pirg_version.restype = ctypes.c_char_p
+        _lib.pirg_server_fft_margin.restype = ctypes.c_double
+        _lib.pirg_server_destroy.restype = None
+    return _lib
+
+
+class _CParams(ctypes.Structure):
+    _fields_ = [
+        ("log_degree", ctypes.c_int),
+        ("log_q", ctypes.c_int),
+        ("log_q1", ctypes.c_int),
+        ("log_q2", ctypes.c_int),
+        ("variance", ctypes.c_int),
+        ("plaintext_modulus", ctypes.c_uint64),
+        ("pack_log_digit", ctypes.c_int),
+        ("pack_num_digits", ctypes.c_int),
+        ("eval_log_digit", ctypes.c_int),
+        ("eval_num_digits", ctypes.c_int),
+        ("num_entries", ctypes.c_int),
+        ("interp_degree", ctypes.c_int),
+        ("entry_size_multiple", ctypes.c_int),
+        ("db_bytes", ctypes.c_int),
+        ("mat_bytes", ctypes.c_int),
+    ]
+
+
+class _CView(ctypes.Structure):
+    _fields_ = [
+        ("num_db_matrices", ctypes.c_int),
+        ("db_cdata", ctypes.POINTER(_u64p)),
+        ("db_rows", ctypes.POINTER(ctypes.c_int)),
+        ("db_cols", ctypes.POINTER(ctypes.c_int)),
+        ("num_outputs", ctypes.c_int),
+        ("outputs_len", ctypes.POINTER(ctypes.c_int)),
+        ("pack", ctypes.POINTER(ctypes.c_void_p)),
+        ("a_tilde", ctypes.POINTER(_u64p)),
+        ("t_vec_h", ctypes.POINTER(_u64p)),
+        ("second_dim_a_len", ctypes.c_int),
+        ("second_dim_a", _u64p),
+    ]
+
+
+class _CRequest(ctypes.Structure):
+    _fields_ = [
+        ("first_dim", _u64p),
+        ("first_dim_len", ctypes.c_size_t),
+        ("second_dim", _u64p),
+        ("second_dim_polys", ctypes.c_size_t),
+        ("packing_key", _u64p),
+        ("packing_key_polys", ctypes.c_size_t),
+    ]
+
+
+@dataclasses.dataclass(frozen=True)
+class PirParams:
+    """PirParams<uint64_t> (crypto/pir/pir_params.h:36-105) with the
+    reference's template types as fields (db_bytes 2 = uint16_t DB, 1 = uint8_t;
+    mat_bytes 4 = int32_t pack matrix, 2 = int16_t)."""
+
+    log_degree: int = 11
+    log_q: int = 52
+    log_q1: int = 29
+    log_q2: int = 18
+    variance: int = 40
+    plaintext_modulus: int = 65535
+    pack_log_digit: int = 19
+    pack_num_digits: int = 2
+    eval_log_digit: int = 19
+    eval_num_digits: int = 3
+    num_entries: int = 4096
+    interp_degree: int = 2
+    entry_size_multiple: int = 1
+    db_bytes: int = 2
+    mat_bytes: int = 4
+
+    @property
+    def degree(self) -> int:
+        return 1 << self.log_degree
+
+    @property
+    def interpolated_rows(self) -> int:
+        """PirParams::InterpolatedRows (pir_params.h:71)."""
+        return self.num_entries // self.interp_degree
+
+    def c(self) -> _CParams:
+        return _CParams(*[getattr(self, f.name) for f in dataclasses.fields(self)])
+
+
+def metrics_tool_params(n: int, log_d: int, t: int, ell: int = 1) -> PirParams:
+    """pir_metrics_tool defaults (crypto/pir/benchmarking/pir_metrics_tool.cc:79-87,111-114)."""
+    return PirParams(log_degree=log_d, log_q=52, log_q1=16 + log_d + 2, log_q2=18,
+                     variance=40, plaintext_modulus=65535, pack_log_digit=19,
+                     pack_num_digits=2, eval_log_digit=19, eval_num_digits=3,
+                     num_entries=n, interp_degree=t, entry_size_multiple=ell)
+
+
+@dataclasses.dataclass
+class PirRequest:
+    """PirRequest<uint64_t> (crypto/pir/pir_params.h:196-208)."""
+
+    first_dimension_query: np.ndarray          # r .. K values
+    second_dimension_query: np.ndarray         # 2e x d (empty when t == 1)
+    packing_key: np.ndarray                    # 2k x d
+
+
+@dataclasses.dataclass
+class PirResponse:
+    """PirResponse<uint64_t> (crypto/pir/pir_params.h:211-216): ell RLWE
+    ciphertexts, as an (ell, 2, d) array of (a, b)."""
+
+    ciphertexts: np.ndarray
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def _u64_2d(a) -> np.ndarray:
+    # Convert before reshaping: numpy infers float64 for Python ints >= 2^63.
+    a = _u64(a)
+    return a.reshape(1, -1) if a.ndim == 1 else a
+
+
+def _check(rc: int, err) -> None:
+    if rc != 0:
+        raise PirError(rc, err.value.decode())
+
+
+def _err():
+    return ctypes.create_string_buffer(1024)
+
+
+class _RequestArrays:
+    """Keeps the numpy buffers of a request alive while C reads them."""
+
+    def __init__(self, req: PirRequest, d: int):
+        self.fd = _u64(req.first_dimension_query).reshape(-1)
+        sd = _u64(req.second_dimension_query)
+        self.sd = sd.reshape(-1) if sd.size else np.zeros(1, dtype=np.uint64)
+        self.sd_polys = sd.shape[0] if sd.ndim == 2 else (sd.size // d)
+        pk = _u64(req.packing_key)
+        self.pk = pk.reshape(-1)
+        self.pk_polys = pk.shape[0] if pk.ndim == 2 else (pk.size // d)
+
+    def view(self) -> _CRequest:
+        return _CRequest(self.fd.ctypes.data_as(_u64p), self.fd.size,
+                         self.sd.ctypes.data_as(_u64p), self.sd_polys,
+                         self.pk.ctypes.data_as(_u64p), self.pk_polys)
+
+
+class PirServer:
+    """PirServer<uint16_t|uint8_t, uint64_t, int32_t|int16_t> on the GPU."""
+
+    def __init__(self, handle, params: PirParams):
+        self._h = handle
+        self.params = params
+
+    @classmethod
+    def create(cls, params: PirParams, db_matrices: Sequence[np.ndarray],
+               db_cols: int, pack: Sequence[Sequence[np.ndarray]],
+               a_tilde: Sequence[Sequence[np.ndarray]],
+               t_vec_h: Sequence[Sequence[np.ndarray]],
+               second_dim_a: Optional[np.ndarray],
+               db_rows: Optional[Sequence[int]] = None,
+               db_cols_each: Optional[Sequence[int]] = None) -> "PirServer":
+        """PirServer::Create (crypto/pir/server/pir_server.cc:52-124).
+
+        db_matrices[k]: condensed words of shard k; pack[k][i], a_tilde[k][i],
+        t_vec_h[k][i]: the PreprocessMatrixPackOutput fields of (shard k, i)."""
+        keep = []
+        dbs = [_u64(m) for m in db_matrices]
+        keep += dbs
+        n_db = len(dbs)
+        db_arr = (_u64p * max(1, n_db))(*[m.ctypes.data_as(_u64p) for m in dbs])
+        rows = list(db_rows) if db_rows is not None else [m.shape[0] for m in dbs]
+        cols = list(db_cols_each) if db_cols_each is not None else [db_cols] * n_db
+        rows_arr = (ctypes.c_int * max(1, n_db))(*rows)
+        cols_arr = (ctypes.c_int * max(1, n_db))(*cols)
+        n_out = len(pack)
+        olen = (ctypes.c_int * max(1, n_out))(*[len(x) for x in pack])
+        flat_pk, flat_at, flat_th = [], [], []
+        for k in range(n_out):
+            for i in range(len(pack[k])):
+                pk = np.ascontiguousarray(pack[k][i])
+                at = _u64(a_tilde[k][i])
+                th = _u64(t_vec_h[k][i])
+                keep += [pk, at, th]
+                flat_pk.append(pk.ctypes.data_as(ctypes.c_void_p))
+                flat_at.append(at.ctypes.data_as(_u64p))
+                flat_th.append(th.ctypes.data_as(_u64p))
+        n_flat = max(1, len(flat_pk))
+        pk_arr = (ctypes.c_void_p * n_flat)(*flat_pk)
+        at_arr = (_u64p * n_flat)(*flat_at)
+        th_arr = (_u64p * n_flat)(*flat_th)
+        if second_dim_a is not None and np.asarray(second_dim_a).size:
+            sda = _u64(second_dim_a)
+            keep.append(sda)
+            sda_len = sda.shape[0]
+            sda_ptr = sda.ctypes.data_as(_u64p)
+        else:
+            sda_len, sda_ptr = 0, None
+        view = _CView(n_db, db_arr, rows_arr, cols_arr, n_out, olen, pk_arr, at_arr,
+                      th_arr, sda_len, sda_ptr)
+        handle = ctypes.c_void_p()
+        err = _err()
+        cp = params.c()
+        rc = lib().pirg_server_create(ctypes.byref(cp), ctypes.byref(view),
+                                      ctypes.byref(handle), err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return cls(handle, params)
+
+    @classmethod
+    def create_flat(cls, params: PirParams, db: np.ndarray, db_cols: int, pack: np.ndarray,
+                    a_tilde: np.ndarray, t_vec_h: np.ndarray,
+                    second_dim_a: Optional[np.ndarray]) -> "PirServer":
+        """Create from the concatenated material (pirg_server_create_flat)."""
+        db = _u64(db)
+        pack = np.ascontiguousarray(pack)
+        a_tilde = _u64(a_tilde)
+        t_vec_h = _u64(t_vec_h)
+        sda = _u64(second_dim_a) if second_dim_a is not None else np.zeros(1, np.uint64)
+        handle = ctypes.c_void_p()
+        err = _err()
+        cp = params.c()
+        rc = lib().pirg_server_create_flat(
+            ctypes.byref(cp), db.ctypes.data_as(_u64p), db_cols,
+            pack.ctypes.data_as(ctypes.c_void_p), a_tilde.ctypes.data_as(_u64p),
+            t_vec_h.ctypes.data_as(_u64p), sda.ctypes.data_as(_u64p), ctypes.byref(handle),
+            err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return cls(handle, params)
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().pirg_server_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def _out_shape(self):
+        p = self.params
+        return (p.entry_size_multiple, 2, p.degree)
+
+    def process_response(self, request: PirRequest) -> PirResponse:
+        """PirServer::ProcessResponse (crypto/pir/server/pir_server.cc:152-306)."""
+        ra = _RequestArrays(request, self.params.degree)
+        out = np.zeros(self._out_shape, dtype=np.uint64)
+        err = _err()
+        rc = lib().pirg_process(self._h, ra.fd.ctypes.data_as(_u64p), ctypes.c_size_t(ra.fd.size),
+                                ra.sd.ctypes.data_as(_u64p), ctypes.c_size_t(ra.sd_polys),
+                                ra.pk.ctypes.data_as(_u64p), ctypes.c_size_t(ra.pk_polys),
+                                out.ctypes.data_as(_u64p), err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return PirResponse(out)
+
+    def process_batch(self, requests: Sequence[PirRequest]):
+        """Answers many requests in one batched pass; returns (responses,
+        statuses, first error message); responses[i] is None when statuses[i] != 0."""
+        n = len(requests)
+        ras = [_RequestArrays(r, self.params.degree) for r in requests]
+        views = (_CRequest * max(1, n))(*[ra.view() for ra in ras])
+        outs = [np.zeros(self._out_shape, dtype=np.uint64) for _ in range(n)]
+        out_ptrs = (_u64p * max(1, n))(*[o.ctypes.data_as(_u64p) for o in outs])
+        statuses = (ctypes.c_int * max(1, n))()
+        err = _err()
+        rc = lib().pirg_process_batch(self._h, n, views, out_ptrs, statuses, err,
+                                      ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        st = list(statuses)[:n]
+        resp: List[Optional[PirResponse]] = [PirResponse(o) if s == 0 else None
+                                             for o, s in zip(outs, st)]
+        return resp, st, err.value.decode()
+
+    # Device-resident batches (inputs staged in HBM before the timed region).
+    def batch_stage(self, requests: Sequence[PirRequest]) -> None:
+        n = len(requests)
+        self._staged = [_RequestArrays(r, self.params.degree) for r in requests]
+        views = (_CRequest * max(1, n))(*[ra.view() for ra in self._staged])
+        err = _err()
+        rc = lib().pirg_batch_stage(self._h, n, views, err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+
+    def batch_run(self, n: int) -> None:
+        err = _err()
+        rc = lib().pirg_batch_run(self._h, n, err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+
+    def batch_fetch(self, n: int) -> List[PirResponse]:
+        outs = [np.zeros(self._out_shape, dtype=np.uint64) for _ in range(n)]
+        out_ptrs = (_u64p * max(1, n))(*[o.ctypes.data_as(_u64p) for o in outs])
+        err = _err()
+        rc = lib().pirg_batch_fetch(self._h, n, out_ptrs, err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return [PirResponse(o) for o in outs]
+
+    def stage_times_ms(self):
+        """(GetDbMultiplyTime, GetPackingTime, GetPolyEvalTime) in ms
+        (crypto/pir/server/pir_server.h:70-72)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        lib().pirg_server_stage_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def fft_margin(self, reset: bool = False) -> float:
+        """Largest |x - round(x)| the FFT products have rounded so far."""
+        return lib().pirg_server_fft_margin(self._h, int(reset))
+
+
+def matrix_multiply_u16(cdata: np.ndarray, cols: int, vec: np.ndarray) -> np.ndarray:
+    """Matrix<uint16_t>::Multiply<uint64_t> on condensed words (crypto/matrix.cc:261-318)."""
+    cdata = _u64(cdata)
+    vec = _u64(vec)
+    rows = cdata.size // (cols // 4)
+    out = np.zeros(rows, dtype=np.uint64)
+    err = _err()
+    rc = lib().pirg_matrix_multiply_u16(cdata.ctypes.data_as(_u64p), rows, cols,
+                                        vec.ctypes.data_as(_u64p), out.ctypes.data_as(_u64p),
+                                        err, ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    return out
+
+
+def matrix_multiply_i32(cdata: np.ndarray, cols: int, vec: np.ndarray) -> np.ndarray:
+    """Matrix<int32_t>::Multiply<uint64_t> on condensed words (crypto/matrix.cc:320-374)."""
+    cdata = _u64(cdata)
+    vec = _u64(vec)
+    rows = cdata.size // (cols // 2)
+    out = np.zeros(rows, dtype=np.uint64)
+    err = _err()
+    rc = lib().pirg_matrix_multiply_i32(cdata.ctypes.data_as(_u64p), rows, cols,
+                                        vec.ctypes.data_as(_u64p), out.ctypes.data_as(_u64p),
+                                        err, ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    return out
+
+
+def inner_product_fft(u, v, u_bits=64, v_bits=64, chunk_bits=20, u_signed=False) -> np.ndarray:
+    """Polynomial::InnerProductFft (crypto/polynomial_fft.cc:68-197) on the GPU."""
+    u = _u64_2d(u)
+    v = _u64_2d(v)
+    n, d = u.shape
+    out = np.zeros(d, dtype=np.uint64)
+    err = _err()
+    rc = lib().pirg_inner_product_fft(u.ctypes.data_as(_u64p), v.ctypes.data_as(_u64p), n, d,
+                                      u_bits, v_bits, chunk_bits, int(u_signed),
+                                      out.ctypes.data_as(_u64p), err, ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    return out
+
+
+def version() -> str:
+    return lib().pirg_version().decode()

@@ -1,0 +1,177 @@
+// Internal declarations shared by the host orchestration (server.cpp) and the
+// HIP kernels (limb_kernels.hip, fft_kernels.hip) of libpirgpu.
+#ifndef INSPIRE_AMD_PIRG_INTERNAL_H_
+#define INSPIRE_AMD_PIRG_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace pirg {
+
+using u64 = uint64_t;
+using i64 = int64_t;
+
+// ---------------------------------------------------------------------------
+// Chunk extraction of Polynomial::InnerProductFft
+// (crypto/polynomial_fft.cc:98-152): chunk c of x is (x >> bits*c) & mask;
+// the top chunk of a signed operand is sign-extended from its remaining bits.
+struct ChunkRule {
+  int chunk_bits;
+  int num_chunks;
+  int bits;        // u_bits / v_bits argument
+  int is_signed;
+};
+
+__host__ __device__ inline double ChunkValue(const ChunkRule& r, u64 x, int c) {
+  const u64 mask = (1ULL << r.chunk_bits) - 1;
+  const u64 chunk = (x >> (r.chunk_bits * c)) & mask;
+  if (r.is_signed && c == r.num_chunks - 1) {
+    const int bits_in_chunk = r.bits - r.chunk_bits * c;
+    const int shift = 64 - bits_in_chunk;
+    const i64 sc = static_cast<i64>(chunk << shift) >> shift;
+    return static_cast<double>(sc);
+  }
+  return static_cast<double>(chunk);
+}
+
+// The (a, b) chunk pairs InnerProductFft accumulates (it skips pairs with
+// chunk_bits*(a+b) >= 64, polynomial_fft.cc:154-173).
+struct PairList {
+  int count;
+  int a[64];
+  int b[64];
+  int shift[64];
+};
+
+inline PairList MakePairs(const ChunkRule& u, const ChunkRule& v) {
+  PairList p{};
+  for (int a = 0; a < u.num_chunks; ++a) {
+    for (int b = 0; b < v.num_chunks; ++b) {
+      if (u.chunk_bits * (a + b) >= 64) continue;
+      p.a[p.count] = a;
+      p.b[p.count] = b;
+      p.shift[p.count] = u.chunk_bits * (a + b);
+      ++p.count;
+    }
+  }
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// Int8-limb operand layout.
+//
+// A matrix A (rows x cols) of small integers is stored as NP int8 "planes":
+// A = C_off + sum_p 2^(8p) plane_p, so that A * y = sum_p 2^(8p) (plane_p * y)
+// + C_off * sum(y). Each plane is tiled for v_mfma_i32_16x16x64_i8: a tile is
+// 16 rows x 64 columns = 64 lanes x 16 bytes; lane l holds row (l % 16),
+// columns 16*(l/16) .. +15. Tiles are ordered [strip][kblock][plane].
+// A vector y (uint64) is split into nl balanced int8 digits
+// y = sum_c 2^(8c) digit_c (mod 2^(8 nl)); the B operand tile for k-block kb
+// holds 16 columns = (query, digit) pairs, qpt queries per tile.
+struct LimbLayout {
+  int rows;      // logical rows
+  int cols;      // logical columns
+  int strips;    // ceil(rows / 16)
+  int kblocks;   // ceil(cols / 64)
+  int planes;    // NP
+  u64 c_off;     // constant offset of the plane decomposition
+};
+
+constexpr int kTileBytes = 1024;
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (limb_kernels.hip).
+
+// Converts condensed uint16 / uint8 DB words (rows x cols/4 or cols/8, device
+// memory) for rows [row0, row0 + nrows) into planes. row0 must be a multiple
+// of 16.
+hipError_t LaunchDbToTiles(const u64* cdata, int db_bytes, int rows, int cols,
+                           int row0, int nrows, const LimbLayout& L,
+                           int8_t* tiles, hipStream_t s);
+// Converts pack rows (int32 condensed words or int16 values, device memory).
+hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
+                             int cols, int row0, int nrows,
+                             const LimbLayout& L, int8_t* tiles,
+                             hipStream_t s);
+// Builds B-operand tiles for nq vectors (vec + q*stride, length len each,
+// zero beyond len) with nl digits, qpt vectors per 16-column tile, and
+// corr[q] = c_off * sum(vec_q) (mod 2^64).
+hipError_t LaunchVectorsToTiles(const u64* vec, int nq, int stride, int len,
+                                int kblocks, int nl, int qpt, u64 c_off,
+                                int8_t* btiles, u64* corr, hipStream_t s);
+// partial[ks][q][row] = sum over the ks-th k-range of the limb product,
+// combined over (plane, digit) pairs with 8(p+c) < out_bits.
+hipError_t LaunchLimbScan(const int8_t* tiles, const LimbLayout& L,
+                          const int8_t* btiles, int nq, int nl, int qpt,
+                          int out_bits, int ksplit, u64* partial,
+                          hipStream_t s);
+int LimbScanKSplit(const LimbLayout& L, int nq);
+
+// ---------------------------------------------------------------------------
+// FFT-side launchers (fft_kernels.hip). N = 2d, twiddles = N/2 complex.
+struct FftDims {
+  int d, log_d, N, log_N;
+};
+
+hipError_t ConfigureFftKernels(int N);
+// dst[(p*num_chunks + c)*N] = FFT of chunk c of poly p, polys addressed as
+// src + (p / per_group) * group_stride + (p % per_group) * d.
+hipError_t LaunchSpectra(const u64* src, int npolys, int per_group,
+                         long long group_stride, const ChunkRule& rule,
+                         const FftDims& F, const double2* tw_fwd,
+                         double2* dst, hipStream_t s);
+
+// Stage arguments of ProcessResponse, batched over nq requests.
+struct FinalizeArgs {
+  const u64* db_partial;   // [ksplit_db][nq][M]
+  int ksplit_db;
+  const u64* corr_db;      // [nq]
+  const u64* pk_partial;   // [ksplit_pk][nq][M]
+  int ksplit_pk;
+  const u64* corr_pk;      // [nq]
+  const u64* fin_partial;  // [nq][ell*t][npairs][d]
+  int npairs;
+  const u64* a_tilde;      // [ell*t][d]
+  u64* ct_b;               // [nq][ell*t][d]
+  u64* st_a;               // [nq][ell][d]
+  u64* st_b;               // [nq][ell][d]
+  u64* out;                // [nq][ell][2][d] (written when t == 1)
+  int nq, M, d, t, ell, log_q, log_q1, log_q2;
+};
+
+struct DigitsArgs {
+  const u64* st_a;   // [nq][ell][d]
+  const u64* st_b;
+  int nq, ell, d, e, log_q, log_digit;
+  ChunkRule rule;    // u rule of the external product (digits)
+  double2* dspec;    // [nq][ell][2e][nu][N]
+};
+
+struct ExtCombineArgs {
+  const u64* ext_partial;  // [nq][ell][2][npairs][d]
+  int npairs;
+  const u64* a_tilde;      // [ell*t][d]
+  const u64* ct_b;         // [nq][ell*t][d]
+  u64* st_a;               // [nq][ell][d]
+  u64* st_b;
+  u64* out;                // [nq][ell][2][d]
+  int nq, ell, t, d, step_i, log_q, log_q1, log_q2;
+};
+
+hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
+                    u64* const* optr, int n_out, int nterms, int nu, int nv,
+                    const PairList& pairs, const FftDims& F,
+                    const double2* tw_bwd, unsigned long long* margin,
+                    hipStream_t s);
+hipError_t LaunchFinalizeCombine(const FinalizeArgs& A, hipStream_t s);
+hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,
+                               const double2* tw_fwd, hipStream_t s);
+hipError_t LaunchExtCombine(const ExtCombineArgs& A, hipStream_t s);
+hipError_t LaunchSumPairs(const u64* partial, int npairs, int d, u64* out,
+                          hipStream_t s);
+
+}  // namespace pirg
+
+#endif  // INSPIRE_AMD_PIRG_INTERNAL_H_

@@ -1,0 +1,1029 @@
+// Host side of libpirgpu: the C ABI of include/pirgpu.h.
+//
+// pirg_server mirrors PirServer<Db, uint64_t, Mat>
+// (crypto/pir/server/pir_server.h:41-111): Create validates and takes the
+// preprocessed material (here: into HBM), ProcessResponse answers a request.
+// Every validation message is the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pirgpu.h"
+#include "pirg_internal.h"
+
+namespace pirg {
+
+namespace {
+
+struct Error {
+  int code = 0;
+  std::string msg;
+  bool ok() const { return code == 0; }
+};
+
+Error Ok() { return Error{}; }
+Error Invalid(const std::string& m) { return Error{PIRG_INVALID_ARGUMENT, m}; }
+Error Internal(const std::string& m) { return Error{PIRG_INTERNAL, m}; }
+Error HipErr(hipError_t e, const char* what) {
+  return Error{PIRG_INTERNAL, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+int Report(const Error& e, char* err, size_t errlen) {
+  if (err != nullptr && errlen > 0) std::snprintf(err, errlen, "%s", e.msg.c_str());
+  return e.code;
+}
+
+#define PIRG_HIP(expr)                                  \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return HipErr(_e, #expr);     \
+  } while (0)
+
+bool IsPow2(u64 x) { return x != 0 && (x & (x - 1)) == 0; }
+
+Error CheckDevice() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    return Error{PIRG_UNAVAILABLE, "no HIP device available"};
+  }
+  return Ok();
+}
+
+// crypto/lwes_to_rlwe.cc:60-74 and crypto/external_product.cc:100-102.
+int ChunkFor(int n_terms, int d) {
+  const int max_allowed = static_cast<int>(
+      (53.0 - std::log2(static_cast<double>(n_terms)) -
+       std::log2(static_cast<double>(d))) /
+      2.0);
+  return std::min(20, max_allowed);
+}
+
+// crypto/polynomial_fft.cc:90-107.
+Error MakeRules(int n_terms, int d, int u_bits, int v_bits, int chunk,
+                bool u_signed, ChunkRule* ur, ChunkRule* vr) {
+  if (u_bits <= 0 || v_bits <= 0 || chunk <= 0 || chunk > 63) {
+    return Invalid("Invalid bits or chunk_bits.");
+  }
+  if (std::log2(static_cast<double>(n_terms)) + std::log2(static_cast<double>(d)) +
+          2.0 * chunk >
+      53.0) {
+    return Invalid("Potential precision overflow in accumulation.");
+  }
+  const int max_chunks = (64 + chunk - 1) / chunk;
+  ur->chunk_bits = vr->chunk_bits = chunk;
+  ur->num_chunks = std::min((u_bits + chunk - 1) / chunk, max_chunks);
+  vr->num_chunks = std::min((v_bits + chunk - 1) / chunk, max_chunks);
+  ur->bits = u_bits;
+  vr->bits = v_bits;
+  ur->is_signed = u_signed ? 1 : 0;
+  vr->is_signed = 0;
+  return Ok();
+}
+
+// Device allocation that is released with the owner.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p != nullptr) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  Error alloc(size_t n) {
+    reset();
+    if (n == 0) n = 8;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return Error{PIRG_RESOURCE_EXHAUSTED,
+                   "device allocation of " + std::to_string(n) + " bytes failed: " +
+                       hipGetErrorString(e)};
+    }
+    bytes = n;
+    return Ok();
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() { reset(); }
+  void reset() {
+    if (p != nullptr) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  Error alloc(size_t n) {
+    reset();
+    if (n == 0) n = 8;
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return Error{PIRG_RESOURCE_EXHAUSTED, "pinned host allocation failed"};
+    }
+    bytes = n;
+    return Ok();
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Per-batch working memory, sized for `cap` requests.
+struct Workspace {
+  int cap = 0;
+  DevBuf fd, sd, pk;             // staged requests
+  DevBuf bt_db, bt_pk;           // B-operand tiles
+  DevBuf corr_db, corr_pk;
+  DevBuf db_partial, pk_partial;
+  DevBuf yh_spec, sdb_spec, d_spec;
+  DevBuf fin_partial, ext_partial;
+  DevBuf ct_b, st_a, st_b, out;
+  DevBuf fin_tab, ext_tab;       // IpKernel pointer tables
+  HostBuf h_fd, h_sd, h_pk, h_out;
+};
+
+}  // namespace
+}  // namespace pirg
+
+using namespace pirg;
+
+struct pirg_server {
+  pirg_params P{};
+  int d = 0, log_d = 0, N = 0, log_N = 0, t = 0, ell = 0, k = 0, e = 0;
+  int r = 0, K = 0, M = 0, nl = 0, qpt = 0, log_q = 0;
+  u64 qmask = 0;
+  FftDims F{};
+  ChunkRule fin_u{}, fin_v{}, ext_u{}, ext_v{};
+  PairList fin_pairs{}, ext_pairs{};
+  Error deferred;  // error ProcessResponse must return (chunk validation)
+  LimbLayout dbL{}, pkL{};
+  int ks_db = 1, ks_pk = 1;
+  DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double db_ms = 0, pack_ms = 0, eval_ms = 0;
+  bool timing_pending = false;
+  Workspace ws;
+  int staged = 0;
+
+  ~pirg_server() {
+    if (stream != nullptr) (void)hipStreamSynchronize(stream);
+    for (auto& x : ev) {
+      if (x != nullptr) (void)hipEventDestroy(x);
+    }
+    if (stream != nullptr) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace pirg {
+namespace {
+
+// pir_params.cc:29-68 and the GPU path's own limits.
+Error DeriveServer(const pirg_params& P, pirg_server* S) {
+  if (P.num_entries <= 0) return Invalid("num_entries must be strictly positive.");
+  if (P.entry_size_multiple <= 0) {
+    return Invalid("entry_size_multiple must be strictly positive.");
+  }
+  if (P.interp_degree <= 0 || !IsPow2(static_cast<u64>(P.interp_degree))) {
+    return Invalid("interpolation_degree must be a power of two.");
+  }
+  if (P.num_entries % P.interp_degree != 0) {
+    return Invalid("interpolation_degree (" + std::to_string(P.interp_degree) +
+                   ") must divide num_entries (" + std::to_string(P.num_entries) +
+                   ").");
+  }
+  if (P.log_degree < 1 || P.log_degree > 12) {
+    return Invalid("log_degree must be in [1, 12] on the GPU path (2d-point FFT in LDS).");
+  }
+  const int d = 1 << P.log_degree;
+  if (P.interp_degree > 2 * d) {
+    return Invalid("interpolation_degree (" + std::to_string(P.interp_degree) +
+                   ") must be less than or equal to 2 * ring degree (" +
+                   std::to_string(2 * d) + ").");
+  }
+  if (P.log_q < 1 || P.log_q > 64 || P.log_q1 < 1 || P.log_q1 > 64 ||
+      P.log_q2 < 1 || P.log_q2 > 64) {
+    return Invalid("log moduli must be in [1, 64].");
+  }
+  if (P.db_bytes != 1 && P.db_bytes != 2) return Invalid("db_bytes must be 1 or 2.");
+  if (P.mat_bytes != 2 && P.mat_bytes != 4) return Invalid("mat_bytes must be 2 or 4.");
+  if (P.pack_num_digits <= 0 || P.eval_num_digits <= 0) {
+    return Invalid("log_digit and num_digits must be positive.");
+  }
+  S->P = P;
+  S->d = d;
+  S->log_d = P.log_degree;
+  S->N = 2 * d;
+  S->log_N = P.log_degree + 1;
+  S->F = FftDims{d, P.log_degree, 2 * d, P.log_degree + 1};
+  S->t = P.interp_degree;
+  S->ell = P.entry_size_multiple;
+  S->k = P.pack_num_digits;
+  S->e = P.eval_num_digits;
+  S->r = P.num_entries / P.interp_degree;
+  S->M = S->ell * S->t * d;
+  S->log_q = P.log_q;
+  S->qmask = P.log_q >= 64 ? ~0ULL : ((1ULL << P.log_q) - 1);
+  S->nl = (P.log_q + 7) / 8;
+  S->qpt = 16 / S->nl;
+  // FinalizeMatrixPack's InnerProduct (lwes_to_rlwe.cc:472-474).
+  Error er = MakeRules(S->k, d, P.pack_log_digit, P.log_q, ChunkFor(S->k, d), true,
+                       &S->fin_u, &S->fin_v);
+  if (!er.ok()) {
+    S->deferred = er;
+  } else {
+    S->fin_pairs = MakePairs(S->fin_u, S->fin_v);
+  }
+  if (S->t > 1 && S->deferred.ok()) {
+    // ExternalProduct (external_product.cc:100-111).
+    er = MakeRules(2 * S->e, d, P.eval_log_digit, P.log_q, ChunkFor(2 * S->e, d),
+                   false, &S->ext_u, &S->ext_v);
+    if (!er.ok()) {
+      S->deferred = er;
+    } else {
+      S->ext_pairs = MakePairs(S->ext_u, S->ext_v);
+    }
+  }
+  if (S->fin_pairs.count > 64 || S->ext_pairs.count > 64) {
+    return Invalid("too many FFT chunk pairs for the GPU path.");
+  }
+  return Ok();
+}
+
+// crypto/fft.h:71-80, computed with the same expression and libm.
+void Twiddles(int n, std::vector<double2>* fwd, std::vector<double2>* bwd) {
+  fwd->resize(n / 2);
+  bwd->resize(n / 2);
+  const double pi = std::acos(-1.0);
+  const size_t size = static_cast<size_t>(n);
+  for (size_t i = 0; i < size / 2; ++i) {
+    double angle = -2.0 * pi * i / size;
+    (*fwd)[i] = make_double2(std::cos(angle), std::sin(angle));
+    (*bwd)[i] = make_double2(std::cos(-angle), std::sin(-angle));
+  }
+}
+
+LimbLayout MakeLayout(int rows, int cols, int planes, u64 c_off) {
+  LimbLayout L;
+  L.rows = rows;
+  L.cols = cols;
+  L.strips = (rows + 15) / 16;
+  L.kblocks = (cols + 63) / 64;
+  L.planes = planes;
+  L.c_off = c_off;
+  return L;
+}
+
+// Uploads rows [0, rows) of a host matrix in chunks and converts them to
+// tiles. `rowbytes` is the size of one host row.
+template <typename Convert>
+Error UploadTiles(const void* host, size_t rowbytes, int rows, bool host_is_device,
+                  hipStream_t s, Convert conv) {
+  if (host_is_device) {
+    PIRG_HIP(conv(host, 0, rows));
+    PIRG_HIP(hipStreamSynchronize(s));
+    return Ok();
+  }
+  const size_t chunk_bytes = 256u << 20;
+  int rows_per = static_cast<int>(std::max<size_t>(16, chunk_bytes / std::max<size_t>(rowbytes, 1)));
+  rows_per = (rows_per / 16) * 16;
+  if (rows_per < 16) rows_per = 16;
+  DevBuf stage;
+  Error er = stage.alloc(static_cast<size_t>(std::min(rows_per, rows)) * rowbytes);
+  if (!er.ok()) return er;
+  for (int r0 = 0; r0 < rows; r0 += rows_per) {
+    const int nr = std::min(rows_per, rows - r0);
+    PIRG_HIP(hipMemcpyAsync(stage.p, static_cast<const char*>(host) + static_cast<size_t>(r0) * rowbytes,
+                            static_cast<size_t>(nr) * rowbytes, hipMemcpyHostToDevice, s));
+    PIRG_HIP(conv(stage.p, r0, nr));
+    PIRG_HIP(hipStreamSynchronize(s));
+  }
+  return Ok();
+}
+
+Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts,
+                    int db_cols, const std::vector<const void*>& pack_parts,
+                    const std::vector<const uint64_t*>& a_parts,
+                    const std::vector<const uint64_t*>& th_parts,
+                    const uint64_t* second_dim_a) {
+  const int d = S->d;
+  S->K = db_cols;
+  PIRG_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  for (auto& x : S->ev) PIRG_HIP(hipEventCreate(&x));
+  PIRG_HIP(ConfigureFftKernels(S->N));
+
+  // DB planes.
+  const int db_planes = S->P.db_bytes == 2 ? 2 : 1;
+  const u64 db_off = S->P.db_bytes == 2 ? 32896ULL : 128ULL;
+  S->dbL = MakeLayout(S->M, db_cols, db_planes, db_off);
+  Error er = S->db_tiles.alloc(static_cast<size_t>(S->dbL.strips) * S->dbL.kblocks *
+                               db_planes * kTileBytes);
+  if (!er.ok()) return er;
+  const int wpr = S->P.db_bytes == 2 ? db_cols / 4 : db_cols / 8;
+  const int rows_per_part = S->M / static_cast<int>(db_parts.size());
+  for (size_t part = 0; part < db_parts.size(); ++part) {
+    const int base = static_cast<int>(part) * rows_per_part;
+    er = UploadTiles(db_parts[part], static_cast<size_t>(wpr) * 8, rows_per_part, false,
+                     S->stream, [&](const void* src, int r0, int nr) {
+                       return LaunchDbToTiles(static_cast<const u64*>(src), S->P.db_bytes,
+                                              S->M, db_cols, base + r0, nr, S->dbL,
+                                              S->db_tiles.as<int8_t>(), S->stream);
+                     });
+    if (!er.ok()) return er;
+  }
+
+  // Pack planes: int32 -> 3 offset bytes + signed top byte; int16 -> 1 + 1.
+  const int pk_cols = S->k * d;
+  const int pk_planes = S->P.mat_bytes == 4 ? 4 : 2;
+  const u64 pk_off = S->P.mat_bytes == 4 ? 128ULL * (1 + 256 + 65536) : 128ULL;
+  S->pkL = MakeLayout(S->M, pk_cols, pk_planes, pk_off);
+  er = S->pk_tiles.alloc(static_cast<size_t>(S->pkL.strips) * S->pkL.kblocks *
+                         pk_planes * kTileBytes);
+  if (!er.ok()) return er;
+  const size_t pk_rowbytes = S->P.mat_bytes == 4 ? static_cast<size_t>(pk_cols) * 4
+                                                 : static_cast<size_t>(pk_cols) * 2;
+  const int pk_rows_per_part = S->M / static_cast<int>(pack_parts.size());
+  for (size_t part = 0; part < pack_parts.size(); ++part) {
+    const int base = static_cast<int>(part) * pk_rows_per_part;
+    er = UploadTiles(pack_parts[part], pk_rowbytes, pk_rows_per_part, false, S->stream,
+                     [&](const void* src, int r0, int nr) {
+                       return LaunchPackToTiles(src, S->P.mat_bytes, S->M, pk_cols,
+                                                base + r0, nr, S->pkL,
+                                                S->pk_tiles.as<int8_t>(), S->stream);
+                     });
+    if (!er.ok()) return er;
+  }
+  S->ks_db = LimbScanKSplit(S->dbL, 1);
+  S->ks_pk = LimbScanKSplit(S->pkL, 1);
+
+  // Small resident polynomials.
+  const int lt = S->ell * S->t;
+  er = S->a_tilde.alloc(static_cast<size_t>(lt) * d * 8);
+  if (!er.ok()) return er;
+  for (int oi = 0; oi < lt; ++oi) {
+    PIRG_HIP(hipMemcpy(S->a_tilde.as<u64>() + static_cast<size_t>(oi) * d, a_parts[oi],
+                       static_cast<size_t>(d) * 8, hipMemcpyHostToDevice));
+  }
+  DevBuf th;
+  er = th.alloc(static_cast<size_t>(lt) * S->k * d * 8);
+  if (!er.ok()) return er;
+  for (int oi = 0; oi < lt; ++oi) {
+    PIRG_HIP(hipMemcpy(th.as<u64>() + static_cast<size_t>(oi) * S->k * d, th_parts[oi],
+                       static_cast<size_t>(S->k) * d * 8, hipMemcpyHostToDevice));
+  }
+
+  // Twiddles.
+  std::vector<double2> fwd, bwd;
+  Twiddles(S->N, &fwd, &bwd);
+  er = S->tw_fwd.alloc(fwd.size() * sizeof(double2));
+  if (!er.ok()) return er;
+  er = S->tw_bwd.alloc(bwd.size() * sizeof(double2));
+  if (!er.ok()) return er;
+  PIRG_HIP(hipMemcpy(S->tw_fwd.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice));
+  PIRG_HIP(hipMemcpy(S->tw_bwd.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice));
+  er = S->margin.alloc(8);
+  if (!er.ok()) return er;
+  PIRG_HIP(hipMemset(S->margin.p, 0, 8));
+
+  // Resident spectra: t_vec_h chunks (finalize u operand) and second_dim_a
+  // chunks (external product v operand of the a-component).
+  if (S->deferred.ok()) {
+    er = S->t_spec.alloc(static_cast<size_t>(lt) * S->k * S->fin_u.num_chunks * S->N *
+                         sizeof(double2));
+    if (!er.ok()) return er;
+    PIRG_HIP(LaunchSpectra(th.as<u64>(), lt * S->k, 1, d, S->fin_u, S->F,
+                           S->tw_fwd.as<double2>(), S->t_spec.as<double2>(), S->stream));
+    if (S->t > 1) {
+      DevBuf sda;
+      er = sda.alloc(static_cast<size_t>(2 * S->e) * d * 8);
+      if (!er.ok()) return er;
+      PIRG_HIP(hipMemcpy(sda.p, second_dim_a, static_cast<size_t>(2 * S->e) * d * 8,
+                         hipMemcpyHostToDevice));
+      er = S->sda_spec.alloc(static_cast<size_t>(2 * S->e) * S->ext_v.num_chunks * S->N *
+                             sizeof(double2));
+      if (!er.ok()) return er;
+      PIRG_HIP(LaunchSpectra(sda.as<u64>(), 2 * S->e, 1, d, S->ext_v, S->F,
+                             S->tw_fwd.as<double2>(), S->sda_spec.as<double2>(), S->stream));
+      PIRG_HIP(hipStreamSynchronize(S->stream));
+    }
+    PIRG_HIP(hipStreamSynchronize(S->stream));
+  }
+  return Ok();
+}
+
+Error EnsureWorkspace(pirg_server* S, int cap) {
+  Workspace& W = S->ws;
+  if (W.cap >= cap) return Ok();
+  const size_t d = S->d, N = S->N;
+  const int lt = S->ell * S->t;
+  const int nt_db = (cap + S->qpt - 1) / S->qpt;
+  const size_t nq = cap;
+  struct Item {
+    DevBuf* b;
+    size_t bytes;
+  };
+  const int nu_f = S->fin_u.num_chunks, nv_f = S->fin_v.num_chunks;
+  const int nu_e = S->t > 1 ? S->ext_u.num_chunks : 0;
+  const int nv_e = S->t > 1 ? S->ext_v.num_chunks : 0;
+  std::vector<Item> items = {
+      {&W.fd, nq * S->K * 8},
+      {&W.sd, nq * 2 * S->e * d * 8},
+      {&W.pk, nq * 2 * S->k * d * 8},
+      {&W.bt_db, static_cast<size_t>(S->dbL.kblocks) * nt_db * kTileBytes},
+      {&W.bt_pk, static_cast<size_t>(S->pkL.kblocks) * nt_db * kTileBytes},
+      {&W.corr_db, nq * 8},
+      {&W.corr_pk, nq * 8},
+      {&W.db_partial, static_cast<size_t>(S->ks_db) * nq * S->M * 8},
+      {&W.pk_partial, static_cast<size_t>(S->ks_pk) * nq * S->M * 8},
+      {&W.yh_spec, nq * S->k * nv_f * N * sizeof(double2)},
+      {&W.sdb_spec, nq * 2 * S->e * nv_e * N * sizeof(double2)},
+      {&W.d_spec, nq * S->ell * 2 * S->e * nu_e * N * sizeof(double2)},
+      {&W.fin_partial, nq * lt * S->fin_pairs.count * d * 8},
+      {&W.ext_partial, nq * S->ell * 2 * std::max(1, S->ext_pairs.count) * d * 8},
+      {&W.ct_b, nq * lt * d * 8},
+      {&W.st_a, nq * S->ell * d * 8},
+      {&W.st_b, nq * S->ell * d * 8},
+      {&W.out, nq * S->ell * 2 * d * 8},
+      {&W.fin_tab, nq * lt * 3 * sizeof(void*)},
+      {&W.ext_tab, nq * S->ell * 2 * 3 * sizeof(void*)},
+  };
+  W.cap = 0;
+  for (auto& it : items) {
+    Error er = it.b->alloc(it.bytes);
+    if (!er.ok()) return er;
+  }
+  Error er = W.h_fd.alloc(nq * S->K * 8);
+  if (!er.ok()) return er;
+  er = W.h_sd.alloc(nq * 2 * S->e * d * 8);
+  if (!er.ok()) return er;
+  er = W.h_pk.alloc(nq * 2 * S->k * d * 8);
+  if (!er.ok()) return er;
+  er = W.h_out.alloc(nq * S->ell * 2 * d * 8);
+  if (!er.ok()) return er;
+
+  // IpKernel pointer tables (q-major, so any prefix of queries is valid).
+  std::vector<const void*> tab(nq * lt * 3);
+  const size_t n_fin = nq * lt;
+  for (size_t o = 0; o < n_fin; ++o) {
+    const size_t q = o / lt, oi = o % lt;
+    tab[o] = S->t_spec.as<double2>() + oi * S->k * nu_f * N;
+    tab[n_fin + o] = W.yh_spec.as<double2>() + q * S->k * nv_f * N;
+    tab[2 * n_fin + o] = W.fin_partial.as<u64>() + o * S->fin_pairs.count * d;
+  }
+  PIRG_HIP(hipMemcpy(W.fin_tab.p, tab.data(), tab.size() * sizeof(void*), hipMemcpyHostToDevice));
+  if (S->t > 1) {
+    const size_t n_ext = nq * S->ell * 2;
+    std::vector<const void*> et(n_ext * 3);
+    for (size_t o = 0; o < n_ext; ++o) {
+      const size_t qs = o / 2, which = o % 2, q = qs / S->ell;
+      et[o] = W.d_spec.as<double2>() + qs * 2 * S->e * nu_e * N;
+      et[n_ext + o] = which == 0 ? S->sda_spec.as<double2>()
+                                 : W.sdb_spec.as<double2>() + q * 2 * S->e * nv_e * N;
+      et[2 * n_ext + o] = W.ext_partial.as<u64>() + o * S->ext_pairs.count * d;
+    }
+    PIRG_HIP(hipMemcpy(W.ext_tab.p, et.data(), et.size() * sizeof(void*), hipMemcpyHostToDevice));
+  }
+  W.cap = cap;
+  return Ok();
+}
+
+// pir_server.cc:161-194 request validation, in the reference's order.
+Error ValidateRequest(const pirg_server* S, size_t fd_len, size_t sd_polys,
+                      size_t pk_polys) {
+  if (pk_polys != static_cast<size_t>(2 * S->k)) {
+    return Invalid("Packing key must contain 2 * num_digits polynomials.");
+  }
+  if (fd_len < static_cast<size_t>(S->r) || fd_len > static_cast<size_t>(S->K)) {
+    return Invalid("First dimension query size (" + std::to_string(fd_len) +
+                   ") is invalid for DB columns (" + std::to_string(S->K) + ").");
+  }
+  if (S->t > 1 && sd_polys != static_cast<size_t>(2 * S->e)) {
+    return Invalid("second_dimension_query size mismatch. Must be 2 * num_digits.");
+  }
+  if (!S->deferred.ok()) return S->deferred;
+  return Ok();
+}
+
+// Copies validated requests (reqs[idx[i]]) into the pinned staging buffers
+// and on to HBM.
+Error StageRequests(pirg_server* S, const pirg_request_view* reqs,
+                    const std::vector<int>& idx) {
+  Workspace& W = S->ws;
+  const size_t d = S->d;
+  const int nq = static_cast<int>(idx.size());
+  u64* hfd = W.h_fd.as<u64>();
+  u64* hsd = W.h_sd.as<u64>();
+  u64* hpk = W.h_pk.as<u64>();
+  for (int i = 0; i < nq; ++i) {
+    const pirg_request_view& R = reqs[idx[i]];
+    u64* f = hfd + static_cast<size_t>(i) * S->K;
+    std::memcpy(f, R.first_dim, R.first_dim_len * 8);
+    std::memset(f + R.first_dim_len, 0, (S->K - R.first_dim_len) * 8);
+    if (S->t > 1) {
+      std::memcpy(hsd + static_cast<size_t>(i) * 2 * S->e * d, R.second_dim,
+                  static_cast<size_t>(2 * S->e) * d * 8);
+    }
+    std::memcpy(hpk + static_cast<size_t>(i) * 2 * S->k * d, R.packing_key,
+                static_cast<size_t>(2 * S->k) * d * 8);
+  }
+  PIRG_HIP(hipMemcpyAsync(W.fd.p, hfd, static_cast<size_t>(nq) * S->K * 8,
+                          hipMemcpyHostToDevice, S->stream));
+  if (S->t > 1) {
+    PIRG_HIP(hipMemcpyAsync(W.sd.p, hsd, static_cast<size_t>(nq) * 2 * S->e * d * 8,
+                            hipMemcpyHostToDevice, S->stream));
+  }
+  PIRG_HIP(hipMemcpyAsync(W.pk.p, hpk, static_cast<size_t>(nq) * 2 * S->k * d * 8,
+                          hipMemcpyHostToDevice, S->stream));
+  return Ok();
+}
+
+// ProcessResponse for nq staged requests (pir_server.cc:207-305).
+Error RunBatch(pirg_server* S, int nq) {
+  Workspace& W = S->ws;
+  hipStream_t st = S->stream;
+  const int d = S->d;
+  const int lt = S->ell * S->t;
+  unsigned long long* margin = S->margin.as<unsigned long long>();
+  PIRG_HIP(hipEventRecord(S->ev[0], st));
+  // DB product (pir_server.cc:215-229).
+  PIRG_HIP(LaunchVectorsToTiles(W.fd.as<u64>(), nq, S->K, S->K, S->dbL.kblocks, S->nl,
+                                S->qpt, S->dbL.c_off, W.bt_db.as<int8_t>(),
+                                W.corr_db.as<u64>(), st));
+  PIRG_HIP(LaunchLimbScan(S->db_tiles.as<int8_t>(), S->dbL, W.bt_db.as<int8_t>(), nq,
+                          S->nl, S->qpt, S->log_q, S->ks_db, W.db_partial.as<u64>(), st));
+  PIRG_HIP(hipEventRecord(S->ev[1], st));
+  // Pack product (pir_server.cc:237-248): y = concat(y_vec_g).
+  PIRG_HIP(LaunchVectorsToTiles(W.pk.as<u64>(), nq, 2 * S->k * d, S->k * d,
+                                S->pkL.kblocks, S->nl, S->qpt, S->pkL.c_off,
+                                W.bt_pk.as<int8_t>(), W.corr_pk.as<u64>(), st));
+  PIRG_HIP(LaunchLimbScan(S->pk_tiles.as<int8_t>(), S->pkL, W.bt_pk.as<int8_t>(), nq,
+                          S->nl, S->qpt, S->log_q, S->ks_pk, W.pk_partial.as<u64>(), st));
+  // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279).
+  PIRG_HIP(LaunchSpectra(W.pk.as<u64>() + static_cast<size_t>(S->k) * d, nq * S->k, S->k,
+                         2LL * S->k * d, S->fin_v, S->F, S->tw_fwd.as<double2>(),
+                         W.yh_spec.as<double2>(), st));
+  {
+    const size_t n_fin_cap = static_cast<size_t>(W.cap) * lt;
+    void* const* tab = W.fin_tab.as<void* const>();
+    PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(tab),
+                      reinterpret_cast<const double2* const*>(tab + n_fin_cap),
+                      reinterpret_cast<u64* const*>(tab + 2 * n_fin_cap),
+                      nq * lt, S->k, S->fin_u.num_chunks, S->fin_v.num_chunks,
+                      S->fin_pairs, S->F, S->tw_bwd.as<double2>(), margin, st));
+  }
+  FinalizeArgs FA{};
+  FA.db_partial = W.db_partial.as<u64>();
+  FA.ksplit_db = S->ks_db;
+  FA.corr_db = W.corr_db.as<u64>();
+  FA.pk_partial = W.pk_partial.as<u64>();
+  FA.ksplit_pk = S->ks_pk;
+  FA.corr_pk = W.corr_pk.as<u64>();
+  FA.fin_partial = W.fin_partial.as<u64>();
+  FA.npairs = S->fin_pairs.count;
+  FA.a_tilde = S->a_tilde.as<u64>();
+  FA.ct_b = W.ct_b.as<u64>();
+  FA.st_a = W.st_a.as<u64>();
+  FA.st_b = W.st_b.as<u64>();
+  FA.out = W.out.as<u64>();
+  FA.nq = nq;
+  FA.M = S->M;
+  FA.d = d;
+  FA.t = S->t;
+  FA.ell = S->ell;
+  FA.log_q = S->log_q;
+  FA.log_q1 = S->P.log_q1;
+  FA.log_q2 = S->P.log_q2;
+  PIRG_HIP(LaunchFinalizeCombine(FA, st));
+  PIRG_HIP(hipEventRecord(S->ev[2], st));
+  // EvalPoly (pir_server.cc:126-150) and the modulus switch.
+  if (S->t > 1) {
+    PIRG_HIP(LaunchSpectra(W.sd.as<u64>(), nq * 2 * S->e, 2 * S->e, 2LL * S->e * d,
+                           S->ext_v, S->F, S->tw_fwd.as<double2>(),
+                           W.sdb_spec.as<double2>(), st));
+    const size_t n_ext_cap = static_cast<size_t>(W.cap) * S->ell * 2;
+    void* const* et = W.ext_tab.as<void* const>();
+    for (int i = S->t - 2; i >= 0; --i) {
+      DigitsArgs DA{};
+      DA.st_a = W.st_a.as<u64>();
+      DA.st_b = W.st_b.as<u64>();
+      DA.nq = nq;
+      DA.ell = S->ell;
+      DA.d = d;
+      DA.e = S->e;
+      DA.log_q = S->log_q;
+      DA.log_digit = S->P.eval_log_digit;
+      DA.rule = S->ext_u;
+      DA.dspec = W.d_spec.as<double2>();
+      PIRG_HIP(LaunchDigitsSpectra(DA, S->F, S->tw_fwd.as<double2>(), st));
+      PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(et),
+                        reinterpret_cast<const double2* const*>(et + n_ext_cap),
+                        reinterpret_cast<u64* const*>(et + 2 * n_ext_cap),
+                        nq * S->ell * 2, 2 * S->e, S->ext_u.num_chunks,
+                        S->ext_v.num_chunks, S->ext_pairs, S->F, S->tw_bwd.as<double2>(),
+                        margin, st));
+      ExtCombineArgs EA{};
+      EA.ext_partial = W.ext_partial.as<u64>();
+      EA.npairs = S->ext_pairs.count;
+      EA.a_tilde = S->a_tilde.as<u64>();
+      EA.ct_b = W.ct_b.as<u64>();
+      EA.st_a = W.st_a.as<u64>();
+      EA.st_b = W.st_b.as<u64>();
+      EA.out = W.out.as<u64>();
+      EA.nq = nq;
+      EA.ell = S->ell;
+      EA.t = S->t;
+      EA.d = d;
+      EA.step_i = i;
+      EA.log_q = S->log_q;
+      EA.log_q1 = S->P.log_q1;
+      EA.log_q2 = S->P.log_q2;
+      PIRG_HIP(LaunchExtCombine(EA, st));
+    }
+  }
+  PIRG_HIP(hipEventRecord(S->ev[3], st));
+  S->timing_pending = true;
+  return Ok();
+}
+
+void CollectTiming(pirg_server* S) {
+  if (!S->timing_pending) return;
+  if (hipEventSynchronize(S->ev[3]) != hipSuccess) return;
+  float a = 0, b = 0, c = 0;
+  (void)hipEventElapsedTime(&a, S->ev[0], S->ev[1]);
+  (void)hipEventElapsedTime(&b, S->ev[1], S->ev[2]);
+  (void)hipEventElapsedTime(&c, S->ev[2], S->ev[3]);
+  S->db_ms += a;
+  S->pack_ms += b;
+  S->eval_ms += c;
+  S->timing_pending = false;
+}
+
+Error FetchOutputs(pirg_server* S, int nq, uint64_t* const* outs,
+                   const std::vector<int>* idx) {
+  Workspace& W = S->ws;
+  const size_t per = static_cast<size_t>(S->ell) * 2 * S->d;
+  PIRG_HIP(hipMemcpyAsync(W.h_out.p, W.out.p, static_cast<size_t>(nq) * per * 8,
+                          hipMemcpyDeviceToHost, S->stream));
+  PIRG_HIP(hipStreamSynchronize(S->stream));
+  CollectTiming(S);
+  for (int i = 0; i < nq; ++i) {
+    const int dst = idx ? (*idx)[i] : i;
+    std::memcpy(outs[dst], W.h_out.as<u64>() + static_cast<size_t>(i) * per, per * 8);
+  }
+  return Ok();
+}
+
+// pir_server.cc:52-124 validation (Create).
+Error ValidateView(const pirg_server* S, const pirg_preprocessed_view* v) {
+  if (v->num_db_matrices != S->ell) return Invalid("db_matrices size mismatch.");
+  const int db_cols = v->db_cols[0];
+  if (db_cols < S->r) return Invalid("DB matrix must have at least N/t columns.");
+  for (int k = 0; k < S->ell; ++k) {
+    if (v->db_rows[k] != S->t * S->d) return Invalid("DB matrix must have td rows.");
+    if (v->db_cols[k] != db_cols) {
+      return Invalid("All DB matrices must have the same number of columns.");
+    }
+  }
+  if (v->num_outputs != S->ell) return Invalid("preprocessed_outputs size mismatch.");
+  for (int k = 0; k < S->ell; ++k) {
+    if (v->outputs_len[k] != S->t) return Invalid("Preprocessed outputs must be of length t.");
+  }
+  if (S->t > 1 && v->second_dim_a_len != 2 * S->e) {
+    return Invalid(
+        "second_dim_query_a must have length 2 * PolyEvalGadgetParams().num_digits.");
+  }
+  const int per = S->P.db_bytes == 2 ? 4 : 8;
+  if (db_cols % per != 0) {
+    return Invalid("DB columns must be a multiple of " + std::to_string(per) +
+                   " in the condensed layout.");
+  }
+  return Ok();
+}
+
+Error CreateFromParts(const pirg_params* params, const pirg_preprocessed_view* v,
+                      std::unique_ptr<pirg_server>* out) {
+  std::unique_ptr<pirg_server> S(new pirg_server());
+  Error er = DeriveServer(*params, S.get());
+  if (!er.ok()) return er;
+  er = ValidateView(S.get(), v);
+  if (!er.ok()) return er;
+  er = CheckDevice();
+  if (!er.ok()) return er;
+  std::vector<const uint64_t*> db(v->db_cdata, v->db_cdata + S->ell);
+  const int lt = S->ell * S->t;
+  std::vector<const void*> pk(v->pack, v->pack + lt);
+  std::vector<const uint64_t*> at(v->a_tilde, v->a_tilde + lt);
+  std::vector<const uint64_t*> th(v->t_vec_h, v->t_vec_h + lt);
+  er = BuildResident(S.get(), db, v->db_cols[0], pk, at, th, v->second_dim_a);
+  if (!er.ok()) return er;
+  *out = std::move(S);
+  return Ok();
+}
+
+thread_local std::string g_version;
+
+}  // namespace
+}  // namespace pirg
+
+namespace pirg {
+namespace {
+
+Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* vec,
+                     uint64_t* out, bool is_u16) {
+  Error er = CheckDevice();
+  if (!er.ok()) return er;
+  hipStream_t s = nullptr;
+  const int planes = is_u16 ? 2 : 4;
+  const u64 c_off = is_u16 ? 32896ULL : 128ULL * (1 + 256 + 65536);
+  LimbLayout L = MakeLayout(rows, cols, planes, c_off);
+  DevBuf tiles, src, v, bt, corr, partial;
+  const size_t wpr = is_u16 ? cols / 4 : cols / 2;
+  if ((is_u16 && cols % 4) || (!is_u16 && cols % 2)) return Invalid("Vector size mismatch.");
+  if ((er = tiles.alloc(static_cast<size_t>(L.strips) * L.kblocks * planes * kTileBytes)).code) return er;
+  if ((er = src.alloc(static_cast<size_t>(rows) * wpr * 8)).code) return er;
+  if ((er = v.alloc(static_cast<size_t>(cols) * 8)).code) return er;
+  const int nl = 8, qpt = 2;
+  if ((er = bt.alloc(static_cast<size_t>(L.kblocks) * kTileBytes)).code) return er;
+  if ((er = corr.alloc(8)).code) return er;
+  const int ks = LimbScanKSplit(L, 1);
+  if ((er = partial.alloc(static_cast<size_t>(ks) * rows * 8)).code) return er;
+  PIRG_HIP(hipMemcpy(src.p, cdata, static_cast<size_t>(rows) * wpr * 8, hipMemcpyHostToDevice));
+  PIRG_HIP(hipMemcpy(v.p, vec, static_cast<size_t>(cols) * 8, hipMemcpyHostToDevice));
+  if (is_u16) {
+    PIRG_HIP(LaunchDbToTiles(src.as<u64>(), 2, rows, cols, 0, rows, L, tiles.as<int8_t>(), s));
+  } else {
+    PIRG_HIP(LaunchPackToTiles(src.p, 4, rows, cols, 0, rows, L, tiles.as<int8_t>(), s));
+  }
+  PIRG_HIP(LaunchVectorsToTiles(v.as<u64>(), 1, cols, cols, L.kblocks, nl, qpt, c_off,
+                                bt.as<int8_t>(), corr.as<u64>(), s));
+  PIRG_HIP(LaunchLimbScan(tiles.as<int8_t>(), L, bt.as<int8_t>(), 1, nl, qpt, 64, ks,
+                          partial.as<u64>(), s));
+  PIRG_HIP(hipDeviceSynchronize());
+  std::vector<u64> part(static_cast<size_t>(ks) * rows);
+  u64 cc = 0;
+  PIRG_HIP(hipMemcpy(part.data(), partial.p, part.size() * 8, hipMemcpyDeviceToHost));
+  PIRG_HIP(hipMemcpy(&cc, corr.p, 8, hipMemcpyDeviceToHost));
+  for (int i = 0; i < rows; ++i) {
+    u64 acc = cc;
+    for (int k = 0; k < ks; ++k) acc += part[static_cast<size_t>(k) * rows + i];
+    out[i] = acc;
+  }
+  return Ok();
+}
+
+}  // namespace
+}  // namespace pirg
+
+extern "C" {
+
+int pirg_server_create(const pirg_params* params,
+                       const pirg_preprocessed_view* data, pirg_server** out,
+                       char* err, size_t errlen) {
+  std::unique_ptr<pirg_server> S;
+  Error er = CreateFromParts(params, data, &S);
+  if (!er.ok()) return Report(er, err, errlen);
+  *out = S.release();
+  return 0;
+}
+
+int pirg_server_create_flat(const pirg_params* params, const uint64_t* db_cdata,
+                            int db_cols, const void* pack,
+                            const uint64_t* a_tilde, const uint64_t* t_vec_h,
+                            const uint64_t* second_dim_a, pirg_server** out,
+                            char* err, size_t errlen) {
+  pirg_server probe;
+  Error er = DeriveServer(*params, &probe);
+  if (!er.ok()) return Report(er, err, errlen);
+  const int ell = probe.ell, t = probe.t, d = probe.d, k = probe.k;
+  const int lt = ell * t;
+  const size_t wpr = params->db_bytes == 2 ? db_cols / 4 : db_cols / 8;
+  const size_t pk_rowbytes = params->mat_bytes == 4 ? static_cast<size_t>(k) * d * 4
+                                                    : static_cast<size_t>(k) * d * 2;
+  std::vector<const uint64_t*> dbp(ell);
+  std::vector<int> rows(ell, t * d), cols(ell, db_cols), olen(ell, t);
+  for (int s = 0; s < ell; ++s) dbp[s] = db_cdata + static_cast<size_t>(s) * t * d * wpr;
+  std::vector<const void*> pkp(lt);
+  std::vector<const uint64_t*> atp(lt), thp(lt);
+  for (int oi = 0; oi < lt; ++oi) {
+    pkp[oi] = static_cast<const char*>(pack) + static_cast<size_t>(oi) * d * pk_rowbytes;
+    atp[oi] = a_tilde + static_cast<size_t>(oi) * d;
+    thp[oi] = t_vec_h + static_cast<size_t>(oi) * k * d;
+  }
+  pirg_preprocessed_view v{};
+  v.num_db_matrices = ell;
+  v.db_cdata = dbp.data();
+  v.db_rows = rows.data();
+  v.db_cols = cols.data();
+  v.num_outputs = ell;
+  v.outputs_len = olen.data();
+  v.pack = pkp.data();
+  v.a_tilde = atp.data();
+  v.t_vec_h = thp.data();
+  v.second_dim_a_len = (t > 1) ? 2 * probe.e : 0;
+  v.second_dim_a = second_dim_a;
+  return pirg_server_create(params, &v, out, err, errlen);
+}
+
+void pirg_server_destroy(pirg_server* server) { delete server; }
+
+int pirg_process(pirg_server* S, const uint64_t* first_dim, size_t first_dim_len,
+                 const uint64_t* second_dim, size_t second_dim_polys,
+                 const uint64_t* packing_key, size_t packing_key_polys,
+                 uint64_t* out, char* err, size_t errlen) {
+  pirg_request_view r{first_dim, first_dim_len, second_dim, second_dim_polys,
+                      packing_key, packing_key_polys};
+  int status = 0;
+  uint64_t* outs[1] = {out};
+  int rc = pirg_process_batch(S, 1, &r, outs, &status, err, errlen);
+  return rc != 0 ? rc : status;
+}
+
+int pirg_process_batch(pirg_server* S, int nreq, const pirg_request_view* reqs,
+                       uint64_t* const* outs, int* statuses, char* err,
+                       size_t errlen) {
+  if (S == nullptr || nreq < 0) return Report(Invalid("invalid server or batch size."), err, errlen);
+  std::vector<int> idx;
+  Error first_bad;
+  for (int i = 0; i < nreq; ++i) {
+    Error er = ValidateRequest(S, reqs[i].first_dim_len, reqs[i].second_dim_polys,
+                               reqs[i].packing_key_polys);
+    statuses[i] = er.code;
+    if (er.ok()) {
+      idx.push_back(i);
+    } else if (first_bad.ok()) {
+      first_bad = er;
+    }
+  }
+  if (!first_bad.ok()) Report(first_bad, err, errlen);
+  if (idx.empty()) return 0;
+  const int cap = static_cast<int>(idx.size());
+  Error er = EnsureWorkspace(S, cap);
+  if (!er.ok()) return Report(er, err, errlen);
+  er = StageRequests(S, reqs, idx);
+  if (!er.ok()) return Report(er, err, errlen);
+  er = RunBatch(S, cap);
+  if (!er.ok()) return Report(er, err, errlen);
+  er = FetchOutputs(S, cap, outs, &idx);
+  if (!er.ok()) return Report(er, err, errlen);
+  return 0;
+}
+
+int pirg_batch_stage(pirg_server* S, int nreq, const pirg_request_view* reqs,
+                     char* err, size_t errlen) {
+  std::vector<int> idx(nreq);
+  for (int i = 0; i < nreq; ++i) {
+    Error er = ValidateRequest(S, reqs[i].first_dim_len, reqs[i].second_dim_polys,
+                               reqs[i].packing_key_polys);
+    if (!er.ok()) return Report(er, err, errlen);
+    idx[i] = i;
+  }
+  Error er = EnsureWorkspace(S, nreq);
+  if (!er.ok()) return Report(er, err, errlen);
+  er = StageRequests(S, reqs, idx);
+  if (!er.ok()) return Report(er, err, errlen);
+  if (hipStreamSynchronize(S->stream) != hipSuccess) {
+    return Report(Internal("stream synchronize failed"), err, errlen);
+  }
+  S->staged = nreq;
+  return 0;
+}
+
+int pirg_batch_run(pirg_server* S, int nreq, char* err, size_t errlen) {
+  if (nreq <= 0 || nreq > S->staged) {
+    return Report(Invalid("batch size exceeds the staged requests."), err, errlen);
+  }
+  Error er = RunBatch(S, nreq);
+  if (!er.ok()) return Report(er, err, errlen);
+  if (hipStreamSynchronize(S->stream) != hipSuccess) {
+    return Report(Internal("stream synchronize failed"), err, errlen);
+  }
+  CollectTiming(S);
+  return 0;
+}
+
+int pirg_batch_fetch(pirg_server* S, int nreq, uint64_t* const* outs, char* err,
+                     size_t errlen) {
+  if (nreq <= 0 || nreq > S->staged) {
+    return Report(Invalid("batch size exceeds the staged requests."), err, errlen);
+  }
+  Error er = FetchOutputs(S, nreq, outs, nullptr);
+  if (!er.ok()) return Report(er, err, errlen);
+  return 0;
+}
+
+int pirg_server_stage_times(pirg_server* S, double* db_mult_ms, double* packing_ms,
+                            double* poly_eval_ms) {
+  CollectTiming(S);
+  if (db_mult_ms) *db_mult_ms = S->db_ms;
+  if (packing_ms) *packing_ms = S->pack_ms;
+  if (poly_eval_ms) *poly_eval_ms = S->eval_ms;
+  return 0;
+}
+
+double pirg_server_fft_margin(pirg_server* S, int reset) {
+  unsigned long long bits = 0;
+  (void)hipStreamSynchronize(S->stream);
+  (void)hipMemcpy(&bits, S->margin.p, 8, hipMemcpyDeviceToHost);
+  if (reset) (void)hipMemset(S->margin.p, 0, 8);
+  double v;
+  std::memcpy(&v, &bits, 8);
+  return v;
+}
+
+
+int pirg_matrix_multiply_u16(const uint64_t* cdata, int rows, int cols,
+                             const uint64_t* vec, uint64_t* out, char* err,
+                             size_t errlen) {
+  Error er = MatrixMultiply(cdata, rows, cols, vec, out, true);
+  return er.ok() ? 0 : Report(er, err, errlen);
+}
+
+int pirg_matrix_multiply_i32(const uint64_t* cdata, int rows, int cols,
+                             const uint64_t* vec, uint64_t* out, char* err,
+                             size_t errlen) {
+  Error er = MatrixMultiply(cdata, rows, cols, vec, out, false);
+  return er.ok() ? 0 : Report(er, err, errlen);
+}
+
+int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
+                           int u_bits, int v_bits, int chunk_bits,
+                           int u_is_signed, uint64_t* out, char* err,
+                           size_t errlen) {
+  auto run = [&]() -> Error {
+    if (n <= 0) return Invalid("Polynomial vectors cannot be empty.");
+    if (d < 2 || d > 4096 || !IsPow2(static_cast<u64>(d))) {
+      return Invalid("Polynomial lengths or context mismatch.");
+    }
+    ChunkRule ur{}, vr{};
+    Error er = MakeRules(n, d, u_bits, v_bits, chunk_bits, u_is_signed != 0, &ur, &vr);
+    if (!er.ok()) return er;
+    er = CheckDevice();
+    if (!er.ok()) return er;
+    const int N = 2 * d;
+    int log_d = 0;
+    while ((1 << log_d) < d) ++log_d;
+    FftDims F{d, log_d, N, log_d + 1};
+    PIRG_HIP(ConfigureFftKernels(N));
+    PairList pairs = MakePairs(ur, vr);
+    std::vector<double2> fwd, bwd;
+    Twiddles(N, &fwd, &bwd);
+    DevBuf du, dv, us, vs, part, res, twf, twb, tab;
+    if ((er = du.alloc(static_cast<size_t>(n) * d * 8)).code) return er;
+    if ((er = dv.alloc(static_cast<size_t>(n) * d * 8)).code) return er;
+    if ((er = us.alloc(static_cast<size_t>(n) * ur.num_chunks * N * sizeof(double2))).code) return er;
+    if ((er = vs.alloc(static_cast<size_t>(n) * vr.num_chunks * N * sizeof(double2))).code) return er;
+    if ((er = part.alloc(static_cast<size_t>(pairs.count) * d * 8)).code) return er;
+    if ((er = res.alloc(static_cast<size_t>(d) * 8)).code) return er;
+    if ((er = twf.alloc(fwd.size() * sizeof(double2))).code) return er;
+    if ((er = twb.alloc(bwd.size() * sizeof(double2))).code) return er;
+    if ((er = tab.alloc(3 * sizeof(void*))).code) return er;
+    PIRG_HIP(hipMemcpy(du.p, u, static_cast<size_t>(n) * d * 8, hipMemcpyHostToDevice));
+    PIRG_HIP(hipMemcpy(dv.p, v, static_cast<size_t>(n) * d * 8, hipMemcpyHostToDevice));
+    PIRG_HIP(hipMemcpy(twf.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice));
+    PIRG_HIP(hipMemcpy(twb.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice));
+    const void* ptrs[3] = {us.p, vs.p, part.p};
+    PIRG_HIP(hipMemcpy(tab.p, ptrs, sizeof(ptrs), hipMemcpyHostToDevice));
+    hipStream_t s = nullptr;
+    PIRG_HIP(LaunchSpectra(du.as<u64>(), n, 1, d, ur, F, twf.as<double2>(), us.as<double2>(), s));
+    PIRG_HIP(LaunchSpectra(dv.as<u64>(), n, 1, d, vr, F, twf.as<double2>(), vs.as<double2>(), s));
+    void* const* t = tab.as<void* const>();
+    PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(t),
+                      reinterpret_cast<const double2* const*>(t + 1),
+                      reinterpret_cast<u64* const*>(t + 2), 1, n,
+                      ur.num_chunks, vr.num_chunks, pairs, F, twb.as<double2>(), nullptr, s));
+    PIRG_HIP(LaunchSumPairs(part.as<u64>(), pairs.count, d, res.as<u64>(), s));
+    PIRG_HIP(hipDeviceSynchronize());
+    PIRG_HIP(hipMemcpy(out, res.p, static_cast<size_t>(d) * 8, hipMemcpyDeviceToHost));
+    return Ok();
+  };
+  Error er = run();
+  return er.ok() ? 0 : Report(er, err, errlen);
+}
+
+const char* pirg_version(void) {
+  g_version = "pirgpu 0.1 (gfx950 int8-limb MFMA + reference-exact FP64 FFT)";
+  return g_version.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+"""The C-ABI boundary without a GPU: both libraries load, every function the
+headers declare is exported, and PirServer::Create's validation
+(crypto/pir/server/pir_server.cc:62-95, pir_params.cc:35-62) returns the
+reference's status codes and messages before any device call."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import inspire_amd as ia
+import oracle as O
+from tests.helpers import PIR_SERVER_TEST_TYPES, pir_server_test_params, to_gpu_params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header, prefix):
+    text = open(header).read()
+    return sorted(set(re.findall(r"\b(%s\w+)\s*\(" % prefix, text)))
+
+
+def test_libpirgpu_exports_every_declared_symbol():
+    names = _declared(os.path.join(ROOT, "include", "pirgpu.h"), "pirg_")
+    assert len(names) >= 10
+    lib = ia.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_liboracle_exports_every_declared_symbol():
+    names = _declared(os.path.join(ROOT, "oracle", "pir_oracle.h"), "orc_")
+    lib = O.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_string():
+    assert "gfx950" in ia.version()
+
+
+def _view_args(P, **over):
+    s = O.sizes(P)
+    ell, t, d, k, e = s.ell, s.t, s.d, s.k, s.e
+    args = dict(
+        db_matrices=[np.zeros((t * d, s.K // 4), np.uint64) for _ in range(ell)],
+        db_cols=s.K,
+        pack=[[np.zeros((d, k * d // 2), np.uint64) for _ in range(t)] for _ in range(ell)],
+        a_tilde=[[np.zeros(d, np.uint64) for _ in range(t)] for _ in range(ell)],
+        t_vec_h=[[np.zeros((k, d), np.uint64) for _ in range(t)] for _ in range(ell)],
+        second_dim_a=np.zeros((2 * e, d), np.uint64),
+    )
+    args.update(over)
+    return args
+
+
+@pytest.mark.parametrize("case,msg", [
+    ("num_db", "db_matrices size mismatch."),
+    ("rows", "DB matrix must have td rows."),
+    ("cols", "DB matrix must have at least N/t columns."),
+    ("cols_differ", "All DB matrices must have the same number of columns."),
+    ("outputs", "preprocessed_outputs size mismatch."),
+    ("outputs_len", "Preprocessed outputs must be of length t."),
+    ("sda", "second_dim_query_a must have length 2 * PolyEvalGadgetParams().num_digits."),
+])
+def test_create_validation_messages(case, msg):
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62"], ell=2)
+    s = O.sizes(P)
+    a = _view_args(P)
+    kw = {}
+    if case == "num_db":
+        a["db_matrices"] = a["db_matrices"][:1]
+    elif case == "rows":
+        kw["db_rows"] = [1, s.t * s.d]
+    elif case == "cols":
+        kw["db_cols_each"] = [1, 1]
+    elif case == "cols_differ":
+        kw["db_cols_each"] = [s.K, s.K + 4]
+    elif case == "outputs":
+        for key in ("pack", "a_tilde", "t_vec_h"):
+            a[key] = a[key][:1]
+    elif case == "outputs_len":
+        for key in ("pack", "a_tilde", "t_vec_h"):
+            a[key][0] = a[key][0][:-1]
+    elif case == "sda":
+        a["second_dim_a"] = a["second_dim_a"][:-1]
+    if case == "cols":
+        a["db_cols"] = 1
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.create(to_gpu_params(P), a["db_matrices"], a["db_cols"], a["pack"],
+                            a["a_tilde"], a["t_vec_h"], a["second_dim_a"], **kw)
+    assert ei.value.code == ia.INVALID_ARGUMENT
+    assert ei.value.message == msg
+
+
+@pytest.mark.parametrize("over,msg", [
+    (dict(num_entries=0), "num_entries must be strictly positive."),
+    (dict(interp_degree=3), "interpolation_degree must be a power of two."),
+    (dict(num_entries=1000), "interpolation_degree (32) must divide num_entries (1000)."),
+    (dict(interp_degree=128),
+     "interpolation_degree (128) must be less than or equal to 2 * ring degree (64)."),
+])
+def test_create_params_validation(over, msg):
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62"])
+    gp = to_gpu_params(P)
+    gp = ia.PirParams(**{**{f: getattr(gp, f) for f in ia.PirParams.__dataclass_fields__},
+                         **over})
+    a = _view_args(P)
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.create(gp, a["db_matrices"], a["db_cols"], a["pack"], a["a_tilde"],
+                            a["t_vec_h"], a["second_dim_a"])
+    assert ei.value.code == ia.INVALID_ARGUMENT
+    assert ei.value.message == msg
+
+
+def test_create_without_device_fails_loudly():
+    """A valid create either succeeds (GPU present) or reports UNAVAILABLE;
+    there is no CPU fallback."""
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62Interp1"])
+    a = _view_args(P)
+    try:
+        srv = ia.PirServer.create(to_gpu_params(P), a["db_matrices"], a["db_cols"], a["pack"],
+                                  a["a_tilde"], a["t_vec_h"], a["second_dim_a"])
+        srv.close()
+    except ia.PirError as e:
+        assert e.code == ia.UNAVAILABLE
