@@ -153,6 +153,14 @@ int pirg_batch_fetch(pirg_server* server, int nreq, uint64_t* const* outs,
 int pirg_server_stage_times(pirg_server* server, double* db_mult_ms,
                             double* packing_ms, double* poly_eval_ms);
 
+/* HIP-event durations of the int8-limb scan kernels alone (DB product and
+ * pack product), summed over `launches` batches since the last reset. No
+ * reference counterpart: used to place the scans against the HBM and MFMA
+ * rooflines. */
+int pirg_server_scan_times(pirg_server* server, double* db_scan_ms,
+                           double* pack_scan_ms, long long* launches,
+                           int reset);
+
 /* Largest |x - round(x)| over every value the FFT products rounded since the
  * last reset, the precision margin of the double-precision products. */
 double pirg_server_fft_margin(pirg_server* server, int reset);
@@ -176,6 +184,16 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
                            int u_bits, int v_bits, int chunk_bits,
                            int u_is_signed, uint64_t* out, char* err,
                            size_t errlen);
+
+/* Streams `bytes` through a plain device-to-device copy kernel `repeats`
+ * times and reports the average kernel time. No reference counterpart: the
+ * known-size kernel that calibrates HBM byte counters and bandwidth. */
+int pirg_copy_probe(size_t bytes, int repeats, double* ms);
+
+/* The twiddle table the FFT kernels use for an n-point transform, as the
+ * interleaved (re, im) pairs of FftPlan's twiddles_forward_ /
+ * twiddles_backward_ (crypto/fft.h:71-80); n/2 pairs each. Host only. */
+int pirg_fft_twiddles(int n, double* fwd, double* bwd);
 
 /* Version / device info string (for logs). */
 const char* pirg_version(void);

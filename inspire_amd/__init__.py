@@ -364,6 +364,13 @@ class PirServer:
         lib().pirg_server_stage_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return a.value, b.value, c.value
 
+    def scan_times_ms(self, reset: bool = False):
+        """(DB-scan ms, pack-scan ms, batches) summed since the last reset."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        lib().pirg_server_scan_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n),
+                                     int(reset))
+        return a.value, b.value, n.value
+
     def fft_margin(self, reset: bool = False) -> float:
         """Largest |x - round(x)| the FFT products have rounded so far."""
         return lib().pirg_server_fft_margin(self._h, int(reset))
@@ -409,6 +416,15 @@ def inner_product_fft(u, v, u_bits=64, v_bits=64, chunk_bits=20, u_signed=False)
                                       out.ctypes.data_as(_u64p), err, ctypes.c_size_t(len(err)))
     _check(rc, err)
     return out
+
+
+def copy_probe(nbytes: int, repeats: int = 10) -> float:
+    """Average ms of a plain nbytes device copy kernel (counter calibration)."""
+    ms = ctypes.c_double()
+    rc = lib().pirg_copy_probe(ctypes.c_size_t(nbytes), repeats, ctypes.byref(ms))
+    if rc != 0:
+        raise PirError(rc, "copy probe failed")
+    return ms.value
 
 
 def version() -> str:

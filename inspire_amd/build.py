@@ -21,6 +21,8 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libpirgpu.so")
 
 SOURCES = ["limb_kernels.hip", "fft_kernels.hip", "server.cpp"]
+# Host-only sources built with g++ (see twiddles.cpp for why).
+GXX_SOURCES = ["twiddles.cpp"]
 ARCH = os.environ.get("PIRG_OFFLOAD_ARCH", "gfx950")
 
 COMMON_FLAGS = [
@@ -60,6 +62,13 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(BUILD_DIR, src.replace(".", "_") + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
         cmd = [cc, *COMMON_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    for src in GXX_SOURCES:
+        obj = os.path.join(BUILD_DIR, src.replace(".", "_") + ".o")
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

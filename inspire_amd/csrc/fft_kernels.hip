@@ -31,32 +31,68 @@ __device__ __forceinline__ void Butterfly(double2& x, double2& y, const double2 
   y = make_double2(ux - vr, uy - vi);
 }
 
-// crypto/fft.h:117-128 on a bit-reversed LDS buffer, two stages per pass.
+// Twiddles of the radix-4 groups a thread owns in the pass starting at stage
+// s: w1 = tw[j * step1] (stage s), w2 = tw[j * step2], w3 = tw[(j + h) * step2]
+// (stage s + 1).
+constexpr int kMaxGroups = 4;  // n / 4 <= kFftThreads * kMaxGroups (n <= 8192)
+
+__device__ __forceinline__ void LoadPassTwiddles(const double2* __restrict__ tw,
+                                                 int log_n, int s, double2* w1,
+                                                 double2* w2, double2* w3) {
+  const int n = 1 << log_n;
+  const int h = 1 << (s - 1);
+  const int step1 = n >> s;
+  const int step2 = n >> (s + 1);
+#pragma unroll
+  for (int m = 0; m < kMaxGroups; ++m) {
+    const int g = threadIdx.x + m * blockDim.x;
+    if (g < n / 4) {
+      const int j = g & (h - 1);
+      w1[m] = tw[j * step1];
+      w2[m] = tw[j * step2];
+      w3[m] = tw[(j + h) * step2];
+    }
+  }
+}
+
+// crypto/fft.h:117-128 on a bit-reversed LDS buffer, two stages per pass; the
+// next pass's twiddles are loaded while the current pass runs.
 __device__ void FftLds(double2* buf, const double2* __restrict__ tw, int log_n) {
   const int n = 1 << log_n;
   int s = 1;
+  double2 w1[kMaxGroups], w2[kMaxGroups], w3[kMaxGroups];
+  double2 n1[kMaxGroups], n2[kMaxGroups], n3[kMaxGroups];
+  if (log_n >= 2) LoadPassTwiddles(tw, log_n, 1, w1, w2, w3);
   for (; s + 1 <= log_n; s += 2) {
     const int h = 1 << (s - 1);
-    const int step1 = n >> s;
-    const int step2 = n >> (s + 1);
-    for (int g = threadIdx.x; g < n / 4; g += blockDim.x) {
-      const int j = g & (h - 1);
-      const int e0 = ((g >> (s - 1)) << (s + 1)) + j;
-      double2 x0 = buf[e0];
-      double2 x1 = buf[e0 + h];
-      double2 x2 = buf[e0 + 2 * h];
-      double2 x3 = buf[e0 + 3 * h];
-      const double2 w1 = tw[j * step1];
-      Butterfly(x0, x1, w1);
-      Butterfly(x2, x3, w1);
-      Butterfly(x0, x2, tw[j * step2]);
-      Butterfly(x1, x3, tw[(j + h) * step2]);
-      buf[e0] = x0;
-      buf[e0 + h] = x1;
-      buf[e0 + 2 * h] = x2;
-      buf[e0 + 3 * h] = x3;
+    if (s + 3 <= log_n) LoadPassTwiddles(tw, log_n, s + 2, n1, n2, n3);
+#pragma unroll
+    for (int m = 0; m < kMaxGroups; ++m) {
+      const int g = threadIdx.x + m * blockDim.x;
+      if (g < n / 4) {
+        const int j = g & (h - 1);
+        const int e0 = ((g >> (s - 1)) << (s + 1)) + j;
+        double2 x0 = buf[e0];
+        double2 x1 = buf[e0 + h];
+        double2 x2 = buf[e0 + 2 * h];
+        double2 x3 = buf[e0 + 3 * h];
+        Butterfly(x0, x1, w1[m]);
+        Butterfly(x2, x3, w1[m]);
+        Butterfly(x0, x2, w2[m]);
+        Butterfly(x1, x3, w3[m]);
+        buf[e0] = x0;
+        buf[e0 + h] = x1;
+        buf[e0 + 2 * h] = x2;
+        buf[e0 + 3 * h] = x3;
+      }
     }
     __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kMaxGroups; ++m) {
+      w1[m] = n1[m];
+      w2[m] = n2[m];
+      w3[m] = n3[m];
+    }
   }
   if (s == log_n) {
     const int h = 1 << (s - 1);
@@ -91,9 +127,22 @@ __global__ __launch_bounds__(kFftThreads) void SpectraKernel(
   const int c = job % rule.num_chunks;
   const u64* poly = src + (long long)(p / per_group) * group_stride +
                     (long long)(p % per_group) * d;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const double v = (j < d) ? ChunkValue(rule, poly[j], c) : 0.0;
-    buf[BitRev(j, log_n)] = make_double2(v, 0.0);
+  constexpr int kElems = 8;
+  for (int j0 = threadIdx.x; j0 < n; j0 += blockDim.x * kElems) {
+    u64 x[kElems];
+#pragma unroll
+    for (int e = 0; e < kElems; ++e) {
+      const int j = j0 + e * blockDim.x;
+      x[e] = (j < d) ? poly[j] : 0ULL;
+    }
+#pragma unroll
+    for (int e = 0; e < kElems; ++e) {
+      const int j = j0 + e * blockDim.x;
+      if (j < n) {
+        const double v = (j < d) ? ChunkValue(rule, x[e], c) : 0.0;
+        buf[BitRev(j, log_n)] = make_double2(v, 0.0);
+      }
+    }
   }
   __syncthreads();
   FftLds(buf, tw, log_n);
@@ -121,17 +170,39 @@ __global__ __launch_bounds__(kFftThreads) void IpKernel(
   const int a = pairs.a[pi], b = pairs.b[pi], sh = pairs.shift[pi];
   const double2* U = jobs.uptr[o];
   const double2* V = jobs.vptr[o];
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double ax = 0.0, ay = 0.0;
-    for (int i = 0; i < nterms; ++i) {
-      const double2 uu = U[((long long)i * nu + a) * n + j];
-      const double2 vv = V[((long long)i * nv + b) * n + j];
-      const double pr = uu.x * vv.x - uu.y * vv.y;
-      const double pim = uu.x * vv.y + uu.y * vv.x;
-      ax = ax + pr;
-      ay = ay + pim;
+  // kElems independent elements per thread keep that many load chains in
+  // flight; the term order of each element's sum is the reference's.
+  constexpr int kElems = 8;
+  for (int j0 = threadIdx.x; j0 < n; j0 += blockDim.x * kElems) {
+    double ax[kElems], ay[kElems];
+#pragma unroll
+    for (int e = 0; e < kElems; ++e) {
+      ax[e] = 0.0;
+      ay[e] = 0.0;
     }
-    buf[BitRev(j, log_n)] = make_double2(ax, ay);
+    for (int i = 0; i < nterms; ++i) {
+      const double2* Ui = U + ((long long)i * nu + a) * n;
+      const double2* Vi = V + ((long long)i * nv + b) * n;
+      double2 uu[kElems], vv[kElems];
+#pragma unroll
+      for (int e = 0; e < kElems; ++e) {
+        const int j = j0 + e * blockDim.x;
+        uu[e] = (j < n) ? Ui[j] : make_double2(0.0, 0.0);
+        vv[e] = (j < n) ? Vi[j] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int e = 0; e < kElems; ++e) {
+        const double pr = uu[e].x * vv[e].x - uu[e].y * vv[e].y;
+        const double pim = uu[e].x * vv[e].y + uu[e].y * vv[e].x;
+        ax[e] = ax[e] + pr;
+        ay[e] = ay[e] + pim;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < kElems; ++e) {
+      const int j = j0 + e * blockDim.x;
+      if (j < n) buf[BitRev(j, log_n)] = make_double2(ax[e], ay[e]);
+    }
   }
   __syncthreads();
   FftLds(buf, tw, log_n);

@@ -21,7 +21,7 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;        // waves per workgroup in the scan kernel
-constexpr int kMaxKbPerSplit = 1024;  // 65536 columns
+constexpr int kMaxColsPerSplitNP1 = 65536;  // 2^16 / NP columns keep int32 sums exact
 
 __device__ inline int8_t OffsetByte(unsigned x) {
   return static_cast<int8_t>(static_cast<int>(x & 255u) - 128);
@@ -133,10 +133,13 @@ __device__ inline int BalancedDigit(u64 y, int c) {
   return dig;
 }
 
-// One thread per (kblock, ntile, lane).
+// B operand tiles. For k-block kb, 16-column tile `tile` and plane p, column
+// j = (query ql, shift s) holds digit (s - p) of the query (zero when s < p),
+// so that plane p's product lands in the accumulator of total shift 8s.
+// Layout [kb][tile][plane][64 lanes][16 bytes]. One thread per (kb, tile, lane).
 __global__ void VectorsToTilesKernel(const u64* __restrict__ vec, int nq,
                                      int stride, int len, int kblocks, int nl,
-                                     int qpt, int nt,
+                                     int qpt, int nt, int planes,
                                      int8_t* __restrict__ btiles) {
   const long long total = (long long)kblocks * nt * 64;
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
@@ -146,18 +149,24 @@ __global__ void VectorsToTilesKernel(const u64* __restrict__ vec, int nq,
     const int tile = static_cast<int>(t % nt);
     const int kb = static_cast<int>(t / nt);
     const int j = lane & 15;
-    const int ql = j / nl, c = j % nl;
+    const int ql = j / nl, sh = j % nl;
     const int q = tile * qpt + ql;
     const bool valid = ql < qpt && q < nq;
     const int kbase = kb * 64 + 16 * (lane >> 4);
-    int8_t out[16];
+    u64 ys[16];
     for (int b = 0; b < 16; ++b) {
       const int k = kbase + b;
-      const u64 y = (valid && k < len) ? vec[(long long)q * stride + k] : 0ULL;
-      out[b] = static_cast<int8_t>(BalancedDigit(y, c));
+      ys[b] = (valid && k < len) ? vec[(long long)q * stride + k] : 0ULL;
     }
-    int8_t* dst = btiles + ((long long)(kb * nt + tile) * 64 + lane) * 16;
-    *reinterpret_cast<int4*>(dst) = *reinterpret_cast<const int4*>(out);
+    for (int p = 0; p < planes; ++p) {
+      int8_t out[16];
+      const int c = sh - p;
+      for (int b = 0; b < 16; ++b) {
+        out[b] = (c >= 0) ? static_cast<int8_t>(BalancedDigit(ys[b], c)) : int8_t{0};
+      }
+      int8_t* dst = btiles + ((((long long)kb * nt + tile) * planes + p) * 64 + lane) * 16;
+      *reinterpret_cast<int4*>(dst) = *reinterpret_cast<const int4*>(out);
+    }
   }
 }
 
@@ -177,14 +186,22 @@ __global__ void VectorSumKernel(const u64* __restrict__ vec, int stride,
   if (threadIdx.x == 0) corr[q] = c_off * part[0];
 }
 
-// The scan: each wave owns R consecutive 16-row strips and a k-range; per
-// 64-column block it loads R*NP A tiles (one 16-byte load per lane each) and T
-// B tiles and issues R*NP*T MFMAs.
+// The scan. A workgroup of 4 waves owns 4*R consecutive 16-row strips and a
+// k-range; wave w owns strips [strip0 + w*R, +R). Per 64-column block:
+//  * the T*NP B tiles of the block are staged in LDS (double buffered) by the
+//    whole workgroup and shared by its 4 waves;
+//  * each wave holds its R*NP A tiles in registers, loaded one block ahead;
+//  * plane p of strip r times B tile (t, p) accumulates into acc[r][t]: the
+//    B tile of plane p carries the query digits shifted by p, so one int32
+//    accumulator per (row, query, total shift) collects every (plane, digit)
+//    pair of that shift.
+// |acc| <= NP * 2^14 * columns, so a k-range of 2^16 / NP columns stays exact.
 template <int NP, int R, int T>
 __global__ __launch_bounds__(256) void LimbScanKernel(
     const int8_t* __restrict__ tiles, int strips, int rows, int kblocks,
     int kb_per_split, const int8_t* __restrict__ btiles, int nt, int nq,
     int nl, int qpt, int out_bits, u64* __restrict__ partial) {
+  __shared__ v4i bsh[2][T * NP * 64];
   __shared__ u64 red[kWaves][16][16];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -193,20 +210,18 @@ __global__ __launch_bounds__(256) void LimbScanKernel(
   const int nt0 = blockIdx.z * T;
   const int kb0 = ks * kb_per_split;
   const int kb1 = min(kblocks, kb0 + kb_per_split);
+  const v4i* A = reinterpret_cast<const v4i*>(tiles);
+  const v4i* Bt = reinterpret_cast<const v4i*>(btiles);
+  constexpr int kBVecs = T * NP * 64;             // v4i per staged block
+  constexpr int kBPerThread = (kBVecs + 255) / 256;
 
-  v4i acc[R][NP][T];
+  v4i acc[R][T];
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
-    for (int p = 0; p < NP; ++p)
-#pragma unroll
-      for (int tt = 0; tt < T; ++tt) acc[r][p][tt] = v4i{0, 0, 0, 0};
+    for (int tt = 0; tt < T; ++tt) acc[r][tt] = v4i{0, 0, 0, 0};
 
-  const v4i* A = reinterpret_cast<const v4i*>(tiles);
-  const v4i* Bt = reinterpret_cast<const v4i*>(btiles);
-  for (int kb = kb0; kb < kb1; ++kb) {
-    v4i a[R][NP];
-    v4i b[T];
+  auto load_a = [&](int kb, v4i (&a)[R][NP]) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int strip = strip0 + r;
@@ -217,52 +232,90 @@ __global__ __launch_bounds__(256) void LimbScanKernel(
                       : v4i{0, 0, 0, 0};
       }
     }
+  };
+  // B tiles of block kb for tiles [nt0, nt0 + T): contiguous in btiles when
+  // all T tiles exist; missing tiles read as zero.
+  auto load_b = [&](int kb, v4i (&tmp)[kBPerThread]) {
 #pragma unroll
-    for (int tt = 0; tt < T; ++tt) {
-      b[tt] = (nt0 + tt < nt) ? Bt[((long long)kb * nt + nt0 + tt) * 64 + lane]
-                              : v4i{0, 0, 0, 0};
+    for (int i = 0; i < kBPerThread; ++i) {
+      const int v = threadIdx.x + i * 256;
+      const int tt = v / (NP * 64);
+      v4i x = v4i{0, 0, 0, 0};
+      if (v < kBVecs && nt0 + tt < nt) {
+        x = Bt[((long long)kb * nt + nt0) * NP * 64 + v];
+      }
+      tmp[i] = x;
     }
+  };
+  auto store_b = [&](int buf, const v4i (&tmp)[kBPerThread]) {
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+    for (int i = 0; i < kBPerThread; ++i) {
+      const int v = threadIdx.x + i * 256;
+      if (v < kBVecs) bsh[buf][v] = tmp[i];
+    }
+  };
+
+  if (kb0 < kb1) {
+    v4i a_cur[R][NP], a_next[R][NP];
+    v4i btmp[kBPerThread];
+    load_a(kb0, a_cur);
+    load_b(kb0, btmp);
+    store_b(0, btmp);
+    __syncthreads();
+    for (int kb = kb0; kb < kb1; ++kb) {
+      const int buf = (kb - kb0) & 1;
+      const bool more = kb + 1 < kb1;
+      if (more) {
+        load_a(kb + 1, a_next);
+        load_b(kb + 1, btmp);
+      }
 #pragma unroll
-      for (int p = 0; p < NP; ++p)
+      for (int tt = 0; tt < T; ++tt) {
 #pragma unroll
-        for (int tt = 0; tt < T; ++tt)
-          acc[r][p][tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-              a[r][p], b[tt], acc[r][p][tt], 0, 0, 0);
+        for (int p = 0; p < NP; ++p) {
+          const v4i b = bsh[buf][(tt * NP + p) * 64 + lane];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            acc[r][tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a_cur[r][p], b, acc[r][tt],
+                                                               0, 0, 0);
+          }
+        }
+      }
+      if (more) store_b(buf ^ 1, btmp);
+      __syncthreads();
+      if (more) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+          for (int p = 0; p < NP; ++p) a_cur[r][p] = a_next[r][p];
+      }
+    }
   }
 
   // Epilogue. MFMA 16x16 output layout: lane l, element v holds
-  // C[row = 4*(l/16) + v][col = l % 16].
+  // C[row = 4*(l/16) + v][col = l % 16]; col = (query, shift).
   const int col = lane & 15;
-  const int c = col % nl;
-  const bool col_ok = col < qpt * nl;
+  const int sh = 8 * (col % nl);
+  const bool col_ok = col < qpt * nl && sh < out_bits;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
 #pragma unroll
     for (int tt = 0; tt < T; ++tt) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        u64 val = 0;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          const int sh = 8 * (p + c);
-          if (col_ok && sh < out_bits) {
-            val += static_cast<u64>(static_cast<i64>(acc[r][p][tt][v])) << sh;
-          }
-        }
+        const u64 val = col_ok ? static_cast<u64>(static_cast<i64>(acc[r][tt][v])) << sh : 0ULL;
         red[wave][4 * (lane >> 4) + v][col] = val;
       }
       __syncthreads();
       if (lane < 16 * qpt) {
         const int row = lane & 15;
         const int ql = lane >> 4;
-        u64 s = 0;
-        for (int cc = 0; cc < nl; ++cc) s += red[wave][row][ql * nl + cc];
+        u64 sum = 0;
+        for (int cc = 0; cc < nl; ++cc) sum += red[wave][row][ql * nl + cc];
         const int q = (nt0 + tt) * qpt + ql;
         const long long grow = (long long)(strip0 + r) * 16 + row;
         if (q < nq && grow < rows) {
-          partial[((long long)ks * nq + q) * rows + grow] = s;
+          partial[((long long)ks * nq + q) * rows + grow] = sum;
         }
       }
       __syncthreads();
@@ -274,7 +327,8 @@ int PickT(int nt) {
   if (nt <= 1) return 1;
   if (nt <= 2) return 2;
   if (nt <= 4) return 4;
-  return 8;
+  if (nt <= 8) return 8;
+  return 16;
 }
 
 int GridBlocks(long long total, int threads) {
@@ -296,29 +350,45 @@ hipError_t LaunchScanT(const int8_t* tiles, const LimbLayout& L,
   return hipGetLastError();
 }
 
+// R strips per wave; R * T * 4 accumulator registers stay <= 128 so the
+// wave fits its registers without spilling.
 template <int NP>
 hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L,
                         const int8_t* btiles, int nt, int nq, int nl, int qpt,
                         int out_bits, int ksplit, u64* partial, hipStream_t s) {
-  const int T = PickT(nt);
-  constexpr int kBudget = 32;  // R * NP * T <= 32 keeps accumulators <= 128 VGPRs
-  switch (T) {
+  switch (PickT(nt)) {
     case 1:
-      return LaunchScanT<NP, (kBudget / NP >= 4 ? 4 : kBudget / NP), 1>(
-          tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
+      return LaunchScanT<NP, 4, 1>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
     case 2:
-      return LaunchScanT<NP, (kBudget / (NP * 2) >= 4 ? 4 : kBudget / (NP * 2)), 2>(
-          tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
+      return LaunchScanT<NP, 4, 2>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
     case 4:
-      return LaunchScanT<NP, (kBudget / (NP * 4) >= 4 ? 4 : kBudget / (NP * 4)), 4>(
-          tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
+      return LaunchScanT<NP, 4, 4>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
+    case 8:
+      return LaunchScanT<NP, 4, 8>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
     default:
-      return LaunchScanT<NP, (kBudget / (NP * 8) >= 1 ? kBudget / (NP * 8) : 1), 8>(
-          tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
+      return LaunchScanT<NP, 2, 16>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit,
+                                    partial, s);
   }
 }
 
 }  // namespace
+
+// Plain streaming copy (one 16-byte load and store per element); the
+// reference kernel for calibrating HBM byte counters.
+__global__ void CopyProbeKernel(const int4* __restrict__ src, int4* __restrict__ dst,
+                                long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    dst[i] = src[i];
+  }
+}
+
+hipError_t LaunchCopyProbe(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const long long n = static_cast<long long>(bytes / 16);
+  hipLaunchKernelGGL(CopyProbeKernel, dim3(4096), dim3(256), 0, s,
+                     static_cast<const int4*>(src), static_cast<int4*>(dst), n);
+  return hipGetLastError();
+}
 
 hipError_t LaunchDbToTiles(const u64* cdata, int db_bytes, int rows, int cols,
                            int row0, int nrows, const LimbLayout& L,
@@ -344,13 +414,13 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
 }
 
 hipError_t LaunchVectorsToTiles(const u64* vec, int nq, int stride, int len,
-                                int kblocks, int nl, int qpt, u64 c_off,
+                                int kblocks, int nl, int qpt, int planes, u64 c_off,
                                 int8_t* btiles, u64* corr, hipStream_t s) {
   const int nt = (nq + qpt - 1) / qpt;
   const long long total = (long long)kblocks * nt * 64;
   hipLaunchKernelGGL(VectorsToTilesKernel, dim3(GridBlocks(total, 256)),
                      dim3(256), 0, s, vec, nq, stride, len, kblocks, nl, qpt,
-                     nt, btiles);
+                     nt, planes, btiles);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (corr != nullptr) {
@@ -363,10 +433,11 @@ hipError_t LaunchVectorsToTiles(const u64* vec, int nq, int stride, int len,
 
 int LimbScanKSplit(const LimbLayout& L, int nq) {
   (void)nq;
-  const int R = 4;  // upper bound of strips per wave
+  const int R = 4;  // strips per wave of the common configurations
   const int wgs = (L.strips + kWaves * R - 1) / (kWaves * R);
-  int ksplit = (2048 + wgs - 1) / wgs;
-  const int min_split = (L.kblocks + kMaxKbPerSplit - 1) / kMaxKbPerSplit;
+  int ksplit = (1024 + wgs - 1) / wgs;
+  const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
+  const int min_split = (L.kblocks + max_kb - 1) / max_kb;
   if (ksplit < min_split) ksplit = min_split;
   if (ksplit > L.kblocks) ksplit = L.kblocks;
   if (ksplit < 1) ksplit = 1;

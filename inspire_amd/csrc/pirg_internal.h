@@ -96,10 +96,10 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
                              const LimbLayout& L, int8_t* tiles,
                              hipStream_t s);
 // Builds B-operand tiles for nq vectors (vec + q*stride, length len each,
-// zero beyond len) with nl digits, qpt vectors per 16-column tile, and
-// corr[q] = c_off * sum(vec_q) (mod 2^64).
+// zero beyond len) with nl digits, qpt vectors per 16-column tile and one
+// digit-shifted copy per plane, and corr[q] = c_off * sum(vec_q) (mod 2^64).
 hipError_t LaunchVectorsToTiles(const u64* vec, int nq, int stride, int len,
-                                int kblocks, int nl, int qpt, u64 c_off,
+                                int kblocks, int nl, int qpt, int planes, u64 c_off,
                                 int8_t* btiles, u64* corr, hipStream_t s);
 // partial[ks][q][row] = sum over the ks-th k-range of the limb product,
 // combined over (plane, digit) pairs with 8(p+c) < out_bits.
@@ -108,6 +108,7 @@ hipError_t LaunchLimbScan(const int8_t* tiles, const LimbLayout& L,
                           int out_bits, int ksplit, u64* partial,
                           hipStream_t s);
 int LimbScanKSplit(const LimbLayout& L, int nq);
+hipError_t LaunchCopyProbe(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // FFT-side launchers (fft_kernels.hip). N = 2d, twiddles = N/2 complex.

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -17,6 +18,8 @@
 
 #include "../../include/pirgpu.h"
 #include "pirg_internal.h"
+
+extern "C" void pirg_internal_twiddles(int n, double* fwd, double* bwd);
 
 namespace pirg {
 
@@ -182,7 +185,10 @@ struct pirg_server {
   DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};  // DB / pack scan
   double db_ms = 0, pack_ms = 0, eval_ms = 0;
+  double db_scan_ms = 0, pk_scan_ms = 0;
+  long long scans = 0;
   bool timing_pending = false;
   Workspace ws;
   int staged = 0;
@@ -190,6 +196,9 @@ struct pirg_server {
   ~pirg_server() {
     if (stream != nullptr) (void)hipStreamSynchronize(stream);
     for (auto& x : ev) {
+      if (x != nullptr) (void)hipEventDestroy(x);
+    }
+    for (auto& x : kev) {
       if (x != nullptr) (void)hipEventDestroy(x);
     }
     if (stream != nullptr) (void)hipStreamDestroy(stream);
@@ -271,16 +280,15 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
   return Ok();
 }
 
-// crypto/fft.h:71-80, computed with the same expression and libm.
+// crypto/fft.h:71-80, from the g++-built table (twiddles.cpp).
 void Twiddles(int n, std::vector<double2>* fwd, std::vector<double2>* bwd) {
+  std::vector<double> f(n), b(n);
+  pirg_internal_twiddles(n, f.data(), b.data());
   fwd->resize(n / 2);
   bwd->resize(n / 2);
-  const double pi = std::acos(-1.0);
-  const size_t size = static_cast<size_t>(n);
-  for (size_t i = 0; i < size / 2; ++i) {
-    double angle = -2.0 * pi * i / size;
-    (*fwd)[i] = make_double2(std::cos(angle), std::sin(angle));
-    (*bwd)[i] = make_double2(std::cos(-angle), std::sin(-angle));
+  for (int i = 0; i < n / 2; ++i) {
+    (*fwd)[i] = make_double2(f[2 * i], f[2 * i + 1]);
+    (*bwd)[i] = make_double2(b[2 * i], b[2 * i + 1]);
   }
 }
 
@@ -305,7 +313,13 @@ Error UploadTiles(const void* host, size_t rowbytes, int rows, bool host_is_devi
     PIRG_HIP(hipStreamSynchronize(s));
     return Ok();
   }
-  const size_t chunk_bytes = 256u << 20;
+  // Chunk size of the staged upload (PIRG_UPLOAD_CHUNK_BYTES overrides it so
+  // tests can exercise the multi-chunk path on small matrices).
+  size_t chunk_bytes = static_cast<size_t>(256) << 20;
+  if (const char* env = std::getenv("PIRG_UPLOAD_CHUNK_BYTES")) {
+    const long long v = std::atoll(env);
+    if (v > 0) chunk_bytes = static_cast<size_t>(v);
+  }
   int rows_per = static_cast<int>(std::max<size_t>(16, chunk_bytes / std::max<size_t>(rowbytes, 1)));
   rows_per = (rows_per / 16) * 16;
   if (rows_per < 16) rows_per = 16;
@@ -331,6 +345,7 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   S->K = db_cols;
   PIRG_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
   for (auto& x : S->ev) PIRG_HIP(hipEventCreate(&x));
+  for (auto& x : S->kev) PIRG_HIP(hipEventCreate(&x));
   PIRG_HIP(ConfigureFftKernels(S->N));
 
   // DB planes.
@@ -450,8 +465,8 @@ Error EnsureWorkspace(pirg_server* S, int cap) {
       {&W.fd, nq * S->K * 8},
       {&W.sd, nq * 2 * S->e * d * 8},
       {&W.pk, nq * 2 * S->k * d * 8},
-      {&W.bt_db, static_cast<size_t>(S->dbL.kblocks) * nt_db * kTileBytes},
-      {&W.bt_pk, static_cast<size_t>(S->pkL.kblocks) * nt_db * kTileBytes},
+      {&W.bt_db, static_cast<size_t>(S->dbL.kblocks) * nt_db * S->dbL.planes * kTileBytes},
+      {&W.bt_pk, static_cast<size_t>(S->pkL.kblocks) * nt_db * S->pkL.planes * kTileBytes},
       {&W.corr_db, nq * 8},
       {&W.corr_pk, nq * 8},
       {&W.db_partial, static_cast<size_t>(S->ks_db) * nq * S->M * 8},
@@ -568,17 +583,21 @@ Error RunBatch(pirg_server* S, int nq) {
   PIRG_HIP(hipEventRecord(S->ev[0], st));
   // DB product (pir_server.cc:215-229).
   PIRG_HIP(LaunchVectorsToTiles(W.fd.as<u64>(), nq, S->K, S->K, S->dbL.kblocks, S->nl,
-                                S->qpt, S->dbL.c_off, W.bt_db.as<int8_t>(),
+                                S->qpt, S->dbL.planes, S->dbL.c_off, W.bt_db.as<int8_t>(),
                                 W.corr_db.as<u64>(), st));
+  PIRG_HIP(hipEventRecord(S->kev[0], st));
   PIRG_HIP(LaunchLimbScan(S->db_tiles.as<int8_t>(), S->dbL, W.bt_db.as<int8_t>(), nq,
                           S->nl, S->qpt, S->log_q, S->ks_db, W.db_partial.as<u64>(), st));
+  PIRG_HIP(hipEventRecord(S->kev[1], st));
   PIRG_HIP(hipEventRecord(S->ev[1], st));
   // Pack product (pir_server.cc:237-248): y = concat(y_vec_g).
   PIRG_HIP(LaunchVectorsToTiles(W.pk.as<u64>(), nq, 2 * S->k * d, S->k * d,
-                                S->pkL.kblocks, S->nl, S->qpt, S->pkL.c_off,
+                                S->pkL.kblocks, S->nl, S->qpt, S->pkL.planes, S->pkL.c_off,
                                 W.bt_pk.as<int8_t>(), W.corr_pk.as<u64>(), st));
+  PIRG_HIP(hipEventRecord(S->kev[2], st));
   PIRG_HIP(LaunchLimbScan(S->pk_tiles.as<int8_t>(), S->pkL, W.bt_pk.as<int8_t>(), nq,
                           S->nl, S->qpt, S->log_q, S->ks_pk, W.pk_partial.as<u64>(), st));
+  PIRG_HIP(hipEventRecord(S->kev[3], st));
   // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279).
   PIRG_HIP(LaunchSpectra(W.pk.as<u64>() + static_cast<size_t>(S->k) * d, nq * S->k, S->k,
                          2LL * S->k * d, S->fin_v, S->F, S->tw_fwd.as<double2>(),
@@ -676,6 +695,12 @@ void CollectTiming(pirg_server* S) {
   S->db_ms += a;
   S->pack_ms += b;
   S->eval_ms += c;
+  float ka = 0, kb = 0;
+  (void)hipEventElapsedTime(&ka, S->kev[0], S->kev[1]);
+  (void)hipEventElapsedTime(&kb, S->kev[2], S->kev[3]);
+  S->db_scan_ms += ka;
+  S->pk_scan_ms += kb;
+  S->scans += 1;
   S->timing_pending = false;
 }
 
@@ -764,7 +789,7 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
   if ((er = src.alloc(static_cast<size_t>(rows) * wpr * 8)).code) return er;
   if ((er = v.alloc(static_cast<size_t>(cols) * 8)).code) return er;
   const int nl = 8, qpt = 2;
-  if ((er = bt.alloc(static_cast<size_t>(L.kblocks) * kTileBytes)).code) return er;
+  if ((er = bt.alloc(static_cast<size_t>(L.kblocks) * planes * kTileBytes)).code) return er;
   if ((er = corr.alloc(8)).code) return er;
   const int ks = LimbScanKSplit(L, 1);
   if ((er = partial.alloc(static_cast<size_t>(ks) * rows * 8)).code) return er;
@@ -775,8 +800,8 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
   } else {
     PIRG_HIP(LaunchPackToTiles(src.p, 4, rows, cols, 0, rows, L, tiles.as<int8_t>(), s));
   }
-  PIRG_HIP(LaunchVectorsToTiles(v.as<u64>(), 1, cols, cols, L.kblocks, nl, qpt, c_off,
-                                bt.as<int8_t>(), corr.as<u64>(), s));
+  PIRG_HIP(LaunchVectorsToTiles(v.as<u64>(), 1, cols, cols, L.kblocks, nl, qpt, planes,
+                                c_off, bt.as<int8_t>(), corr.as<u64>(), s));
   PIRG_HIP(LaunchLimbScan(tiles.as<int8_t>(), L, bt.as<int8_t>(), 1, nl, qpt, 64, ks,
                           partial.as<u64>(), s));
   PIRG_HIP(hipDeviceSynchronize());
@@ -941,6 +966,19 @@ int pirg_server_stage_times(pirg_server* S, double* db_mult_ms, double* packing_
   return 0;
 }
 
+int pirg_server_scan_times(pirg_server* S, double* db_scan_ms, double* pack_scan_ms,
+                           long long* launches, int reset) {
+  CollectTiming(S);
+  if (db_scan_ms) *db_scan_ms = S->db_scan_ms;
+  if (pack_scan_ms) *pack_scan_ms = S->pk_scan_ms;
+  if (launches) *launches = S->scans;
+  if (reset) {
+    S->db_scan_ms = S->pk_scan_ms = 0;
+    S->scans = 0;
+  }
+  return 0;
+}
+
 double pirg_server_fft_margin(pirg_server* S, int reset) {
   unsigned long long bits = 0;
   (void)hipStreamSynchronize(S->stream);
@@ -1019,6 +1057,39 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
   };
   Error er = run();
   return er.ok() ? 0 : Report(er, err, errlen);
+}
+
+int pirg_copy_probe(size_t bytes, int repeats, double* ms) {
+  auto run = [&]() -> Error {
+    Error er = CheckDevice();
+    if (!er.ok()) return er;
+    DevBuf a, b;
+    if ((er = a.alloc(bytes)).code) return er;
+    if ((er = b.alloc(bytes)).code) return er;
+    PIRG_HIP(hipMemset(a.p, 1, bytes));
+    hipEvent_t e0, e1;
+    PIRG_HIP(hipEventCreate(&e0));
+    PIRG_HIP(hipEventCreate(&e1));
+    PIRG_HIP(LaunchCopyProbe(a.p, b.p, bytes, nullptr));
+    PIRG_HIP(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < repeats; ++i) PIRG_HIP(LaunchCopyProbe(a.p, b.p, bytes, nullptr));
+    PIRG_HIP(hipEventRecord(e1, nullptr));
+    PIRG_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    PIRG_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / std::max(repeats, 1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return Ok();
+  };
+  Error er = run();
+  return er.code;
+}
+
+int pirg_fft_twiddles(int n, double* fwd, double* bwd) {
+  if (n < 2 || n > (1 << 20) || (n & (n - 1)) != 0) return PIRG_INVALID_ARGUMENT;
+  pirg_internal_twiddles(n, fwd, bwd);
+  return 0;
 }
 
 const char* pirg_version(void) {

@@ -1420,6 +1420,18 @@ int orc_inner_product_exact_w(const uint64_t* u, const uint64_t* v, int n, int d
   ORC_GUARD_END(err, errlen)
 }
 
+// The twiddle table of FftPlan (crypto/fft.h:71-80) as this build computes it
+// (interleaved re, im).
+void orc_fft_twiddles(int n, double* fwd, double* bwd) {
+  FftPlan plan(static_cast<size_t>(n));
+  for (int i = 0; i < n / 2; ++i) {
+    fwd[2 * i] = plan.tw_fwd[i].real();
+    fwd[2 * i + 1] = plan.tw_fwd[i].imag();
+    bwd[2 * i] = plan.tw_bwd[i].real();
+    bwd[2 * i + 1] = plan.tw_bwd[i].imag();
+  }
+}
+
 // Raw std::mt19937_64 output (standard-specified), the generator of
 // crypto/polynomial_test.cc:443,492,552.
 void orc_mt19937_64(uint64_t seed, uint64_t* out, size_t n) {

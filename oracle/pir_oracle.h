@@ -100,6 +100,7 @@ int orc_inner_product_exact_w(const uint64_t* u, const uint64_t* v, int n, int d
                               int u_is_signed, int width, uint64_t* out,
                               char* err, size_t errlen);
 void orc_mt19937_64(uint64_t seed, uint64_t* out, size_t n);
+void orc_fft_twiddles(int n, double* fwd, double* bwd);
 
 /* ---- Matrix (crypto/matrix.cc) ---- */
 int orc_condense_u16(const uint16_t* data, int rows, int cols, uint64_t* out,

@@ -125,3 +125,21 @@ def test_create_without_device_fails_loudly():
         srv.close()
     except ia.PirError as e:
         assert e.code == ia.UNAVAILABLE
+
+
+@pytest.mark.parametrize("n", [4, 64, 4096, 8192])
+def test_fft_twiddles_match_oracle(n):
+    """The FFT kernels' twiddle table is bit-identical to the oracle's FftPlan
+    (crypto/fft.h:71-80) and within one ulp of cos/sin."""
+    import math
+    dp = ctypes.POINTER(ctypes.c_double)
+    a, b, c, d = (np.zeros(n) for _ in range(4))
+    assert ia.lib().pirg_fft_twiddles(n, a.ctypes.data_as(dp), b.ctypes.data_as(dp)) == 0
+    O.lib().orc_fft_twiddles(n, c.ctypes.data_as(dp), d.ctypes.data_as(dp))
+    np.testing.assert_array_equal(a, c)
+    np.testing.assert_array_equal(b, d)
+    for i in range(0, n // 2, max(1, n // 64)):
+        ang = -2.0 * math.pi * i / n
+        for got, want in [(a[2 * i], math.cos(ang)), (a[2 * i + 1], math.sin(ang)),
+                          (b[2 * i], math.cos(-ang)), (b[2 * i + 1], math.sin(-ang))]:
+            assert abs(got - want) <= 2 * math.ulp(max(abs(want), 1e-300))

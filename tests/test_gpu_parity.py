@@ -103,3 +103,110 @@ def test_process_metrics_params_synthetic(log_d, n, t, ell):
     mat = O.synthetic_material(P, seed=log_d * 100 + t)
     reqs = [O.synthetic_request(P, seed=i) for i in range(2)]
     _compare(P, mat, reqs)
+
+
+def test_inner_product_fft_d4096_many_trials():
+    # At d = 4096 the ExternalProduct chunks sit at the rounding limit, so any
+    # difference in FFT arithmetic (e.g. a one-ulp twiddle) shows up here.
+    rng = np.random.default_rng(4096)
+    for _ in range(8):
+        u = rng.integers(0, 1 << 19, size=(6, 4096), dtype=np.uint64)
+        v = rng.integers(0, 1 << 52, size=(6, 4096), dtype=np.uint64)
+        want, err = O.inner_product_fft(u, v, 19, 52, 19)
+        got = ia.inner_product_fft(u, v, 19, 52, 19)
+        np.testing.assert_array_equal(got, want)
+
+
+def test_batch_equals_sequential_with_invalid_request():
+    P = O.metrics_tool_params(n=2048, log_d=8, t=4, ell=2)
+    mat = O.synthetic_material(P, seed=21)
+    reqs = [O.synthetic_request(P, seed=100 + i) for i in range(37)]
+    bad = O.Request(reqs[5].first_dim, reqs[5].second_dim, reqs[5].packing_key[:-1])
+    greqs = [gpu_request(r) for r in reqs]
+    greqs[5] = gpu_request(bad)
+    with server_from_material(P, mat) as srv:
+        batch, statuses, msg = srv.process_batch(greqs)
+        assert statuses[5] == ia.INVALID_ARGUMENT
+        assert "Packing key must contain 2 * num_digits polynomials." in msg
+        assert batch[5] is None
+        for i, r in enumerate(reqs):
+            if i == 5:
+                continue
+            single = srv.process_response(gpu_request(r)).ciphertexts
+            np.testing.assert_array_equal(batch[i].ciphertexts, single)
+            if i % 6 == 0:
+                want, _, _ = O.process_response(P, mat, r)
+                np.testing.assert_array_equal(single, want)
+
+
+@pytest.mark.parametrize("nq", [1, 2, 3, 16, 33, 64])
+def test_wide_batches_match_oracle(nq):
+    # Every scan configuration (T = 1, 2, 4, 8, 16 tiles per pass).
+    P = O.metrics_tool_params(n=4096, log_d=9, t=2)
+    mat = O.synthetic_material(P, seed=nq)
+    reqs = [O.synthetic_request(P, seed=1000 * nq + i) for i in range(nq)]
+    with server_from_material(P, mat) as srv:
+        batch, statuses, _ = srv.process_batch([gpu_request(r) for r in reqs])
+    assert statuses == [0] * nq
+    for i in range(0, nq, max(1, nq // 5)):
+        want, _, _ = O.process_response(P, mat, reqs[i])
+        np.testing.assert_array_equal(batch[i].ciphertexts, want)
+
+
+def test_multi_chunk_upload(monkeypatch):
+    # The staged host -> HBM conversion in many small chunks (chunk boundaries
+    # that split 16-row strips).
+    monkeypatch.setenv("PIRG_UPLOAD_CHUNK_BYTES", "3000")
+    P = O.metrics_tool_params(n=8192, log_d=8, t=4, ell=2)
+    mat = O.synthetic_material(P, seed=8)
+    req = O.synthetic_request(P, seed=9)
+    want, _, _ = O.process_response(P, mat, req)
+    with server_from_material(P, mat) as srv:
+        got = srv.process_response(gpu_request(req)).ciphertexts
+    np.testing.assert_array_equal(got, want)
+
+
+def test_process_errors_match_reference_messages():
+    # pir_server_test.cc:343-449.
+    P = O.metrics_tool_params(n=2048, log_d=8, t=2)
+    s = O.sizes(P)
+    mat = O.synthetic_material(P, seed=1)
+    req = O.synthetic_request(P, seed=2)
+    cases = [
+        (O.Request(req.first_dim, req.second_dim, req.packing_key[:-1]),
+         "Packing key must contain 2 * num_digits polynomials."),
+        (O.Request(req.first_dim[:-1], req.second_dim, req.packing_key),
+         f"First dimension query size ({s.r - 1}) is invalid for DB columns ({s.K})."),
+        (O.Request(np.zeros(100000, np.uint64), req.second_dim, req.packing_key),
+         f"First dimension query size (100000) is invalid for DB columns ({s.K})."),
+        (O.Request(req.first_dim, req.second_dim[:-1], req.packing_key),
+         "second_dimension_query size mismatch. Must be 2 * num_digits."),
+    ]
+    with server_from_material(P, mat) as srv:
+        for bad, msg in cases:
+            with pytest.raises(ia.PirError) as ei:
+                srv.process_response(gpu_request(bad))
+            assert ei.value.code == ia.INVALID_ARGUMENT
+            assert ei.value.message == msg
+        # The server keeps answering after rejected requests.
+        want, _, _ = O.process_response(P, mat, req)
+        np.testing.assert_array_equal(srv.process_response(gpu_request(req)).ciphertexts, want)
+
+
+def test_full_size_response_matches_oracle():
+    """The headline shape (n = 2^20, d = 4096, t = 8): one request, GPU vs the
+    oracle, bit for bit, on range-faithful synthetic material."""
+    import bench
+    P = ia.metrics_tool_params(n=1 << 20, log_d=12, t=8)
+    mat = bench.synthetic_material(P, seed=2026)
+    req = bench.synthetic_requests(P, 1, seed=7)[0]
+    OP = O.Params(**{f: getattr(P, f) for f in O.Params.__dataclass_fields__})
+    omat = O.Material(mat["db"], mat["pack"], mat["a_tilde"], mat["t_vec_h"],
+                      mat["second_dim_a"], mat["db_cols"])
+    want, _, _ = O.process_response(OP, omat, O.Request(req.first_dimension_query,
+                                                        req.second_dimension_query,
+                                                        req.packing_key))
+    with ia.PirServer.create_flat(P, mat["db"], mat["db_cols"], mat["pack"], mat["a_tilde"],
+                                  mat["t_vec_h"], mat["second_dim_a"]) as srv:
+        got = srv.process_response(req).ciphertexts
+    np.testing.assert_array_equal(got, want)
