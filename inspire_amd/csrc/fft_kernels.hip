@@ -7,147 +7,50 @@
 // the ExternalProduct chunks reach 2^52.6 in magnitude), so its integer output
 // depends on the exact sequence of IEEE operations. These kernels therefore
 // perform, butterfly for butterfly, the same operations as crypto/fft.h:106-151
-// -- radix-2 decimation in time on bit-reversed input, twiddles computed on the
-// host by the same cos/sin expression, complex products as (ac - bd, ad + bc)
-// with no fused multiply-add (this file is compiled with -ffp-contract=off) --
-// and accumulate the chunk-pair spectra in the reference's order. Grouping two
-// radix-2 stages per LDS pass changes the schedule, not the arithmetic.
+// (the register schedule of fft_regs.h), with twiddles computed on the host by
+// the same cos/sin expression, complex products as (ac - bd, ad + bc) with no
+// fused multiply-add (this file is compiled with -ffp-contract=off), and
+// accumulate the chunk-pair spectra in the reference's term order.
 //
-// A workgroup owns one 2d-point transform held in LDS (16 * 2d bytes: 128 KiB
-// at d = 4096, within the 160 KiB of a CDNA4 CU).
-#include "pirg_internal.h"
+// A workgroup of N/16 threads owns one N-point transform (N = 2d); the kernels
+// are instantiated per log2 N and dispatched at launch.
+#include "fft_regs.h"
 
 namespace pirg {
 namespace {
 
-constexpr int kFftThreads = 512;
-
-// crypto/fft.h:122-125 for one butterfly: v = y * w, x' = x + v, y' = x - v.
-__device__ __forceinline__ void Butterfly(double2& x, double2& y, const double2 w) {
-  const double vr = y.x * w.x - y.y * w.y;
-  const double vi = y.x * w.y + y.y * w.x;
-  const double ux = x.x, uy = x.y;
-  x = make_double2(ux + vr, uy + vi);
-  y = make_double2(ux - vr, uy - vi);
-}
-
-// Twiddles of the radix-4 groups a thread owns in the pass starting at stage
-// s: w1 = tw[j * step1] (stage s), w2 = tw[j * step2], w3 = tw[(j + h) * step2]
-// (stage s + 1).
-constexpr int kMaxGroups = 4;  // n / 4 <= kFftThreads * kMaxGroups (n <= 8192)
-
-__device__ __forceinline__ void LoadPassTwiddles(const double2* __restrict__ tw,
-                                                 int log_n, int s, double2* w1,
-                                                 double2* w2, double2* w3) {
-  const int n = 1 << log_n;
-  const int h = 1 << (s - 1);
-  const int step1 = n >> s;
-  const int step2 = n >> (s + 1);
-#pragma unroll
-  for (int m = 0; m < kMaxGroups; ++m) {
-    const int g = threadIdx.x + m * blockDim.x;
-    if (g < n / 4) {
-      const int j = g & (h - 1);
-      w1[m] = tw[j * step1];
-      w2[m] = tw[j * step2];
-      w3[m] = tw[(j + h) * step2];
-    }
-  }
-}
-
-// crypto/fft.h:117-128 on a bit-reversed LDS buffer, two stages per pass; the
-// next pass's twiddles are loaded while the current pass runs.
-__device__ void FftLds(double2* buf, const double2* __restrict__ tw, int log_n) {
-  const int n = 1 << log_n;
-  int s = 1;
-  double2 w1[kMaxGroups], w2[kMaxGroups], w3[kMaxGroups];
-  double2 n1[kMaxGroups], n2[kMaxGroups], n3[kMaxGroups];
-  if (log_n >= 2) LoadPassTwiddles(tw, log_n, 1, w1, w2, w3);
-  for (; s + 1 <= log_n; s += 2) {
-    const int h = 1 << (s - 1);
-    if (s + 3 <= log_n) LoadPassTwiddles(tw, log_n, s + 2, n1, n2, n3);
-#pragma unroll
-    for (int m = 0; m < kMaxGroups; ++m) {
-      const int g = threadIdx.x + m * blockDim.x;
-      if (g < n / 4) {
-        const int j = g & (h - 1);
-        const int e0 = ((g >> (s - 1)) << (s + 1)) + j;
-        double2 x0 = buf[e0];
-        double2 x1 = buf[e0 + h];
-        double2 x2 = buf[e0 + 2 * h];
-        double2 x3 = buf[e0 + 3 * h];
-        Butterfly(x0, x1, w1[m]);
-        Butterfly(x2, x3, w1[m]);
-        Butterfly(x0, x2, w2[m]);
-        Butterfly(x1, x3, w3[m]);
-        buf[e0] = x0;
-        buf[e0 + h] = x1;
-        buf[e0 + 2 * h] = x2;
-        buf[e0 + 3 * h] = x3;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < kMaxGroups; ++m) {
-      w1[m] = n1[m];
-      w2[m] = n2[m];
-      w3[m] = n3[m];
-    }
-  }
-  if (s == log_n) {
-    const int h = 1 << (s - 1);
-    const int step = n >> s;
-    for (int b = threadIdx.x; b < n / 2; b += blockDim.x) {
-      const int j = b & (h - 1);
-      const int e0 = ((b >> (s - 1)) << s) + j;
-      double2 x0 = buf[e0];
-      double2 x1 = buf[e0 + h];
-      Butterfly(x0, x1, tw[j * step]);
-      buf[e0] = x0;
-      buf[e0 + h] = x1;
-    }
-    __syncthreads();
-  }
-}
-
-__device__ __forceinline__ int BitRev(int j, int log_n) {
-  return static_cast<int>(__brev(static_cast<unsigned>(j)) >> (32 - log_n));
-}
+using fftr::Geo;
+using fftr::InSlot;
+using fftr::OutSlot;
+using fftr::Transform;
 
 // Forward spectra (crypto/polynomial_fft.cc:119-152): one workgroup per
-// (polynomial, chunk).
-__global__ __launch_bounds__(kFftThreads) void SpectraKernel(
+// (polynomial, chunk). Input j = t + NT*k is coefficient j for j < d, zero
+// padding above.
+template <int L>
+__global__ __launch_bounds__(Geo<L>::NT) void SpectraKernel(
     const u64* __restrict__ src, int per_group, long long group_stride,
-    ChunkRule rule, int d, int log_n, const double2* __restrict__ tw,
-    double2* __restrict__ dst) {
-  extern __shared__ double2 buf[];
-  const int n = 1 << log_n;
+    ChunkRule rule, const double2* __restrict__ st, double2* __restrict__ dst) {
+  extern __shared__ double2 lds[];
+  constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
+  const int t = threadIdx.x;
   const int job = blockIdx.x;
   const int p = job / rule.num_chunks;
   const int c = job % rule.num_chunks;
   const u64* poly = src + (long long)(p / per_group) * group_stride +
-                    (long long)(p % per_group) * d;
-  constexpr int kElems = 8;
-  for (int j0 = threadIdx.x; j0 < n; j0 += blockDim.x * kElems) {
-    u64 x[kElems];
+                    (long long)(p % per_group) * d + t;
+  u64 x[E / 2];
 #pragma unroll
-    for (int e = 0; e < kElems; ++e) {
-      const int j = j0 + e * blockDim.x;
-      x[e] = (j < d) ? poly[j] : 0ULL;
-    }
+  for (int k = 0; k < E / 2; ++k) x[k] = poly[NT * k];
+  double2 v[16];
 #pragma unroll
-    for (int e = 0; e < kElems; ++e) {
-      const int j = j0 + e * blockDim.x;
-      if (j < n) {
-        const double v = (j < d) ? ChunkValue(rule, x[e], c) : 0.0;
-        buf[BitRev(j, log_n)] = make_double2(v, 0.0);
-      }
-    }
-  }
-  __syncthreads();
-  FftLds(buf, tw, log_n);
-  double2* out = dst + (long long)job * n;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) out[j] = buf[j];
+  for (int k = 0; k < E / 2; ++k) v[InSlot<L>(k)] = make_double2(ChunkValue(rule, x[k], c), 0.0);
+#pragma unroll
+  for (int k = E / 2; k < E; ++k) v[InSlot<L>(k)] = make_double2(0.0, 0.0);
+  Transform<L>(v, lds, st);
+  double2* out = dst + (long long)job * N + t;
+#pragma unroll
+  for (int e = 0; e < E; ++e) out[NT * e] = v[OutSlot<L>(e)];
 }
 
 struct IpJob {
@@ -156,74 +59,79 @@ struct IpJob {
   u64* const* optr;            // [n_out] base of the pair partials (npairs x d)
 };
 
+// Maximum of non-negative doubles (their bit patterns order the same way as
+// the values): a wave-level reduction and one atomic per wave when the
+// workgroup is made of full waves, one atomic per thread otherwise.
+template <int NT>
+__device__ __forceinline__ void MarginMax(double local, unsigned long long* margin) {
+  unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(local));
+  if constexpr (NT % 64 == 0) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(bits, off);
+      bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(margin, bits);
+  } else {
+    atomicMax(margin, bits);
+  }
+}
+
 // Accumulate-invert-round (crypto/polynomial_fft.cc:154-196) for one
 // (output, chunk pair): acc[j] = sum_i U_i,a[j] * V_i,b[j] in term order, the
 // inverse FFT, real / 2d, round half away from zero, shift by chunk_bits*(a+b)
 // and fold X^d = -1. One workgroup per (output, pair).
-__global__ __launch_bounds__(kFftThreads) void IpKernel(
-    IpJob jobs, int nterms, int nu, int nv, PairList pairs, int d, int log_n,
-    const double2* __restrict__ tw, unsigned long long* margin) {
-  extern __shared__ double2 buf[];
-  const int n = 1 << log_n;
+template <int L>
+__global__ __launch_bounds__(Geo<L>::NT) void IpKernel(
+    IpJob jobs, int nterms, int nu, int nv, PairList pairs,
+    const double2* __restrict__ st, unsigned long long* margin) {
+  extern __shared__ double2 lds[];
+  constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
+  const int t = threadIdx.x;
   const int o = blockIdx.x / pairs.count;
   const int pi = blockIdx.x % pairs.count;
   const int a = pairs.a[pi], b = pairs.b[pi], sh = pairs.shift[pi];
-  const double2* U = jobs.uptr[o];
-  const double2* V = jobs.vptr[o];
-  // kElems independent elements per thread keep that many load chains in
-  // flight; the term order of each element's sum is the reference's.
-  constexpr int kElems = 8;
-  for (int j0 = threadIdx.x; j0 < n; j0 += blockDim.x * kElems) {
-    double ax[kElems], ay[kElems];
+  const double2* U = jobs.uptr[o] + t;
+  const double2* V = jobs.vptr[o] + t;
+  double2 v[16];
 #pragma unroll
-    for (int e = 0; e < kElems; ++e) {
-      ax[e] = 0.0;
-      ay[e] = 0.0;
-    }
-    for (int i = 0; i < nterms; ++i) {
-      const double2* Ui = U + ((long long)i * nu + a) * n;
-      const double2* Vi = V + ((long long)i * nv + b) * n;
-      double2 uu[kElems], vv[kElems];
+  for (int k = 0; k < E; ++k) v[k] = make_double2(0.0, 0.0);
+  for (int i = 0; i < nterms; ++i) {
+    const double2* Ui = U + ((long long)i * nu + a) * N;
+    const double2* Vi = V + ((long long)i * nv + b) * N;
+    double2 uu[E], vv[E];
 #pragma unroll
-      for (int e = 0; e < kElems; ++e) {
-        const int j = j0 + e * blockDim.x;
-        uu[e] = (j < n) ? Ui[j] : make_double2(0.0, 0.0);
-        vv[e] = (j < n) ? Vi[j] : make_double2(0.0, 0.0);
-      }
-#pragma unroll
-      for (int e = 0; e < kElems; ++e) {
-        const double pr = uu[e].x * vv[e].x - uu[e].y * vv[e].y;
-        const double pim = uu[e].x * vv[e].y + uu[e].y * vv[e].x;
-        ax[e] = ax[e] + pr;
-        ay[e] = ay[e] + pim;
-      }
+    for (int k = 0; k < E; ++k) {
+      uu[k] = Ui[NT * k];
+      vv[k] = Vi[NT * k];
     }
 #pragma unroll
-    for (int e = 0; e < kElems; ++e) {
-      const int j = j0 + e * blockDim.x;
-      if (j < n) buf[BitRev(j, log_n)] = make_double2(ax[e], ay[e]);
+    for (int k = 0; k < E; ++k) {
+      const double pr = uu[k].x * vv[k].x - uu[k].y * vv[k].y;
+      const double pim = uu[k].x * vv[k].y + uu[k].y * vv[k].x;
+      double2& acc = v[InSlot<L>(k)];
+      acc.x = acc.x + pr;
+      acc.y = acc.y + pim;
     }
   }
-  __syncthreads();
-  FftLds(buf, tw, log_n);
+  Transform<L>(v, lds, st);
   // x / 2d of polynomial_fft.cc:184: 2d is a power of two, so the product by
   // its exact reciprocal is the same correctly rounded value.
-  const double inv_n = ldexp(1.0, -log_n);
-  u64* out = jobs.optr[o] + (long long)pi * d;
+  const double inv_n = 1.0 / static_cast<double>(N);
+  u64* out = jobs.optr[o] + (long long)pi * d + t;
   double local_max = 0.0;
-  for (int m = threadIdx.x; m < d; m += blockDim.x) {
-    const double x0 = buf[m].x * inv_n;
-    const double x1 = buf[m + d].x * inv_n;
+#pragma unroll
+  for (int e = 0; e < E / 2; ++e) {
+    const double x0 = v[OutSlot<L>(e)].x * inv_n;           // position t + NT*e
+    const double x1 = v[OutSlot<L>(e + E / 2)].x * inv_n;   // position + d
     const double r0 = round(x0);
     const double r1 = round(x1);
     local_max = fmax(local_max, fmax(fabs(x0 - r0), fabs(x1 - r1)));
     const u64 v0 = static_cast<u64>(static_cast<i64>(r0)) << sh;
     const u64 v1 = static_cast<u64>(static_cast<i64>(r1)) << sh;
-    out[m] = v0 - v1;
+    out[NT * e] = v0 - v1;
   }
-  if (margin != nullptr) {
-    atomicMax(margin, static_cast<unsigned long long>(__double_as_longlong(local_max)));
-  }
+  if (margin != nullptr) MarginMax<NT>(local_max, margin);
 }
 
 __device__ __forceinline__ u64 RescaleCoeff(u64 c, int log_cur, int log_new) {
@@ -284,37 +192,41 @@ __global__ void FinalizeCombineKernel(FinalizeArgs A) {
 
 // ExternalProduct's decomposition (crypto/external_product.cc:60-72,
 // polynomial.cc:340-374) fused with the forward FFT of each digit chunk.
-__global__ __launch_bounds__(kFftThreads) void DigitsSpectraKernel(
-    DigitsArgs A, int log_n, const double2* __restrict__ tw) {
-  extern __shared__ double2 buf[];
-  const int n = 1 << log_n;
+template <int L>
+__global__ __launch_bounds__(Geo<L>::NT) void DigitsSpectraKernel(
+    DigitsArgs A, const double2* __restrict__ st) {
+  extern __shared__ double2 lds[];
+  constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT;
+  const int t = threadIdx.x;
   const int nu = A.rule.num_chunks;
   const int job = blockIdx.x;                 // ((q*ell + s)*2e + jj)*nu + a
   const int a = job % nu;
   const int jj = (job / nu) % (2 * A.e);
   const long long qs = job / (nu * 2 * A.e);  // q*ell + s
-  const u64* st = (jj < A.e ? A.st_a : A.st_b) + qs * A.d;
+  const u64* src = (jj < A.e ? A.st_a : A.st_b) + qs * (N / 2) + t;
   const int digit = jj % A.e;
   const int to_shift = A.log_q - A.e * A.log_digit;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double v = 0.0;
-    if (j < A.d) {
-      u64 x = LowBitsD(st[j], A.log_q);
-      if (to_shift > 0) {
-        x = static_cast<u64>((static_cast<unsigned __int128>(x) +
-                              (1ULL << (to_shift - 1))) >> to_shift);
-      }
-      u64 cur = x;
-      for (int t = 0; t < digit; ++t) cur = (A.log_digit >= 64) ? 0 : (cur >> A.log_digit);
-      const u64 dg = LowBitsD(cur, A.log_digit);
-      v = ChunkValue(A.rule, dg, a);
+  u64 x[E / 2];
+#pragma unroll
+  for (int k = 0; k < E / 2; ++k) x[k] = src[NT * k];
+  double2 v[16];
+#pragma unroll
+  for (int k = 0; k < E / 2; ++k) {
+    u64 y = LowBitsD(x[k], A.log_q);
+    if (to_shift > 0) {
+      y = static_cast<u64>((static_cast<unsigned __int128>(y) + (1ULL << (to_shift - 1))) >>
+                           to_shift);
     }
-    buf[BitRev(j, log_n)] = make_double2(v, 0.0);
+    u64 cur = y;
+    for (int r = 0; r < digit; ++r) cur = (A.log_digit >= 64) ? 0 : (cur >> A.log_digit);
+    v[InSlot<L>(k)] = make_double2(ChunkValue(A.rule, LowBitsD(cur, A.log_digit), a), 0.0);
   }
-  __syncthreads();
-  FftLds(buf, tw, log_n);
-  double2* out = A.dspec + (long long)job * n;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) out[j] = buf[j];
+#pragma unroll
+  for (int k = E / 2; k < E; ++k) v[InSlot<L>(k)] = make_double2(0.0, 0.0);
+  Transform<L>(v, lds, st);
+  double2* out = A.dspec + (long long)job * N + t;
+#pragma unroll
+  for (int e2 = 0; e2 < E; ++e2) out[NT * e2] = v[OutSlot<L>(e2)];
 }
 
 
@@ -359,24 +271,97 @@ __global__ void SumPairsKernel(const u64* __restrict__ partial, int npairs,
 // ---------------------------------------------------------------------------
 // Launchers.
 
-static int GridFor(long long total) {
+namespace {
+
+int GridFor(long long total) {
   long long b = (total + 255) / 256;
   if (b > 65535LL * 8) b = 65535LL * 8;
   return static_cast<int>(b < 1 ? 1 : b);
 }
 
-hipError_t ConfigureFftKernels(int n) {
-  const int bytes = n * static_cast<int>(sizeof(double2));
-  hipError_t e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(SpectraKernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(IpKernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(DigitsSpectraKernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  return e;
+// Instantiations per log2 N: Fn<L>::Run(args...) for the runtime L.
+template <template <int> class Fn, typename... Args>
+hipError_t DispatchLogN(int L, Args&&... args) {
+  switch (L) {
+    case 1: return Fn<1>::Run(args...);
+    case 2: return Fn<2>::Run(args...);
+    case 3: return Fn<3>::Run(args...);
+    case 4: return Fn<4>::Run(args...);
+    case 5: return Fn<5>::Run(args...);
+    case 6: return Fn<6>::Run(args...);
+    case 7: return Fn<7>::Run(args...);
+    case 8: return Fn<8>::Run(args...);
+    case 9: return Fn<9>::Run(args...);
+    case 10: return Fn<10>::Run(args...);
+    case 11: return Fn<11>::Run(args...);
+    case 12: return Fn<12>::Run(args...);
+    case 13: return Fn<13>::Run(args...);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int L>
+struct Configure {
+  static hipError_t Run() {
+    const int bytes = static_cast<int>(Geo<L>::kLds * sizeof(double2));
+    hipError_t e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(SpectraKernel<L>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(IpKernel<L>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(DigitsSpectraKernel<L>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  }
+};
+
+template <int L>
+struct SpectraL {
+  static hipError_t Run(const u64* src, int npolys, int per_group, long long group_stride,
+                        const ChunkRule& rule, const double2* st, double2* dst,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(SpectraKernel<L>, dim3(npolys * rule.num_chunks), dim3(Geo<L>::NT),
+                       Geo<L>::kLds * sizeof(double2), s, src, per_group, group_stride, rule,
+                       st, dst);
+    return hipGetLastError();
+  }
+};
+
+template <int L>
+struct IpL {
+  static hipError_t Run(const IpJob& jobs, int n_out, int nterms, int nu, int nv,
+                        const PairList& pairs, const double2* st,
+                        unsigned long long* margin, hipStream_t s) {
+    hipLaunchKernelGGL(IpKernel<L>, dim3(n_out * pairs.count), dim3(Geo<L>::NT),
+                       Geo<L>::kLds * sizeof(double2), s, jobs, nterms, nu, nv, pairs, st,
+                       margin);
+    return hipGetLastError();
+  }
+};
+
+template <int L>
+struct DigitsL {
+  static hipError_t Run(const DigitsArgs& A, const double2* st, hipStream_t s) {
+    const int jobs = A.nq * A.ell * 2 * A.e * A.rule.num_chunks;
+    hipLaunchKernelGGL(DigitsSpectraKernel<L>, dim3(jobs), dim3(Geo<L>::NT),
+                       Geo<L>::kLds * sizeof(double2), s, A, st);
+    return hipGetLastError();
+  }
+};
+
+bool ValidDims(const FftDims& F) {
+  return F.log_N >= 1 && F.log_N <= fftr::kMaxLogN && F.N == (1 << F.log_N) &&
+         F.d * 2 == F.N;
+}
+
+}  // namespace
+
+hipError_t ConfigureFftKernels(int N) {
+  int L = 0;
+  while ((1 << L) < N) ++L;
+  if ((1 << L) != N) return hipErrorInvalidValue;
+  return DispatchLogN<Configure>(L);
 }
 
 hipError_t LaunchSpectra(const u64* src, int npolys, int per_group,
@@ -384,10 +369,9 @@ hipError_t LaunchSpectra(const u64* src, int npolys, int per_group,
                          const FftDims& F, const double2* tw_fwd,
                          double2* dst, hipStream_t s) {
   if (npolys <= 0) return hipSuccess;
-  hipLaunchKernelGGL(SpectraKernel, dim3(npolys * rule.num_chunks),
-                     dim3(kFftThreads), F.N * sizeof(double2), s, src, per_group,
-                     group_stride, rule, F.d, F.log_N, tw_fwd, dst);
-  return hipGetLastError();
+  if (!ValidDims(F)) return hipErrorInvalidValue;
+  return DispatchLogN<SpectraL>(F.log_N, src, npolys, per_group, group_stride, rule, tw_fwd,
+                                dst, s);
 }
 
 hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
@@ -395,25 +379,21 @@ hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
                     const PairList& pairs, const FftDims& F,
                     const double2* tw_bwd, unsigned long long* margin,
                     hipStream_t s) {
-  if (n_out <= 0) return hipSuccess;
+  if (n_out <= 0 || pairs.count <= 0) return hipSuccess;
+  if (!ValidDims(F)) return hipErrorInvalidValue;
   IpJob jobs{uptr, vptr, optr};
-  hipLaunchKernelGGL(IpKernel, dim3(n_out * pairs.count), dim3(kFftThreads),
-                     F.N * sizeof(double2), s, jobs, nterms, nu, nv, pairs, F.d,
-                     F.log_N, tw_bwd, margin);
-  return hipGetLastError();
+  return DispatchLogN<IpL>(F.log_N, jobs, n_out, nterms, nu, nv, pairs, tw_bwd, margin, s);
+}
+
+hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,
+                               const double2* tw_fwd, hipStream_t s) {
+  if (!ValidDims(F)) return hipErrorInvalidValue;
+  return DispatchLogN<DigitsL>(F.log_N, A, tw_fwd, s);
 }
 
 hipError_t LaunchFinalizeCombine(const FinalizeArgs& A, hipStream_t s) {
   hipLaunchKernelGGL(FinalizeCombineKernel, dim3(GridFor((long long)A.nq * A.M)),
                      dim3(256), 0, s, A);
-  return hipGetLastError();
-}
-
-hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,
-                               const double2* tw_fwd, hipStream_t s) {
-  const int jobs = A.nq * A.ell * 2 * A.e * A.rule.num_chunks;
-  hipLaunchKernelGGL(DigitsSpectraKernel, dim3(jobs), dim3(kFftThreads),
-                     F.N * sizeof(double2), s, A, F.log_N, tw_fwd);
   return hipGetLastError();
 }
 

@@ -280,15 +280,23 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
   return Ok();
 }
 
-// crypto/fft.h:71-80, from the g++-built table (twiddles.cpp).
+// crypto/fft.h:71-80, from the g++-built table (twiddles.cpp), reordered by
+// stage for the kernels: stage s (1..log2 n) occupies entries
+// [2^(s-1) - 1, 2^s - 1), and entry 2^(s-1) - 1 + k is tw[k * (n >> s)], the
+// twiddle the reference's stage s reads for offset k. n - 1 entries.
 void Twiddles(int n, std::vector<double2>* fwd, std::vector<double2>* bwd) {
   std::vector<double> f(n), b(n);
   pirg_internal_twiddles(n, f.data(), b.data());
-  fwd->resize(n / 2);
-  bwd->resize(n / 2);
-  for (int i = 0; i < n / 2; ++i) {
-    (*fwd)[i] = make_double2(f[2 * i], f[2 * i + 1]);
-    (*bwd)[i] = make_double2(b[2 * i], b[2 * i + 1]);
+  fwd->assign(n - 1, make_double2(0.0, 0.0));
+  bwd->assign(n - 1, make_double2(0.0, 0.0));
+  for (int s = 1; (1 << s) <= n; ++s) {
+    const int base = (1 << (s - 1)) - 1;
+    const int step = n >> s;
+    for (int k = 0; k < (1 << (s - 1)); ++k) {
+      const int i = k * step;
+      (*fwd)[base + k] = make_double2(f[2 * i], f[2 * i + 1]);
+      (*bwd)[base + k] = make_double2(b[2 * i], b[2 * i + 1]);
+    }
   }
 }
 
