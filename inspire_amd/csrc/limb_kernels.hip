@@ -20,7 +20,6 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int kWaves = 4;        // waves per workgroup in the scan kernel
 constexpr int kMaxColsPerSplitNP1 = 65536;  // 2^16 / NP columns keep int32 sums exact
 
 __device__ inline int8_t OffsetByte(unsigned x) {
@@ -133,202 +132,350 @@ __device__ inline int BalancedDigit(u64 y, int c) {
   return dig;
 }
 
-// B operand tiles. For k-block kb, 16-column tile `tile` and plane p, column
-// j = (query ql, shift s) holds digit (s - p) of the query (zero when s < p),
-// so that plane p's product lands in the accumulator of total shift 8s.
-// Layout [kb][tile][plane][64 lanes][16 bytes]. One thread per (kb, tile, lane).
-__global__ void VectorsToTilesKernel(const u64* __restrict__ vec, int nq,
-                                     int stride, int len, int kblocks, int nl,
-                                     int qpt, int nt, int planes,
-                                     int8_t* __restrict__ btiles) {
-  const long long total = (long long)kblocks * nt * 64;
+// B operand tiles (plane-0 digits only). Layout [kb][group][tile][64 lanes]
+// [16 bytes]. Group g holds queries [g*qpg, (g+1)*qpg); its column j is
+// (query g*qpg + j / ns, shift s = j % ns) for j < qpg*ns and zero above; the
+// column holds balanced digit s of the query. Lane l of a tile holds column
+// l % 16 for the 16 k's 16*(l/16) .. +15 of the k-block. One thread per
+// (kb, group, tile, lane).
+__global__ void QueryDigitsKernel(const u64* __restrict__ vec, int nq, int stride,
+                                  int len, int kblocks, int ns, int qpg, int ngp,
+                                  int ctg, int8_t* __restrict__ btiles) {
+  const long long total = (long long)kblocks * ngp * ctg * 64;
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
        g += (long long)gridDim.x * blockDim.x) {
     const int lane = static_cast<int>(g & 63);
-    const long long t = g >> 6;
-    const int tile = static_cast<int>(t % nt);
-    const int kb = static_cast<int>(t / nt);
-    const int j = lane & 15;
-    const int ql = j / nl, sh = j % nl;
-    const int q = tile * qpt + ql;
-    const bool valid = ql < qpt && q < nq;
+    long long t = g >> 6;
+    const int ct = static_cast<int>(t % ctg);
+    t /= ctg;
+    const int grp = static_cast<int>(t % ngp);
+    const int kb = static_cast<int>(t / ngp);
+    const int j = ct * 16 + (lane & 15);
+    const int q = grp * qpg + j / ns;
+    const int s = j % ns;
+    const bool valid = j < qpg * ns && q < nq;
     const int kbase = kb * 64 + 16 * (lane >> 4);
-    u64 ys[16];
+    int8_t out[16];
+#pragma unroll
     for (int b = 0; b < 16; ++b) {
       const int k = kbase + b;
-      ys[b] = (valid && k < len) ? vec[(long long)q * stride + k] : 0ULL;
+      const u64 y = (valid && k < len) ? vec[(long long)q * stride + k] : 0ULL;
+      out[b] = static_cast<int8_t>(BalancedDigit(y, s));
     }
-    for (int p = 0; p < planes; ++p) {
-      int8_t out[16];
-      const int c = sh - p;
-      for (int b = 0; b < 16; ++b) {
-        out[b] = (c >= 0) ? static_cast<int8_t>(BalancedDigit(ys[b], c)) : int8_t{0};
-      }
-      int8_t* dst = btiles + ((((long long)kb * nt + tile) * planes + p) * 64 + lane) * 16;
-      *reinterpret_cast<int4*>(dst) = *reinterpret_cast<const int4*>(out);
-    }
+    int8_t* dst = btiles + (((((long long)kb * ngp + grp) * ctg + ct) * 64) + lane) * 16;
+    *reinterpret_cast<int4*>(dst) = *reinterpret_cast<const int4*>(out);
   }
 }
 
-// corr[q] = c_off * sum_k vec[q][k] mod 2^64. One block per vector.
-__global__ void VectorSumKernel(const u64* __restrict__ vec, int stride,
-                                int len, u64 c_off, u64* __restrict__ corr) {
-  __shared__ u64 part[256];
-  const int q = blockIdx.x;
+// corr[q] = c_off * sum_k vec[q][k] mod 2^64, in two passes: partial sums of
+// chunks, then one thread per query.
+__global__ void VectorPartialSumKernel(const u64* __restrict__ vec, int stride,
+                                       int len, int chunk, int nchunks,
+                                       u64* __restrict__ part) {
+  __shared__ u64 red[256];
+  const int q = blockIdx.y;
+  const int c = blockIdx.x;
+  const int k0 = c * chunk;
+  const int k1 = min(len, k0 + chunk);
   u64 s = 0;
-  for (int k = threadIdx.x; k < len; k += blockDim.x) s += vec[(long long)q * stride + k];
-  part[threadIdx.x] = s;
+  for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) s += vec[(long long)q * stride + k];
+  red[threadIdx.x] = s;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) corr[q] = c_off * part[0];
+  if (threadIdx.x == 0) part[(long long)q * nchunks + c] = red[0];
 }
 
-// The scan. A workgroup of 4 waves owns 4*R consecutive 16-row strips and a
-// k-range; wave w owns strips [strip0 + w*R, +R). Per 64-column block:
-//  * the T*NP B tiles of the block are staged in LDS (double buffered) by the
-//    whole workgroup and shared by its 4 waves;
-//  * each wave holds its R*NP A tiles in registers, loaded one block ahead;
-//  * plane p of strip r times B tile (t, p) accumulates into acc[r][t]: the
-//    B tile of plane p carries the query digits shifted by p, so one int32
-//    accumulator per (row, query, total shift) collects every (plane, digit)
-//    pair of that shift.
-// |acc| <= NP * 2^14 * columns, so a k-range of 2^16 / NP columns stays exact.
-template <int NP, int R, int T>
-__global__ __launch_bounds__(256) void LimbScanKernel(
-    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks,
-    int kb_per_split, const int8_t* __restrict__ btiles, int nt, int nq,
-    int nl, int qpt, int out_bits, u64* __restrict__ partial) {
-  __shared__ v4i bsh[2][T * NP * 64];
-  __shared__ u64 red[kWaves][16][16];
+__global__ void VectorSumFinalKernel(const u64* __restrict__ part, int nchunks, int nq,
+                                     u64 c_off, u64* __restrict__ corr) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  u64 s = 0;
+  for (int c = 0; c < nchunks; ++c) s += part[(long long)q * nchunks + c];
+  corr[q] = c_off * s;
+}
+
+// Plane p of the B operand is plane 0 with every column moved p places
+// right: column (query, s) of plane p holds digit s - p of the query. Within
+// a 16-column tile that is a DPP shift inside each 16-lane row (the 16 lanes
+// of a row are the 16 columns); the first p columns come from the previous
+// tile (row rotate). Columns with s < p belong to the previous query and are
+// zeroed by the caller's mask.
+template <int P>
+__device__ __forceinline__ v4i ShiftColumns(const v4i cur, const v4i prev) {
+  if constexpr (P == 0) {
+    (void)prev;
+    return cur;
+  } else {
+    v4i out;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int rot = __builtin_amdgcn_update_dpp(0, prev[e], 0x120 + P, 0xF, 0xF, false);
+      out[e] = __builtin_amdgcn_update_dpp(rot, cur[e], 0x110 + P, 0xF, 0xF, false);
+    }
+    return out;
+  }
+}
+
+__device__ __forceinline__ v4i Masked(const v4i x, bool keep) {
+  return keep ? x : v4i{0, 0, 0, 0};
+}
+
+// Column-validity bits of a lane: bit (t * 4 + p) is set when column
+// t*16 + lane%16 of a group has shift >= p (or lies in the padding).
+__device__ __forceinline__ unsigned ColumnMask(int ntiles, int np, int ns, int lane) {
+  unsigned m = 0;
+  for (int t = 0; t < ntiles; ++t) {
+    const int s = (t * 16 + (lane & 15)) % ns;
+    for (int p = 0; p < np; ++p) {
+      if (s >= p) m |= 1u << (t * 4 + p);
+    }
+  }
+  return m;
+}
+
+constexpr int kQMax = 16;  // queries per group (bounds the epilogue buffer)
+
+// Epilogue of a wave: acc[r][t] (16x16 MFMA layout: lane l, element v holds
+// row 4*(l/16) + v of strip r and column t*16 + l%16 of the wave's group)
+// -> partial[ks][q][row] = sum over shifts s of acc << 8s. The wave reduces
+// through its own LDS region red (nstrips*16 rows x kQMax queries).
+template <int NS, int NT>
+__device__ __forceinline__ void WaveEpilogue(const v4i (&acc)[NS][NT], u64* red, int lane,
+                                             int ns, int qpg, int grp, int nq, int rows,
+                                             long long row0, int ks, u64* partial) {
+  constexpr int kRows = NS * 16;
+  for (int i = lane; i < kRows * kQMax; i += 64) red[i] = 0;
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int j = t * 16 + (lane & 15);
+      if (j < qpg * ns) {
+        const int q = j / ns;
+        const int sh = 8 * (j - q * ns);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rl = r * 16 + 4 * (lane >> 4) + v;
+          atomicAdd(reinterpret_cast<unsigned long long*>(&red[rl * kQMax + q]),
+                    static_cast<unsigned long long>(static_cast<u64>(static_cast<i64>(acc[r][t][v])) << sh));
+        }
+      }
+    }
+  }
+  for (int i = lane; i < kRows * qpg; i += 64) {
+    const int q = i / kRows;
+    const int rl = i % kRows;
+    const int qg = grp * qpg + q;
+    const long long row = row0 + rl;
+    if (qg < nq && row < rows) {
+      partial[((long long)ks * nq + qg) * rows + row] = red[rl * kQMax + q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// NarrowScan: few queries, bound by the A stream. Each of the 4 waves owns R
+// strips and the workgroup's group of T tiles; the waves never synchronise.
+// A and B fragments are prefetched D-1 k-blocks ahead into rotating register
+// sets (the loop is unrolled by D so every set is a fixed register range).
+template <int NP, int R, int T, int D>
+__global__ __launch_bounds__(256) void NarrowScanKernel(
+    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps,
+    const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
+    u64* __restrict__ partial) {
+  __shared__ u64 red[4][R * 16 * kQMax];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int strip0 = (blockIdx.x * kWaves + wave) * R;
+  const int strip0 = (blockIdx.x * 4 + wave) * R;
   const int ks = blockIdx.y;
-  const int nt0 = blockIdx.z * T;
-  const int kb0 = ks * kb_per_split;
-  const int kb1 = min(kblocks, kb0 + kb_per_split);
+  const int grp = blockIdx.z;
+  const int kb0 = ks * kbps;
+  const int nkb = min(kblocks, kb0 + kbps) - kb0;
   const v4i* A = reinterpret_cast<const v4i*>(tiles);
-  const v4i* Bt = reinterpret_cast<const v4i*>(btiles);
-  constexpr int kBVecs = T * NP * 64;             // v4i per staged block
-  constexpr int kBPerThread = (kBVecs + 255) / 256;
-
+  const v4i* Bg = reinterpret_cast<const v4i*>(btiles) + (long long)grp * T * 64 + lane;
+  const v4i* Ar[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    Ar[r] = A + (long long)min(strip0 + r, strips - 1) * kblocks * NP * 64 + lane;
+  }
+  const unsigned cmask = ColumnMask(T, NP, ns, lane);
   v4i acc[R][T];
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
-    for (int tt = 0; tt < T; ++tt) acc[r][tt] = v4i{0, 0, 0, 0};
-
-  auto load_a = [&](int kb, v4i (&a)[R][NP]) {
+    for (int t = 0; t < T; ++t) acc[r][t] = v4i{0, 0, 0, 0};
+  v4i a[D][R][NP];
+  v4i b[D][T];
+  // Steps past the k-range read A's last k-block and B's all-zero block
+  // `kblocks`, so they add nothing.
+  auto load = [&](int i, v4i (&as)[R][NP], v4i (&bs)[T]) {
+    const int kb = kb0 + min(i, nkb - 1);
+    const int kbb = i < nkb ? kb0 + i : kblocks;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int strip = strip0 + r;
+    for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        a[r][p] = (strip < strips)
-                      ? A[(((long long)strip * kblocks + kb) * NP + p) * 64 + lane]
-                      : v4i{0, 0, 0, 0};
+      for (int p = 0; p < NP; ++p) as[r][p] = Ar[r][((long long)kb * NP + p) * 64];
+#pragma unroll
+    for (int t = 0; t < T; ++t) bs[t] = Bg[((long long)kbb * ngp * T + t) * 64];
+  };
+  auto compute = [&](const v4i (&as)[R][NP], const v4i (&b0)[T]) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      v4i bp[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const v4i prev = t > 0 ? b0[t > 0 ? t - 1 : 0] : v4i{0, 0, 0, 0};
+        v4i x;
+        if (p == 0) x = b0[t];
+        else if (p == 1) x = ShiftColumns<1>(b0[t], prev);
+        else if (p == 2) x = ShiftColumns<2>(b0[t], prev);
+        else x = ShiftColumns<3>(b0[t], prev);
+        bp[t] = p == 0 ? x : Masked(x, (cmask >> (t * 4 + p)) & 1u);
       }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+          acc[r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(as[r][p], bp[t], acc[r][t], 0, 0, 0);
     }
   };
-  // B tiles of block kb for tiles [nt0, nt0 + T): contiguous in btiles when
-  // all T tiles exist; missing tiles read as zero.
-  auto load_b = [&](int kb, v4i (&tmp)[kBPerThread]) {
 #pragma unroll
-    for (int i = 0; i < kBPerThread; ++i) {
-      const int v = threadIdx.x + i * 256;
-      const int tt = v / (NP * 64);
-      v4i x = v4i{0, 0, 0, 0};
-      if (v < kBVecs && nt0 + tt < nt) {
-        x = Bt[((long long)kb * nt + nt0) * NP * 64 + v];
-      }
-      tmp[i] = x;
-    }
-  };
-  auto store_b = [&](int buf, const v4i (&tmp)[kBPerThread]) {
+  for (int d = 0; d < D - 1; ++d) load(d, a[d], b[d]);
+  for (int i = 0; i < nkb; i += D) {
 #pragma unroll
-    for (int i = 0; i < kBPerThread; ++i) {
-      const int v = threadIdx.x + i * 256;
-      if (v < kBVecs) bsh[buf][v] = tmp[i];
-    }
-  };
-
-  if (kb0 < kb1) {
-    v4i a_cur[R][NP], a_next[R][NP];
-    v4i btmp[kBPerThread];
-    load_a(kb0, a_cur);
-    load_b(kb0, btmp);
-    store_b(0, btmp);
-    __syncthreads();
-    for (int kb = kb0; kb < kb1; ++kb) {
-      const int buf = (kb - kb0) & 1;
-      const bool more = kb + 1 < kb1;
-      if (more) {
-        load_a(kb + 1, a_next);
-        load_b(kb + 1, btmp);
-      }
-#pragma unroll
-      for (int tt = 0; tt < T; ++tt) {
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          const v4i b = bsh[buf][(tt * NP + p) * 64 + lane];
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            acc[r][tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a_cur[r][p], b, acc[r][tt],
-                                                               0, 0, 0);
-          }
-        }
-      }
-      if (more) store_b(buf ^ 1, btmp);
-      __syncthreads();
-      if (more) {
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-          for (int p = 0; p < NP; ++p) a_cur[r][p] = a_next[r][p];
-      }
+    for (int d = 0; d < D; ++d) {
+      const int nx = (d + D - 1) % D;
+      load(i + d + D - 1, a[nx], b[nx]);
+      compute(a[d], b[d]);
     }
   }
-
-  // Epilogue. MFMA 16x16 output layout: lane l, element v holds
-  // C[row = 4*(l/16) + v][col = l % 16]; col = (query, shift).
-  const int col = lane & 15;
-  const int sh = 8 * (col % nl);
-  const bool col_ok = col < qpt * nl && sh < out_bits;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-#pragma unroll
-    for (int tt = 0; tt < T; ++tt) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const u64 val = col_ok ? static_cast<u64>(static_cast<i64>(acc[r][tt][v])) << sh : 0ULL;
-        red[wave][4 * (lane >> 4) + v][col] = val;
-      }
-      __syncthreads();
-      if (lane < 16 * qpt) {
-        const int row = lane & 15;
-        const int ql = lane >> 4;
-        u64 sum = 0;
-        for (int cc = 0; cc < nl; ++cc) sum += red[wave][row][ql * nl + cc];
-        const int q = (nt0 + tt) * qpt + ql;
-        const long long grow = (long long)(strip0 + r) * 16 + row;
-        if (q < nq && grow < rows) {
-          partial[((long long)ks * nq + q) * rows + grow] = sum;
-        }
-      }
-      __syncthreads();
-    }
-  }
+  WaveEpilogue<R, T>(acc, red[wave], lane, ns, qpg, grp, nq, rows,
+                     (long long)strip0 * 16, ks, partial);
 }
 
-int PickT(int nt) {
-  if (nt <= 1) return 1;
-  if (nt <= 2) return 2;
-  if (nt <= 4) return 4;
-  if (nt <= 8) return 8;
-  return 16;
+// ---------------------------------------------------------------------------
+// WideScan: batches, bound by MFMA. A workgroup owns RS = 8 strips (128 rows)
+// and 4 consecutive groups, one per wave (CT tiles = up to 16 queries each);
+// per k-block every wave runs 8 x CT x NP MFMAs.
+//  * A (shared by the 4 waves) is staged through a 2-slot LDS ring: each wave
+//    loads a quarter of the next k-blocks' A into AD rotating register sets
+//    and writes one k-block per step.
+//  * B (private to a wave) goes straight to registers, 3 sets, issued two
+//    k-blocks ahead and before that step's A loads, so waiting for B never
+//    drains the A prefetch (vector loads complete in issue order).
+//  * The per-step barrier is s_barrier after an LDS-only wait: HIP's
+//    __syncthreads also waits for every outstanding global load, which would
+//    drain the prefetch at every k-block.
+__device__ __forceinline__ void LdsBarrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <int NP, int CT, int AD>
+__global__ __launch_bounds__(256) void WideScanKernel(
+    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps,
+    const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
+    u64* __restrict__ partial) {
+  constexpr int RS = 8;
+  constexpr int AF = RS * NP / 4;  // A fragments staged per wave per k-block
+  constexpr int BD = 3;
+  static_assert(AD == 2 || AD == 3, "A staging depth");
+  __shared__ v4i aring[2][RS * NP * 64];
+  __shared__ u64 red[4][RS * 16 * kQMax];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int strip0 = blockIdx.x * RS;
+  const int ks = blockIdx.y;
+  const int grp = blockIdx.z * 4 + wave;
+  const int kb0 = ks * kbps;
+  const int nkb = min(kblocks, kb0 + kbps) - kb0;
+  const v4i* A = reinterpret_cast<const v4i*>(tiles);
+  const v4i* Bg = reinterpret_cast<const v4i*>(btiles) + (long long)grp * CT * 64 + lane;
+  // Fragment f = wave*AF + jf of a k-block is (strip f / NP, plane f % NP).
+  const v4i* Af[AF];
+#pragma unroll
+  for (int jf = 0; jf < AF; ++jf) {
+    const int f = wave * AF + jf;
+    Af[jf] = A + ((long long)min(strip0 + f / NP, strips - 1) * kblocks * NP + f % NP) * 64 + lane;
+  }
+  const unsigned cmask = ColumnMask(CT, NP, ns, lane);
+  v4i acc[RS][CT];
+#pragma unroll
+  for (int r = 0; r < RS; ++r)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[r][c] = v4i{0, 0, 0, 0};
+  v4i as[AD][AF];
+  v4i bs[BD][CT];
+  auto load_a = [&](int i, v4i (&dst)[AF]) {
+    const int kb = kb0 + min(i, nkb - 1);
+#pragma unroll
+    for (int jf = 0; jf < AF; ++jf) dst[jf] = Af[jf][(long long)kb * NP * 64];
+  };
+  auto load_b = [&](int i, v4i (&dst)[CT]) {
+    const int kb = i < nkb ? kb0 + i : kblocks;  // all-zero block past the range
+#pragma unroll
+    for (int c = 0; c < CT; ++c) dst[c] = Bg[((long long)kb * ngp * CT + c) * 64];
+  };
+  auto store_a = [&](int slot, const v4i (&src)[AF]) {
+#pragma unroll
+    for (int jf = 0; jf < AF; ++jf) aring[slot][(wave * AF + jf) * 64 + lane] = src[jf];
+  };
+  auto compute = [&](int slot, const v4i (&b0)[CT]) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      v4i bp[CT];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        const v4i prev = c > 0 ? b0[c > 0 ? c - 1 : 0] : v4i{0, 0, 0, 0};
+        v4i x;
+        if (p == 0) x = b0[c];
+        else if (p == 1) x = ShiftColumns<1>(b0[c], prev);
+        else if (p == 2) x = ShiftColumns<2>(b0[c], prev);
+        else x = ShiftColumns<3>(b0[c], prev);
+        bp[c] = p == 0 ? x : Masked(x, (cmask >> (c * 4 + p)) & 1u);
+      }
+#pragma unroll
+      for (int r = 0; r < RS; ++r) {
+        const v4i af = aring[slot][(r * NP + p) * 64 + lane];
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+          acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bp[c], acc[r][c], 0, 0, 0);
+      }
+    }
+  };
+  // Prologue: A(0) into slot 0; B(0), A(1), B(1), A(2..AD-1) in flight.
+  load_a(0, as[0]);
+  store_a(0, as[0]);
+  load_b(0, bs[0]);
+#pragma unroll
+  for (int d = 1; d < AD; ++d) {
+    load_a(d, as[d % AD]);
+    if (d == 1) load_b(1, bs[1]);
+  }
+  LdsBarrier();
+  // Step i: issue B(i+2), A(i+AD); compute on slot i%2 with B(i); write
+  // A(i+1) into the other slot; barrier.
+  auto step = [&](int i, int u) {
+    const int bnext = (u + 2) % BD;
+    const int anext = (u + AD) % AD;
+    load_b(i + 2, bs[bnext]);
+    load_a(i + AD, as[anext]);
+    compute(u % 2, bs[u % BD]);
+    store_a((u + 1) % 2, as[(u + 1) % AD]);
+    LdsBarrier();
+  };
+  for (int i = 0; i < nkb; i += 6) {
+    step(i + 0, 0);
+    step(i + 1, 1);
+    step(i + 2, 2);
+    step(i + 3, 3);
+    step(i + 4, 4);
+    step(i + 5, 5);
+  }
+  WaveEpilogue<RS, CT>(acc, red[wave], lane, ns, qpg, grp, nq, rows,
+                       (long long)strip0 * 16, ks, partial);
 }
 
 int GridBlocks(long long total, int threads) {
@@ -338,36 +485,46 @@ int GridBlocks(long long total, int threads) {
   return static_cast<int>(b);
 }
 
-template <int NP, int R, int T>
-hipError_t LaunchScanT(const int8_t* tiles, const LimbLayout& L,
-                       const int8_t* btiles, int nt, int nq, int nl, int qpt,
-                       int out_bits, int ksplit, u64* partial, hipStream_t s) {
-  const int kbps = (L.kblocks + ksplit - 1) / ksplit;
-  dim3 grid((L.strips + kWaves * R - 1) / (kWaves * R), ksplit, (nt + T - 1) / T);
-  hipLaunchKernelGGL((LimbScanKernel<NP, R, T>), grid, dim3(256), 0, s, tiles,
-                     L.strips, L.rows, L.kblocks, kbps, btiles, nt, nq, nl, qpt,
-                     out_bits, partial);
+constexpr int kNarrowR = 4;
+constexpr int kNarrowD = 4;
+
+template <int NP, int T>
+hipError_t LaunchNarrowT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
+                         const int8_t* btiles, u64* partial, hipStream_t s) {
+  dim3 grid((L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR), P.ksplit, P.ngp);
+  hipLaunchKernelGGL((NarrowScanKernel<NP, kNarrowR, T, kNarrowD>), grid, dim3(256), 0, s,
+                     tiles, L.strips, L.rows, L.kblocks, P.kb_per_split, btiles, P.ngp,
+                     P.nq, P.ns, P.qpg, partial);
   return hipGetLastError();
 }
 
-// R strips per wave; R * T * 4 accumulator registers stay <= 128 so the
-// wave fits its registers without spilling.
+template <int NP, int CT>
+hipError_t LaunchWideT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
+                       const int8_t* btiles, u64* partial, hipStream_t s) {
+  constexpr int AD = (NP >= 4 || CT >= 7) ? 2 : 3;
+  dim3 grid((L.strips + 7) / 8, P.ksplit, P.ngp / 4);
+  hipLaunchKernelGGL((WideScanKernel<NP, CT, AD>), grid, dim3(256), 0, s, tiles, L.strips,
+                     L.rows, L.kblocks, P.kb_per_split, btiles, P.ngp, P.nq, P.ns, P.qpg,
+                     partial);
+  return hipGetLastError();
+}
+
 template <int NP>
-hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L,
-                        const int8_t* btiles, int nt, int nq, int nl, int qpt,
-                        int out_bits, int ksplit, u64* partial, hipStream_t s) {
-  switch (PickT(nt)) {
-    case 1:
-      return LaunchScanT<NP, 4, 1>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
-    case 2:
-      return LaunchScanT<NP, 4, 2>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
-    case 4:
-      return LaunchScanT<NP, 4, 4>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
-    case 8:
-      return LaunchScanT<NP, 4, 8>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
-    default:
-      return LaunchScanT<NP, 2, 16>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit,
-                                    partial, s);
+hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
+                        const int8_t* btiles, u64* partial, hipStream_t s) {
+  if (!P.wide) {
+    switch (P.ctg) {
+      case 1: return LaunchNarrowT<NP, 1>(tiles, L, P, btiles, partial, s);
+      case 2: return LaunchNarrowT<NP, 2>(tiles, L, P, btiles, partial, s);
+      case 4: return LaunchNarrowT<NP, 4>(tiles, L, P, btiles, partial, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (P.ctg) {
+    case 2: return LaunchWideT<NP, 2>(tiles, L, P, btiles, partial, s);
+    case 4: return LaunchWideT<NP, 4>(tiles, L, P, btiles, partial, s);
+    case 7: return LaunchWideT<NP, 7>(tiles, L, P, btiles, partial, s);
+    default: return hipErrorInvalidValue;
   }
 }
 
@@ -413,52 +570,95 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
   return hipGetLastError();
 }
 
-hipError_t LaunchVectorsToTiles(const u64* vec, int nq, int stride, int len,
-                                int kblocks, int nl, int qpt, int planes, u64 c_off,
-                                int8_t* btiles, u64* corr, hipStream_t s) {
-  const int nt = (nq + qpt - 1) / qpt;
-  const long long total = (long long)kblocks * nt * 64;
-  hipLaunchKernelGGL(VectorsToTilesKernel, dim3(GridBlocks(total, 256)),
-                     dim3(256), 0, s, vec, nq, stride, len, kblocks, nl, qpt,
-                     nt, planes, btiles);
+int QueriesPerGroup(int tiles, int ns) {
+  const int q = tiles * 16 / ns;
+  return q < kQMax ? q : kQMax;
+}
+
+// Tile widths the kernels are instantiated for.
+constexpr int kNarrowTiles[] = {1, 2, 4};
+constexpr int kWideTiles[] = {2, 4, 7};
+
+ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries) {
+  ScanPlan P{};
+  P.nq = nq;
+  P.ns = (out_bits + 7) / 8;
+  const int ns = P.ns;
+  // Narrow when the queries fit one group of at most 4 tiles and the batch is
+  // below the wide threshold.
+  int narrow_tiles = 0;
+  for (int t : kNarrowTiles) {
+    if (QueriesPerGroup(t, ns) >= nq) {
+      narrow_tiles = t;
+      break;
+    }
+  }
+  if (narrow_tiles > 0 && nq < wide_min_queries) {
+    P.wide = 0;
+    P.ctg = narrow_tiles;
+  } else {
+    P.wide = 1;
+    P.ctg = kWideTiles[2];
+    for (int t : kWideTiles) {
+      if (4 * QueriesPerGroup(t, ns) >= nq) {
+        P.ctg = t;
+        break;
+      }
+    }
+  }
+  P.qpg = QueriesPerGroup(P.ctg, ns);
+  const int groups = (nq + P.qpg - 1) / P.qpg;
+  P.ngp = P.wide ? (groups + 3) / 4 * 4 : groups;
+  // K-split: exact int32 sums need at most 65536 / planes columns per split;
+  // beyond that, enough workgroups to occupy every CU a few times over.
+  const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
+  const int row_wgs = P.wide ? (L.strips + 7) / 8 : (L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR);
+  const int z = P.wide ? P.ngp / 4 : P.ngp;
+  const int target = P.wide ? 512 : 2048;
+  int ks = (target + row_wgs * z - 1) / (row_wgs * z);
+  const int min_split = (L.kblocks + max_kb - 1) / max_kb;
+  if (ks < min_split) ks = min_split;
+  if (ks > L.kblocks) ks = L.kblocks;
+  if (ks < 1) ks = 1;
+  P.kb_per_split = (L.kblocks + ks - 1) / ks;
+  P.ksplit = (L.kblocks + P.kb_per_split - 1) / P.kb_per_split;
+  P.btile_bytes = static_cast<size_t>(L.kblocks + 1) * P.ngp * P.ctg * kTileBytes;
+  return P;
+}
+
+hipError_t LaunchQueryDigits(const u64* vec, int stride, int len, const LimbLayout& L,
+                             const ScanPlan& P, int8_t* btiles, hipStream_t s) {
+  // kblocks + 1 blocks: the last one is all zero (its k's are >= len).
+  const long long total = (long long)(L.kblocks + 1) * P.ngp * P.ctg * 64;
+  hipLaunchKernelGGL(QueryDigitsKernel, dim3(GridBlocks(total, 256)), dim3(256), 0, s, vec,
+                     P.nq, stride, len, L.kblocks + 1, P.ns, P.qpg, P.ngp, P.ctg, btiles);
+  return hipGetLastError();
+}
+
+int VectorSumScratchWords(int nq) { return nq * kVectorSumChunks; }
+
+hipError_t LaunchVectorSums(const u64* vec, int nq, int stride, int len, u64 c_off,
+                            u64* scratch, u64* corr, hipStream_t s) {
+  const int chunk = (len + kVectorSumChunks - 1) / kVectorSumChunks;
+  const int nchunks = chunk > 0 ? (len + chunk - 1) / chunk : 1;
+  hipLaunchKernelGGL(VectorPartialSumKernel, dim3(nchunks, nq), dim3(256), 0, s, vec,
+                     stride, len, chunk > 0 ? chunk : 1, nchunks, scratch);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (corr != nullptr) {
-    hipLaunchKernelGGL(VectorSumKernel, dim3(nq), dim3(256), 0, s, vec, stride,
-                       len, c_off, corr);
-    e = hipGetLastError();
-  }
-  return e;
+  hipLaunchKernelGGL(VectorSumFinalKernel, dim3((nq + 63) / 64), dim3(64), 0, s, scratch,
+                     nchunks, nq, c_off, corr);
+  return hipGetLastError();
 }
 
-int LimbScanKSplit(const LimbLayout& L, int nq) {
-  (void)nq;
-  const int R = 4;  // strips per wave of the common configurations
-  const int wgs = (L.strips + kWaves * R - 1) / (kWaves * R);
-  int ksplit = (1024 + wgs - 1) / wgs;
-  const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
-  const int min_split = (L.kblocks + max_kb - 1) / max_kb;
-  if (ksplit < min_split) ksplit = min_split;
-  if (ksplit > L.kblocks) ksplit = L.kblocks;
-  if (ksplit < 1) ksplit = 1;
-  const int kbps = (L.kblocks + ksplit - 1) / ksplit;
-  return (L.kblocks + kbps - 1) / kbps;
-}
-
-hipError_t LaunchLimbScan(const int8_t* tiles, const LimbLayout& L,
-                          const int8_t* btiles, int nq, int nl, int qpt,
-                          int out_bits, int ksplit, u64* partial,
-                          hipStream_t s) {
-  const int nt = (nq + qpt - 1) / qpt;
+hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
+                      const int8_t* btiles, u64* partial, hipStream_t s) {
+  if (P.nq <= 0) return hipSuccess;
+  if (P.qpg < 1 || P.ngp < 1) return hipErrorInvalidValue;
   switch (L.planes) {
-    case 1:
-      return LaunchScanNP<1>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
-    case 2:
-      return LaunchScanNP<2>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
-    case 4:
-      return LaunchScanNP<4>(tiles, L, btiles, nt, nq, nl, qpt, out_bits, ksplit, partial, s);
-    default:
-      return hipErrorInvalidValue;
+    case 1: return LaunchScanNP<1>(tiles, L, P, btiles, partial, s);
+    case 2: return LaunchScanNP<2>(tiles, L, P, btiles, partial, s);
+    case 4: return LaunchScanNP<4>(tiles, L, P, btiles, partial, s);
+    default: return hipErrorInvalidValue;
   }
 }
 

@@ -95,19 +95,39 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
                              int cols, int row0, int nrows,
                              const LimbLayout& L, int8_t* tiles,
                              hipStream_t s);
-// Builds B-operand tiles for nq vectors (vec + q*stride, length len each,
-// zero beyond len) with nl digits, qpt vectors per 16-column tile and one
-// digit-shifted copy per plane, and corr[q] = c_off * sum(vec_q) (mod 2^64).
-hipError_t LaunchVectorsToTiles(const u64* vec, int nq, int stride, int len,
-                                int kblocks, int nl, int qpt, int planes, u64 c_off,
-                                int8_t* btiles, u64* corr, hipStream_t s);
+// How one limb product over nq query vectors is launched. Columns of the
+// int8 product are (query, shift s) pairs, s < ns = ceil(out_bits / 8): column
+// (q, s) collects plane p times digit s - p of query q. Queries are grouped,
+// qpg per group of ctg 16-column tiles; a wave of the scan owns one group.
+struct ScanPlan {
+  int wide;          // 0: NarrowScan (A-stream bound), 1: WideScan (MFMA bound)
+  int nq;            // query vectors
+  int ns;            // shifts per query
+  int ctg;           // tiles per group
+  int qpg;           // queries per group
+  int ngp;           // groups in the B layout (padded for WideScan)
+  int ksplit;        // k-ranges (exact int32 sums and occupancy)
+  int kb_per_split;
+  size_t btile_bytes;
+};
+
+// wide_min_queries: batches of at least this many queries use WideScan even
+// when they would fit NarrowScan.
+ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries);
+// Plane-0 digit tiles of nq vectors (vec + q*stride, length len each, zero
+// beyond len): layout [kb][group][tile][64 lanes][16 bytes].
+hipError_t LaunchQueryDigits(const u64* vec, int stride, int len, const LimbLayout& L,
+                             const ScanPlan& P, int8_t* btiles, hipStream_t s);
+// corr[q] = c_off * sum(vec_q) mod 2^64; scratch holds
+// VectorSumScratchWords(nq) words.
+constexpr int kVectorSumChunks = 64;
+int VectorSumScratchWords(int nq);
+hipError_t LaunchVectorSums(const u64* vec, int nq, int stride, int len, u64 c_off,
+                            u64* scratch, u64* corr, hipStream_t s);
 // partial[ks][q][row] = sum over the ks-th k-range of the limb product,
-// combined over (plane, digit) pairs with 8(p+c) < out_bits.
-hipError_t LaunchLimbScan(const int8_t* tiles, const LimbLayout& L,
-                          const int8_t* btiles, int nq, int nl, int qpt,
-                          int out_bits, int ksplit, u64* partial,
-                          hipStream_t s);
-int LimbScanKSplit(const LimbLayout& L, int nq);
+// combined over (plane, digit) pairs with shift < 8 * ns.
+hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
+                      const int8_t* btiles, u64* partial, hipStream_t s);
 hipError_t LaunchCopyProbe(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // ---------------------------------------------------------------------------

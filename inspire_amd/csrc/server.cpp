@@ -174,14 +174,15 @@ using namespace pirg;
 struct pirg_server {
   pirg_params P{};
   int d = 0, log_d = 0, N = 0, log_N = 0, t = 0, ell = 0, k = 0, e = 0;
-  int r = 0, K = 0, M = 0, nl = 0, qpt = 0, log_q = 0;
+  int r = 0, K = 0, M = 0, log_q = 0;
   u64 qmask = 0;
   FftDims F{};
   ChunkRule fin_u{}, fin_v{}, ext_u{}, ext_v{};
   PairList fin_pairs{}, ext_pairs{};
   Error deferred;  // error ProcessResponse must return (chunk validation)
   LimbLayout dbL{}, pkL{};
-  int ks_db = 1, ks_pk = 1;
+  // Batches of at least this many requests use the MFMA-bound WideScan.
+  int wide_min = 10;
   DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -254,8 +255,6 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
   S->M = S->ell * S->t * d;
   S->log_q = P.log_q;
   S->qmask = P.log_q >= 64 ? ~0ULL : ((1ULL << P.log_q) - 1);
-  S->nl = (P.log_q + 7) / 8;
-  S->qpt = 16 / S->nl;
   // FinalizeMatrixPack's InnerProduct (lwes_to_rlwe.cc:472-474).
   Error er = MakeRules(S->k, d, P.pack_log_digit, P.log_q, ChunkFor(S->k, d), true,
                        &S->fin_u, &S->fin_v);
@@ -397,8 +396,6 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
                      });
     if (!er.ok()) return er;
   }
-  S->ks_db = LimbScanKSplit(S->dbL, 1);
-  S->ks_pk = LimbScanKSplit(S->pkL, 1);
 
   // Small resident polynomials.
   const int lt = S->ell * S->t;
@@ -460,8 +457,18 @@ Error EnsureWorkspace(pirg_server* S, int cap) {
   if (W.cap >= cap) return Ok();
   const size_t d = S->d, N = S->N;
   const int lt = S->ell * S->t;
-  const int nt_db = (cap + S->qpt - 1) / S->qpt;
   const size_t nq = cap;
+  // Scan buffers: the largest plan over every batch size up to cap.
+  size_t bt_db = 0, bt_pk = 0, part_db = 0, part_pk = 0;
+  for (int n = 1; n <= cap; ++n) {
+    const ScanPlan pd = MakeScanPlan(S->dbL, n, S->log_q, S->wide_min);
+    const ScanPlan pp = MakeScanPlan(S->pkL, n, S->log_q, S->wide_min);
+    bt_db = std::max(bt_db, pd.btile_bytes);
+    bt_pk = std::max(bt_pk, pp.btile_bytes);
+    part_db = std::max(part_db, static_cast<size_t>(pd.ksplit) * n);
+    part_pk = std::max(part_pk, static_cast<size_t>(pp.ksplit) * n);
+  }
+  const size_t corr_words = nq + VectorSumScratchWords(cap);
   struct Item {
     DevBuf* b;
     size_t bytes;
@@ -473,12 +480,12 @@ Error EnsureWorkspace(pirg_server* S, int cap) {
       {&W.fd, nq * S->K * 8},
       {&W.sd, nq * 2 * S->e * d * 8},
       {&W.pk, nq * 2 * S->k * d * 8},
-      {&W.bt_db, static_cast<size_t>(S->dbL.kblocks) * nt_db * S->dbL.planes * kTileBytes},
-      {&W.bt_pk, static_cast<size_t>(S->pkL.kblocks) * nt_db * S->pkL.planes * kTileBytes},
-      {&W.corr_db, nq * 8},
-      {&W.corr_pk, nq * 8},
-      {&W.db_partial, static_cast<size_t>(S->ks_db) * nq * S->M * 8},
-      {&W.pk_partial, static_cast<size_t>(S->ks_pk) * nq * S->M * 8},
+      {&W.bt_db, bt_db},
+      {&W.bt_pk, bt_pk},
+      {&W.corr_db, corr_words * 8},
+      {&W.corr_pk, corr_words * 8},
+      {&W.db_partial, part_db * S->M * 8},
+      {&W.pk_partial, part_pk * S->M * 8},
       {&W.yh_spec, nq * S->k * nv_f * N * sizeof(double2)},
       {&W.sdb_spec, nq * 2 * S->e * nv_e * N * sizeof(double2)},
       {&W.d_spec, nq * S->ell * 2 * S->e * nu_e * N * sizeof(double2)},
@@ -590,21 +597,24 @@ Error RunBatch(pirg_server* S, int nq) {
   unsigned long long* margin = S->margin.as<unsigned long long>();
   PIRG_HIP(hipEventRecord(S->ev[0], st));
   // DB product (pir_server.cc:215-229).
-  PIRG_HIP(LaunchVectorsToTiles(W.fd.as<u64>(), nq, S->K, S->K, S->dbL.kblocks, S->nl,
-                                S->qpt, S->dbL.planes, S->dbL.c_off, W.bt_db.as<int8_t>(),
-                                W.corr_db.as<u64>(), st));
+  const ScanPlan pd = MakeScanPlan(S->dbL, nq, S->log_q, S->wide_min);
+  const ScanPlan pp = MakeScanPlan(S->pkL, nq, S->log_q, S->wide_min);
+  PIRG_HIP(LaunchQueryDigits(W.fd.as<u64>(), S->K, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
+  PIRG_HIP(LaunchVectorSums(W.fd.as<u64>(), nq, S->K, S->K, S->dbL.c_off,
+                            W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
   PIRG_HIP(hipEventRecord(S->kev[0], st));
-  PIRG_HIP(LaunchLimbScan(S->db_tiles.as<int8_t>(), S->dbL, W.bt_db.as<int8_t>(), nq,
-                          S->nl, S->qpt, S->log_q, S->ks_db, W.db_partial.as<u64>(), st));
+  PIRG_HIP(LaunchScan(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
+                      W.db_partial.as<u64>(), st));
   PIRG_HIP(hipEventRecord(S->kev[1], st));
   PIRG_HIP(hipEventRecord(S->ev[1], st));
   // Pack product (pir_server.cc:237-248): y = concat(y_vec_g).
-  PIRG_HIP(LaunchVectorsToTiles(W.pk.as<u64>(), nq, 2 * S->k * d, S->k * d,
-                                S->pkL.kblocks, S->nl, S->qpt, S->pkL.planes, S->pkL.c_off,
-                                W.bt_pk.as<int8_t>(), W.corr_pk.as<u64>(), st));
+  PIRG_HIP(LaunchQueryDigits(W.pk.as<u64>(), 2 * S->k * d, S->k * d, S->pkL, pp,
+                              W.bt_pk.as<int8_t>(), st));
+  PIRG_HIP(LaunchVectorSums(W.pk.as<u64>(), nq, 2 * S->k * d, S->k * d, S->pkL.c_off,
+                            W.corr_pk.as<u64>() + nq, W.corr_pk.as<u64>(), st));
   PIRG_HIP(hipEventRecord(S->kev[2], st));
-  PIRG_HIP(LaunchLimbScan(S->pk_tiles.as<int8_t>(), S->pkL, W.bt_pk.as<int8_t>(), nq,
-                          S->nl, S->qpt, S->log_q, S->ks_pk, W.pk_partial.as<u64>(), st));
+  PIRG_HIP(LaunchScan(S->pk_tiles.as<int8_t>(), S->pkL, pp, W.bt_pk.as<int8_t>(),
+                      W.pk_partial.as<u64>(), st));
   PIRG_HIP(hipEventRecord(S->kev[3], st));
   // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279).
   PIRG_HIP(LaunchSpectra(W.pk.as<u64>() + static_cast<size_t>(S->k) * d, nq * S->k, S->k,
@@ -621,10 +631,10 @@ Error RunBatch(pirg_server* S, int nq) {
   }
   FinalizeArgs FA{};
   FA.db_partial = W.db_partial.as<u64>();
-  FA.ksplit_db = S->ks_db;
+  FA.ksplit_db = pd.ksplit;
   FA.corr_db = W.corr_db.as<u64>();
   FA.pk_partial = W.pk_partial.as<u64>();
-  FA.ksplit_pk = S->ks_pk;
+  FA.ksplit_pk = pp.ksplit;
   FA.corr_pk = W.corr_pk.as<u64>();
   FA.fin_partial = W.fin_partial.as<u64>();
   FA.npairs = S->fin_pairs.count;
@@ -759,6 +769,12 @@ Error CreateFromParts(const pirg_params* params, const pirg_preprocessed_view* v
   std::unique_ptr<pirg_server> S(new pirg_server());
   Error er = DeriveServer(*params, S.get());
   if (!er.ok()) return er;
+  // PIRG_WIDE_MIN sets the batch size from which the DB and pack products
+  // use the MFMA-bound WideScan (tests force either kernel with it).
+  if (const char* env = std::getenv("PIRG_WIDE_MIN")) {
+    const int v = std::atoi(env);
+    if (v > 0) S->wide_min = v;
+  }
   er = ValidateView(S.get(), v);
   if (!er.ok()) return er;
   er = CheckDevice();
@@ -796,10 +812,10 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
   if ((er = tiles.alloc(static_cast<size_t>(L.strips) * L.kblocks * planes * kTileBytes)).code) return er;
   if ((er = src.alloc(static_cast<size_t>(rows) * wpr * 8)).code) return er;
   if ((er = v.alloc(static_cast<size_t>(cols) * 8)).code) return er;
-  const int nl = 8, qpt = 2;
-  if ((er = bt.alloc(static_cast<size_t>(L.kblocks) * planes * kTileBytes)).code) return er;
-  if ((er = corr.alloc(8)).code) return er;
-  const int ks = LimbScanKSplit(L, 1);
+  const ScanPlan plan = MakeScanPlan(L, 1, 64, 1 << 30);
+  const int ks = plan.ksplit;
+  if ((er = bt.alloc(plan.btile_bytes)).code) return er;
+  if ((er = corr.alloc((1 + VectorSumScratchWords(1)) * 8)).code) return er;
   if ((er = partial.alloc(static_cast<size_t>(ks) * rows * 8)).code) return er;
   PIRG_HIP(hipMemcpy(src.p, cdata, static_cast<size_t>(rows) * wpr * 8, hipMemcpyHostToDevice));
   PIRG_HIP(hipMemcpy(v.p, vec, static_cast<size_t>(cols) * 8, hipMemcpyHostToDevice));
@@ -808,10 +824,10 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
   } else {
     PIRG_HIP(LaunchPackToTiles(src.p, 4, rows, cols, 0, rows, L, tiles.as<int8_t>(), s));
   }
-  PIRG_HIP(LaunchVectorsToTiles(v.as<u64>(), 1, cols, cols, L.kblocks, nl, qpt, planes,
-                                c_off, bt.as<int8_t>(), corr.as<u64>(), s));
-  PIRG_HIP(LaunchLimbScan(tiles.as<int8_t>(), L, bt.as<int8_t>(), 1, nl, qpt, 64, ks,
-                          partial.as<u64>(), s));
+  PIRG_HIP(LaunchQueryDigits(v.as<u64>(), cols, cols, L, plan, bt.as<int8_t>(), s));
+  PIRG_HIP(LaunchVectorSums(v.as<u64>(), 1, cols, cols, c_off, corr.as<u64>() + 1,
+                            corr.as<u64>(), s));
+  PIRG_HIP(LaunchScan(tiles.as<int8_t>(), L, plan, bt.as<int8_t>(), partial.as<u64>(), s));
   PIRG_HIP(hipDeviceSynchronize());
   std::vector<u64> part(static_cast<size_t>(ks) * rows);
   u64 cc = 0;

@@ -139,9 +139,13 @@ def test_batch_equals_sequential_with_invalid_request():
                 np.testing.assert_array_equal(single, want)
 
 
-@pytest.mark.parametrize("nq", [1, 2, 3, 16, 33, 64])
-def test_wide_batches_match_oracle(nq):
-    # Every scan configuration (T = 1, 2, 4, 8, 16 tiles per pass).
+@pytest.mark.parametrize("nq,wide_min", [(1, 10), (2, 10), (3, 10), (9, 10), (1, 1), (3, 1),
+                                         (16, 10), (33, 10), (64, 10), (70, 10)])
+def test_wide_batches_match_oracle(nq, wide_min, monkeypatch):
+    # Every scan configuration: NarrowScan with 1, 2 and 4 tiles per group,
+    # WideScan with 2, 4 and 7 tiles per wave, and several workgroup column
+    # groups (70 queries).
+    monkeypatch.setenv("PIRG_WIDE_MIN", str(wide_min))
     P = O.metrics_tool_params(n=4096, log_d=9, t=2)
     mat = O.synthetic_material(P, seed=nq)
     reqs = [O.synthetic_request(P, seed=1000 * nq + i) for i in range(nq)]
