@@ -154,7 +154,9 @@ def main():
         for _ in range(2):
             srv.batch_run(B)
         copy_ms = ia.copy_probe(2 << 30, repeats=3)
+        read_ms = ia.read_probe(4 << 30, repeats=3)
         print(json.dumps({"profile": True, "copy_bytes": 2 << 30, "copy_ms": copy_ms,
+                          "read_bytes": 4 << 30, "read_ms": read_ms,
                           "scan_ms": srv.scan_times_ms()}))
         srv.close()
         return

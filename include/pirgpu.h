@@ -190,6 +190,18 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
  * known-size kernel that calibrates HBM byte counters and bandwidth. */
 int pirg_copy_probe(size_t bytes, int repeats, double* ms);
 
+/* Sweeps a plain read-only kernel (8 loads of 16 bytes in flight per thread)
+ * `repeats` times over `bytes` within one launch and reports the time per
+ * sweep: the achievable read bandwidth the scans are compared with (HBM for
+ * large buffers, the caches for small ones). No reference counterpart. */
+int pirg_read_probe(size_t bytes, int repeats, double* ms);
+
+/* Runs blocks x 256 threads of register-only v_mfma_i32_16x16x64_i8 (16
+ * independent accumulators per wave, `iters` rounds) and reports the kernel
+ * time and the int8 rate in TOP/s: the achievable MFMA rate the batched scan
+ * is compared with. No reference counterpart. */
+int pirg_mfma_probe(int blocks, int iters, double* ms, double* tops);
+
 /* The twiddle table the FFT kernels use for an n-point transform, as the
  * interleaved (re, im) pairs of FftPlan's twiddles_forward_ /
  * twiddles_backward_ (crypto/fft.h:71-80); n/2 pairs each. Host only. */

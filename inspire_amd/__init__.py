@@ -427,5 +427,25 @@ def copy_probe(nbytes: int, repeats: int = 10) -> float:
     return ms.value
 
 
+def read_probe(nbytes: int, repeats: int = 10) -> float:
+    """Average ms of a plain nbytes streaming-read kernel (achievable read
+    bandwidth)."""
+    ms = ctypes.c_double()
+    rc = lib().pirg_read_probe(ctypes.c_size_t(nbytes), repeats, ctypes.byref(ms))
+    if rc != 0:
+        raise PirError(rc, "read probe failed")
+    return ms.value
+
+
+def mfma_probe(blocks: int = 1024, iters: int = 4096):
+    """(ms, TOP/s) of a register-only int8 MFMA stream (achievable MFMA rate)."""
+    ms = ctypes.c_double()
+    tops = ctypes.c_double()
+    rc = lib().pirg_mfma_probe(blocks, iters, ctypes.byref(ms), ctypes.byref(tops))
+    if rc != 0:
+        raise PirError(rc, "mfma probe failed")
+    return ms.value, tops.value
+
+
 def version() -> str:
     return lib().pirg_version().decode()

@@ -21,6 +21,11 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libpirgpu.so")
 
 SOURCES = ["limb_kernels.hip", "fft_kernels.hip", "server.cpp"]
+# Per-source flags. The scan kernels keep up to 224 int32 accumulators per
+# lane; LLVM's default AGPR form for their MFMAs makes the register allocator
+# shuttle accumulators between AGPRs and VGPRs every k-block, while the VGPR
+# form keeps them in place.
+EXTRA_FLAGS = {"limb_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 # Host-only sources built with g++ (see twiddles.cpp for why).
 GXX_SOURCES = ["twiddles.cpp"]
 ARCH = os.environ.get("PIRG_OFFLOAD_ARCH", "gfx950")
@@ -61,7 +66,8 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(BUILD_DIR, src.replace(".", "_") + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [cc, *COMMON_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc, *COMMON_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c",
+               os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

@@ -357,72 +357,107 @@ __global__ __launch_bounds__(256) void NarrowScanKernel(
 }
 
 // ---------------------------------------------------------------------------
-// WideScan: batches, bound by MFMA. A workgroup owns RS = 8 strips (128 rows)
-// and 4 consecutive groups, one per wave (CT tiles = up to 16 queries each);
-// per k-block every wave runs 8 x CT x NP MFMAs.
-//  * A (shared by the 4 waves) is staged through a 2-slot LDS ring: each wave
-//    loads a quarter of the next k-blocks' A into AD rotating register sets
-//    and writes one k-block per step.
-//  * B (private to a wave) goes straight to registers, 3 sets, issued two
-//    k-blocks ahead and before that step's A loads, so waiting for B never
-//    drains the A prefetch (vector loads complete in issue order).
-//  * The per-step barrier is s_barrier after an LDS-only wait: HIP's
-//    __syncthreads also waits for every outstanding global load, which would
-//    drain the prefetch at every k-block.
+// WideScan: batches, bound by MFMA. A workgroup of 8 waves owns RS = 8 strips
+// (128 rows) and 4 consecutive groups (CT tiles = up to 16 queries each).
+// Wave (h, g) computes strips [4h, 4h + 4) of group g: 4 x CT x NP MFMAs per
+// k-block, two waves per SIMD so one computes while the other waits.
+//  * Every global load is an LDS-DMA (global_load_lds_dwordx4) into an
+//    S-stage LDS ring: the k-block's A fragments (2*NP per wave) and the
+//    groups' B tiles (a group's CT tiles split between its two waves). No
+//    load has a register destination, so nothing waits on the loads but the
+//    explicit vmcnt wait before each step's barrier (DMA ops complete in
+//    issue order).
+//  * The barrier is s_barrier after an LDS-only wait: HIP's __syncthreads
+//    also waits for every outstanding load, which would drain the ring.
+//  * Plane p's B fragments are derived from plane 0's with DPP shifts.
 __device__ __forceinline__ void LdsBarrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int NP, int CT, int AD>
-__global__ __launch_bounds__(256) void WideScanKernel(
+template <int N>
+__device__ __forceinline__ void WaitVmcnt() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+__device__ __forceinline__ void DmaLoad16(const v4i* src, v4i* lds_dst) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_dst, 16,
+                                   0, 0);
+}
+
+template <int NP, int CT>
+struct WideShape {
+  static constexpr int RS = 8;                // strips per workgroup
+  static constexpr int RW = 4;                // strips per wave
+  static constexpr int AF = RS * NP / 8;      // A fragments DMA'd per wave per k-block
+  static constexpr int BH0 = (CT + 1) / 2;    // B tiles DMA'd by the h = 0 wave of a group
+  static constexpr int BH1 = CT - BH0;
+  static constexpr int kA = RS * NP * 64;     // v4i of A per stage
+  static constexpr int kB = 4 * CT * 64;      // v4i of B per stage
+  static constexpr int kStage = kA + kB;
+  static constexpr int kBudget = 150 * 1024 / 16;
+  static constexpr int S = kBudget / kStage >= 4 ? 4 : (kBudget / kStage >= 3 ? 3 : 2);
+  static constexpr int kRedWords = 8 * RW * 16 * kQMax;
+  static constexpr size_t kLdsBytes =
+      (static_cast<size_t>(S) * kStage * 16 > static_cast<size_t>(kRedWords) * 8)
+          ? static_cast<size_t>(S) * kStage * 16
+          : static_cast<size_t>(kRedWords) * 8;
+};
+
+template <int NP, int CT>
+__global__ __launch_bounds__(512) void WideScanKernel(
     const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps,
     const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
     u64* __restrict__ partial) {
-  constexpr int RS = 8;
-  constexpr int AF = RS * NP / 4;  // A fragments staged per wave per k-block
-  constexpr int BD = 3;
-  static_assert(AD == 2 || AD == 3, "A staging depth");
-  __shared__ v4i aring[2][RS * NP * 64];
-  __shared__ u64 red[4][RS * 16 * kQMax];
+  using W = WideShape<NP, CT>;
+  constexpr int RS = W::RS, RW = W::RW, AF = W::AF, S = W::S;
+  extern __shared__ v4i lds[];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  const int h = wave >> 2;       // strip half
+  const int g = wave & 3;        // group within the workgroup
   const int strip0 = blockIdx.x * RS;
   const int ks = blockIdx.y;
-  const int grp = blockIdx.z * 4 + wave;
+  const int grp = blockIdx.z * 4 + g;
   const int kb0 = ks * kbps;
   const int nkb = min(kblocks, kb0 + kbps) - kb0;
   const v4i* A = reinterpret_cast<const v4i*>(tiles);
   const v4i* Bg = reinterpret_cast<const v4i*>(btiles) + (long long)grp * CT * 64 + lane;
-  // Fragment f = wave*AF + jf of a k-block is (strip f / NP, plane f % NP).
+  // A fragment f = wave*AF + jf of a k-block is (strip f / NP, plane f % NP)
+  // and lands at stage offset f*64: the stage holds A as [strip][plane].
   const v4i* Af[AF];
 #pragma unroll
   for (int jf = 0; jf < AF; ++jf) {
     const int f = wave * AF + jf;
     Af[jf] = A + ((long long)min(strip0 + f / NP, strips - 1) * kblocks * NP + f % NP) * 64 + lane;
   }
+  const int c0 = h == 0 ? 0 : W::BH0;  // first B tile this wave issues
   const unsigned cmask = ColumnMask(CT, NP, ns, lane);
-  v4i acc[RS][CT];
+  v4i acc[RW][CT];
 #pragma unroll
-  for (int r = 0; r < RS; ++r)
+  for (int r = 0; r < RW; ++r)
 #pragma unroll
     for (int c = 0; c < CT; ++c) acc[r][c] = v4i{0, 0, 0, 0};
-  v4i as[AD][AF];
-  v4i bs[BD][CT];
-  auto load_a = [&](int i, v4i (&dst)[AF]) {
-    const int kb = kb0 + min(i, nkb - 1);
+
+  // Steps past the k-range read A's last k-block and B's all-zero block.
+  auto issue = [&](int i) {
+    v4i* st = lds + (i % S) * W::kStage;
+    const long long kba = kb0 + min(i, nkb - 1);
+    const long long kbb = i < nkb ? kb0 + i : kblocks;
 #pragma unroll
-    for (int jf = 0; jf < AF; ++jf) dst[jf] = Af[jf][(long long)kb * NP * 64];
-  };
-  auto load_b = [&](int i, v4i (&dst)[CT]) {
-    const int kb = i < nkb ? kb0 + i : kblocks;  // all-zero block past the range
+    for (int jf = 0; jf < AF; ++jf) DmaLoad16(Af[jf] + kba * NP * 64, st + (wave * AF + jf) * 64);
 #pragma unroll
-    for (int c = 0; c < CT; ++c) dst[c] = Bg[((long long)kb * ngp * CT + c) * 64];
+    for (int cc = 0; cc < W::BH0; ++cc) {
+      if (h == 0 || cc < W::BH1) {
+        const int c = c0 + cc;
+        DmaLoad16(Bg + (kbb * ngp * CT + c) * 64, st + W::kA + (g * CT + c) * 64);
+      }
+    }
   };
-  auto store_a = [&](int slot, const v4i (&src)[AF]) {
+  auto compute = [&](int i) {
+    const v4i* st = lds + (i % S) * W::kStage;
+    v4i b0[CT];
 #pragma unroll
-    for (int jf = 0; jf < AF; ++jf) aring[slot][(wave * AF + jf) * 64 + lane] = src[jf];
-  };
-  auto compute = [&](int slot, const v4i (&b0)[CT]) {
+    for (int c = 0; c < CT; ++c) b0[c] = st[W::kA + (g * CT + c) * 64 + lane];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       v4i bp[CT];
@@ -437,45 +472,34 @@ __global__ __launch_bounds__(256) void WideScanKernel(
         bp[c] = p == 0 ? x : Masked(x, (cmask >> (c * 4 + p)) & 1u);
       }
 #pragma unroll
-      for (int r = 0; r < RS; ++r) {
-        const v4i af = aring[slot][(r * NP + p) * 64 + lane];
+      for (int r = 0; r < RW; ++r) {
+        const v4i af = st[((h * RW + r) * NP + p) * 64 + lane];
 #pragma unroll
         for (int c = 0; c < CT; ++c)
           acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bp[c], acc[r][c], 0, 0, 0);
       }
     }
   };
-  // Prologue: A(0) into slot 0; B(0), A(1), B(1), A(2..AD-1) in flight.
-  load_a(0, as[0]);
-  store_a(0, as[0]);
-  load_b(0, bs[0]);
 #pragma unroll
-  for (int d = 1; d < AD; ++d) {
-    load_a(d, as[d % AD]);
-    if (d == 1) load_b(1, bs[1]);
+  for (int s = 0; s < S - 1; ++s) issue(s);
+  for (int i = 0; i < nkb; ++i) {
+    // This wave's DMAs of step i have landed (the later S-2 steps' may not).
+    if (h == 0) {
+      WaitVmcnt<(AF + W::BH0) * (S - 2)>();
+    } else {
+      WaitVmcnt<(AF + W::BH1) * (S - 2)>();
+    }
+    LdsBarrier();  // every wave's DMAs of step i have landed; step i-1 is consumed
+    issue(i + S - 1);
+    compute(i);
   }
+  // Drain the ring (the last issues target clamped / zero blocks) before
+  // its memory is reused by the epilogue.
+  WaitVmcnt<0>();
   LdsBarrier();
-  // Step i: issue B(i+2), A(i+AD); compute on slot i%2 with B(i); write
-  // A(i+1) into the other slot; barrier.
-  auto step = [&](int i, int u) {
-    const int bnext = (u + 2) % BD;
-    const int anext = (u + AD) % AD;
-    load_b(i + 2, bs[bnext]);
-    load_a(i + AD, as[anext]);
-    compute(u % 2, bs[u % BD]);
-    store_a((u + 1) % 2, as[(u + 1) % AD]);
-    LdsBarrier();
-  };
-  for (int i = 0; i < nkb; i += 6) {
-    step(i + 0, 0);
-    step(i + 1, 1);
-    step(i + 2, 2);
-    step(i + 3, 3);
-    step(i + 4, 4);
-    step(i + 5, 5);
-  }
-  WaveEpilogue<RS, CT>(acc, red[wave], lane, ns, qpg, grp, nq, rows,
-                       (long long)strip0 * 16, ks, partial);
+  u64* red = reinterpret_cast<u64*>(lds) + wave * (RW * 16 * kQMax);
+  WaveEpilogue<RW, CT>(acc, red, lane, ns, qpg, grp, nq, rows,
+                       (long long)(strip0 + h * RW) * 16, ks, partial);
 }
 
 int GridBlocks(long long total, int threads) {
@@ -501,11 +525,16 @@ hipError_t LaunchNarrowT(const int8_t* tiles, const LimbLayout& L, const ScanPla
 template <int NP, int CT>
 hipError_t LaunchWideT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                        const int8_t* btiles, u64* partial, hipStream_t s) {
-  constexpr int AD = (NP >= 4 || CT >= 7) ? 2 : 3;
+  using W = WideShape<NP, CT>;
+  static_assert(W::kLdsBytes <= 160 * 1024, "WideScan LDS");
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanKernel<NP, CT>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(W::kLdsBytes));
+  if (e != hipSuccess) return e;
   dim3 grid((L.strips + 7) / 8, P.ksplit, P.ngp / 4);
-  hipLaunchKernelGGL((WideScanKernel<NP, CT, AD>), grid, dim3(256), 0, s, tiles, L.strips,
-                     L.rows, L.kblocks, P.kb_per_split, btiles, P.ngp, P.nq, P.ns, P.qpg,
-                     partial);
+  hipLaunchKernelGGL((WideScanKernel<NP, CT>), grid, dim3(512), W::kLdsBytes, s, tiles,
+                     L.strips, L.rows, L.kblocks, P.kb_per_split, btiles, P.ngp, P.nq, P.ns,
+                     P.qpg, partial);
   return hipGetLastError();
 }
 
@@ -538,6 +567,69 @@ __global__ void CopyProbeKernel(const int4* __restrict__ src, int4* __restrict__
        i += (long long)gridDim.x * blockDim.x) {
     dst[i] = src[i];
   }
+}
+
+// Plain streaming read: `passes` sweeps over the buffer, every 16-byte
+// element loaded once per sweep (8 loads in flight per thread) and folded by
+// XOR; one word per thread is written. The reference kernel for the
+// achievable read bandwidth (from HBM for large buffers, from the caches for
+// small ones).
+__global__ __launch_bounds__(256) void ReadProbeKernel(const int4* __restrict__ src,
+                                                       long long n, int passes,
+                                                       int* __restrict__ out) {
+  const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  int acc = 0;
+  for (int pass = 0; pass < passes; ++pass) {
+    long long i = tid;
+    for (; i + 7 * stride < n; i += 8 * stride) {
+      int4 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    for (; i < n; i += stride) {
+      const int4 v = src[i];
+      acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+  }
+  out[tid] = acc;
+}
+
+// Register-only v_mfma_i32_16x16x64_i8 stream: 16 independent accumulators
+// per wave, `iters` rounds. The reference kernel for the achievable int8 MFMA
+// rate (one output word per thread keeps the work observable).
+__global__ __launch_bounds__(256) void MfmaProbeKernel(int iters, int* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  v4i acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = v4i{0, 0, 0, 0};
+  const v4i a = v4i{lane, 1, 2, 3};
+  const v4i b = v4i{3, lane, 1, 2};
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[j], 0, 0, 0);
+  }
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+hipError_t LaunchMfmaProbe(int blocks, int iters, int* out, hipStream_t s) {
+  hipLaunchKernelGGL(MfmaProbeKernel, dim3(blocks), dim3(256), 0, s, iters, out);
+  return hipGetLastError();
+}
+
+int ReadProbeOutWords() { return 2048 * 256; }
+
+hipError_t LaunchReadProbe(const void* src, size_t bytes, int passes, int* out,
+                          hipStream_t s) {
+  const long long n = static_cast<long long>(bytes / 16);
+  hipLaunchKernelGGL(ReadProbeKernel, dim3(2048), dim3(256), 0, s,
+                     static_cast<const int4*>(src), n, passes, out);
+  return hipGetLastError();
 }
 
 hipError_t LaunchCopyProbe(const void* src, void* dst, size_t bytes, hipStream_t s) {

@@ -1110,6 +1110,63 @@ int pirg_copy_probe(size_t bytes, int repeats, double* ms) {
   return er.code;
 }
 
+int pirg_read_probe(size_t bytes, int repeats, double* ms) {
+  auto run = [&]() -> Error {
+    Error er = CheckDevice();
+    if (!er.ok()) return er;
+    DevBuf a, o;
+    if ((er = a.alloc(bytes)).code) return er;
+    if ((er = o.alloc(static_cast<size_t>(ReadProbeOutWords()) * 4)).code) return er;
+    PIRG_HIP(hipMemset(a.p, 1, bytes));
+    hipEvent_t e0, e1;
+    PIRG_HIP(hipEventCreate(&e0));
+    PIRG_HIP(hipEventCreate(&e1));
+    const int passes = std::max(repeats, 1);
+    PIRG_HIP(LaunchReadProbe(a.p, bytes, 1, o.as<int>(), nullptr));
+    PIRG_HIP(hipEventRecord(e0, nullptr));
+    PIRG_HIP(LaunchReadProbe(a.p, bytes, passes, o.as<int>(), nullptr));
+    PIRG_HIP(hipEventRecord(e1, nullptr));
+    PIRG_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    PIRG_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / passes;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return Ok();
+  };
+  Error er = run();
+  return er.code;
+}
+
+int pirg_mfma_probe(int blocks, int iters, double* ms, double* tops) {
+  auto run = [&]() -> Error {
+    Error er = CheckDevice();
+    if (!er.ok()) return er;
+    if (blocks <= 0 || iters <= 0) return Invalid("blocks and iters must be positive.");
+    DevBuf o;
+    if ((er = o.alloc(static_cast<size_t>(blocks) * 256 * 4)).code) return er;
+    hipEvent_t e0, e1;
+    PIRG_HIP(hipEventCreate(&e0));
+    PIRG_HIP(hipEventCreate(&e1));
+    PIRG_HIP(LaunchMfmaProbe(blocks, iters, o.as<int>(), nullptr));
+    PIRG_HIP(hipEventRecord(e0, nullptr));
+    PIRG_HIP(LaunchMfmaProbe(blocks, iters, o.as<int>(), nullptr));
+    PIRG_HIP(hipEventRecord(e1, nullptr));
+    PIRG_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    PIRG_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    // 4 waves per block, 16 MFMAs per round, 16 x 16 x 64 x 2 ops each.
+    const double ops = static_cast<double>(blocks) * 4 * iters * 16 * 32768.0;
+    if (tops) *tops = ops / (t * 1e-3) / 1e12;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return Ok();
+  };
+  Error er = run();
+  return er.code;
+}
+
 int pirg_fft_twiddles(int n, double* fwd, double* bwd) {
   if (n < 2 || n > (1 << 20) || (n & (n - 1)) != 0) return PIRG_INVALID_ARGUMENT;
   pirg_internal_twiddles(n, fwd, bwd);
