@@ -16,8 +16,8 @@
  *
  * Threading: a pirg_server is externally synchronised (like the reference,
  * whose ProcessResponse is not safe to call concurrently), except for the
- * serving-loop calls pirg_serve_submit / pirg_serve_poll, which are
- * thread-safe.
+ * serving-loop calls pirg_serve_submit / pirg_serve_poll / pirg_serve_stats,
+ * which are thread-safe and may run beside the other calls.
  */
 #ifndef PIRGPU_H_
 #define PIRGPU_H_
@@ -31,6 +31,7 @@ extern "C" {
 
 #define PIRG_OK 0
 #define PIRG_INVALID_ARGUMENT 3
+#define PIRG_DEADLINE_EXCEEDED 4
 #define PIRG_RESOURCE_EXHAUSTED 8
 #define PIRG_FAILED_PRECONDITION 9
 #define PIRG_INTERNAL 13
@@ -145,6 +146,46 @@ int pirg_batch_stage(pirg_server* server, int nreq,
 int pirg_batch_run(pirg_server* server, int nreq, char* err, size_t errlen);
 int pirg_batch_fetch(pirg_server* server, int nreq, uint64_t* const* outs,
                      char* err, size_t errlen);
+
+/* Makes pirg_process_batch / pirg_batch_run replay one hipGraph per batch
+ * size (captured on first use: the whole answer pipeline as one graph launch)
+ * instead of launching the kernels one by one. Off by default; graph replays
+ * do not add to the stage times below. No reference counterpart. */
+int pirg_server_use_graphs(pirg_server* server, int enable);
+
+/* ---- Serving loop (no reference counterpart: the reference answers one
+ * request per call). The DB and preprocessed material stay resident; a
+ * worker thread keeps `slots` execution slots busy, each answering up to
+ * `max_batch` requests per graph launch, and every request waiting when a
+ * slot frees joins that slot's next batch (continuous batching; the slots'
+ * batches overlap on the GPU). Slots and batch capacity are reduced at start
+ * to fit 80% of the free HBM; a submit is refused with
+ * PIRG_RESOURCE_EXHAUSTED while max_outstanding requests are submitted but
+ * not yet polled. */
+int pirg_serve_start(pirg_server* server, int max_batch, int slots,
+                     int max_outstanding, char* err, size_t errlen);
+/* Validates like ProcessResponse (same messages) and queues the request; the
+ * buffers are copied before return. Thread-safe. */
+int pirg_serve_submit(pirg_server* server, const pirg_request_view* req,
+                      uint64_t tag, char* err, size_t errlen);
+/* Waits up to timeout_ms for a finished request: returns 0 and fills tag,
+ * status (its ProcessResponse status), latency_ms (submit to completion) and
+ * out (ell x 2 x d, when status == 0); returns PIRG_DEADLINE_EXCEEDED when
+ * none finished in time. Thread-safe. */
+int pirg_serve_poll(pirg_server* server, int timeout_ms, uint64_t* tag,
+                    uint64_t* out, int* status, double* latency_ms, char* err,
+                    size_t errlen);
+typedef struct {
+  long long submitted, answered, batches;
+  double mean_batch;            /* requests per launched batch */
+  double p50_ms, p99_ms, max_ms; /* submit-to-completion latency */
+  double elapsed_ms;            /* since start or the last reset */
+  int slots, max_batch;         /* after admission by free HBM */
+} pirg_serve_stats_t;
+int pirg_serve_stats(pirg_server* server, pirg_serve_stats_t* stats, int reset);
+/* Answers everything submitted, then stops the worker (unpolled results are
+ * dropped). pirg_server_destroy stops a running loop itself. */
+int pirg_serve_stop(pirg_server* server);
 
 /* PirServer::GetDbMultiplyTime / GetPackingTime / GetPolyEvalTime
  * (pir_server.h:70-72): cumulative GPU time, in milliseconds, of the DB

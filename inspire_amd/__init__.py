@@ -28,6 +28,7 @@ _lib = None
 
 OK = 0
 INVALID_ARGUMENT = 3
+DEADLINE_EXCEEDED = 4
 RESOURCE_EXHAUSTED = 8
 FAILED_PRECONDITION = 9
 INTERNAL = 13
@@ -54,7 +55,21 @@ def lib() -> ctypes.CDLL:
         _lib.pirg_version.restype = ctypes.c_char_p
         _lib.pirg_server_fft_margin.restype = ctypes.c_double
         _lib.pirg_server_destroy.restype = None
+        _lib.pirg_serve_submit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                           ctypes.c_char_p, ctypes.c_size_t]
+        _lib.pirg_serve_poll.argtypes = [ctypes.c_void_p, ctypes.c_int, _u64p, _u64p,
+                                         ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_double), ctypes.c_char_p,
+                                         ctypes.c_size_t]
     return _lib
+
+
+class _CServeStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("answered", ctypes.c_longlong),
+                ("batches", ctypes.c_longlong), ("mean_batch", ctypes.c_double),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
+                ("slots", ctypes.c_int), ("max_batch", ctypes.c_int)]
 
 
 class _CParams(ctypes.Structure):
@@ -374,6 +389,53 @@ class PirServer:
     def fft_margin(self, reset: bool = False) -> float:
         """Largest |x - round(x)| the FFT products have rounded so far."""
         return lib().pirg_server_fft_margin(self._h, int(reset))
+
+    def use_graphs(self, enable: bool = True) -> None:
+        """process_batch / batch_run replay one captured hipGraph per batch size."""
+        _check(lib().pirg_server_use_graphs(self._h, int(enable)), _err())
+
+    # Serving loop (continuous batching; thread-safe submit / poll).
+    def serve_start(self, max_batch: int = 64, slots: int = 2,
+                    max_outstanding: int = 4096) -> None:
+        err = _err()
+        rc = lib().pirg_serve_start(self._h, max_batch, slots, max_outstanding, err,
+                                    ctypes.c_size_t(len(err)))
+        _check(rc, err)
+
+    def submit(self, request: PirRequest, tag: int) -> None:
+        """Queues a request (validated like ProcessResponse); raises PirError
+        (RESOURCE_EXHAUSTED when max_outstanding requests await polling)."""
+        ra = _RequestArrays(request, self.params.degree)
+        view = ra.view()
+        err = _err()
+        rc = lib().pirg_serve_submit(self._h, ctypes.byref(view), ctypes.c_uint64(tag), err,
+                                     ctypes.c_size_t(len(err)))
+        _check(rc, err)
+
+    def poll(self, timeout_ms: int = 1000):
+        """Next finished request as (tag, response or None, status, latency_ms,
+        message), or None when nothing finished within timeout_ms."""
+        out = np.zeros(self._out_shape, dtype=np.uint64)
+        tag = ctypes.c_uint64()
+        status = ctypes.c_int()
+        lat = ctypes.c_double()
+        err = _err()
+        rc = lib().pirg_serve_poll(self._h, int(timeout_ms), ctypes.byref(tag),
+                                   out.ctypes.data_as(_u64p), ctypes.byref(status),
+                                   ctypes.byref(lat), err, ctypes.c_size_t(len(err)))
+        if rc == DEADLINE_EXCEEDED:
+            return None
+        _check(rc, err)
+        resp = PirResponse(out) if status.value == 0 else None
+        return tag.value, resp, status.value, lat.value, err.value.decode()
+
+    def serve_stats(self, reset: bool = False) -> dict:
+        st = _CServeStats()
+        _check(lib().pirg_serve_stats(self._h, ctypes.byref(st), int(reset)), _err())
+        return {f: getattr(st, f) for f, _ in _CServeStats._fields_}
+
+    def serve_stop(self) -> None:
+        lib().pirg_serve_stop(self._h)
 
 
 def matrix_multiply_u16(cdata: np.ndarray, cols: int, vec: np.ndarray) -> np.ndarray:

@@ -527,10 +527,6 @@ hipError_t LaunchWideT(const int8_t* tiles, const LimbLayout& L, const ScanPlan&
                        const int8_t* btiles, u64* partial, hipStream_t s) {
   using W = WideShape<NP, CT>;
   static_assert(W::kLdsBytes <= 160 * 1024, "WideScan LDS");
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanKernel<NP, CT>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     static_cast<int>(W::kLdsBytes));
-  if (e != hipSuccess) return e;
   dim3 grid((L.strips + 7) / 8, P.ksplit, P.ngp / 4);
   hipLaunchKernelGGL((WideScanKernel<NP, CT>), grid, dim3(512), W::kLdsBytes, s, tiles,
                      L.strips, L.rows, L.kblocks, P.kb_per_split, btiles, P.ngp, P.nq, P.ns,
@@ -660,6 +656,25 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
                      0, s, pack, mat_bytes, rows, cols, row0, nrows, L.strips,
                      L.kblocks, L.planes, tiles);
   return hipGetLastError();
+}
+
+template <int NP, int CT>
+hipError_t ConfigureWide() {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanKernel<NP, CT>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(WideShape<NP, CT>::kLdsBytes));
+}
+
+// The WideScan instantiations need more than the default 64 KiB of dynamic
+// LDS; set once, outside any stream capture.
+hipError_t ConfigureScanKernels() {
+  const hipError_t es[] = {ConfigureWide<1, 2>(), ConfigureWide<1, 4>(), ConfigureWide<1, 7>(),
+                           ConfigureWide<2, 2>(), ConfigureWide<2, 4>(), ConfigureWide<2, 7>(),
+                           ConfigureWide<4, 2>(), ConfigureWide<4, 4>(), ConfigureWide<4, 7>()};
+  for (hipError_t e : es) {
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 int QueriesPerGroup(int tiles, int ns) {

@@ -114,6 +114,9 @@ struct ScanPlan {
 // wide_min_queries: batches of at least this many queries use WideScan even
 // when they would fit NarrowScan.
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries);
+// Sets the launch attributes the scan kernels need; call once per process
+// (and device) before the first scan, outside any stream capture.
+hipError_t ConfigureScanKernels();
 // Plane-0 digit tiles of nq vectors (vec + q*stride, length len each, zero
 // beyond len): layout [kb][group][tile][64 lanes][16 bytes].
 hipError_t LaunchQueryDigits(const u64* vec, int stride, int len, const LimbLayout& L,

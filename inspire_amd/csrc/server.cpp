@@ -16,38 +16,82 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pirgpu.h"
-#include "pirg_internal.h"
+#include "server_internal.h"
 
 extern "C" void pirg_internal_twiddles(int n, double* fwd, double* bwd);
 
 namespace pirg {
-
-namespace {
-
-struct Error {
-  int code = 0;
-  std::string msg;
-  bool ok() const { return code == 0; }
-};
-
-Error Ok() { return Error{}; }
-Error Invalid(const std::string& m) { return Error{PIRG_INVALID_ARGUMENT, m}; }
-Error Internal(const std::string& m) { return Error{PIRG_INTERNAL, m}; }
-Error HipErr(hipError_t e, const char* what) {
-  return Error{PIRG_INTERNAL, std::string(what) + ": " + hipGetErrorString(e)};
-}
 
 int Report(const Error& e, char* err, size_t errlen) {
   if (err != nullptr && errlen > 0) std::snprintf(err, errlen, "%s", e.msg.c_str());
   return e.code;
 }
 
-#define PIRG_HIP(expr)                                  \
-  do {                                                  \
-    hipError_t _e = (expr);                             \
-    if (_e != hipSuccess) return HipErr(_e, #expr);     \
-  } while (0)
+void DevBuf::reset() {
+  if (p != nullptr) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+Error DevBuf::alloc(size_t n) {
+  reset();
+  if (n == 0) n = 8;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return Error{PIRG_RESOURCE_EXHAUSTED, "device allocation of " + std::to_string(n) +
+                                              " bytes failed: " + hipGetErrorString(e)};
+  }
+  bytes = n;
+  return Ok();
+}
+
+void HostBuf::reset() {
+  if (p != nullptr) (void)hipHostFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+Error HostBuf::alloc(size_t n) {
+  reset();
+  if (n == 0) n = 8;
+  hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return Error{PIRG_RESOURCE_EXHAUSTED, "pinned host allocation failed"};
+  }
+  bytes = n;
+  return Ok();
+}
+
+Slot::~Slot() {
+  if (stream != nullptr) (void)hipStreamSynchronize(stream);
+  DropGraphs();
+  for (auto& x : ev) {
+    if (x != nullptr) (void)hipEventDestroy(x);
+  }
+  for (auto& x : kev) {
+    if (x != nullptr) (void)hipEventDestroy(x);
+  }
+  if (done != nullptr) (void)hipEventDestroy(done);
+  if (stream != nullptr) (void)hipStreamDestroy(stream);
+}
+
+// Streams keep the default priority.
+Error Slot::Init() {
+  PIRG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  for (auto& x : ev) PIRG_HIP(hipEventCreate(&x));
+  for (auto& x : kev) PIRG_HIP(hipEventCreate(&x));
+  PIRG_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  return Ok();
+}
+
+void Slot::DropGraphs() {
+  for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+  graphs.clear();
+}
+
+namespace {
 
 bool IsPow2(u64 x) { return x != 0 && (x & (x - 1)) == 0; }
 
@@ -91,120 +135,10 @@ Error MakeRules(int n_terms, int d, int u_bits, int v_bits, int chunk,
   return Ok();
 }
 
-// Device allocation that is released with the owner.
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { reset(); }
-  void reset() {
-    if (p != nullptr) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  Error alloc(size_t n) {
-    reset();
-    if (n == 0) n = 8;
-    hipError_t e = hipMalloc(&p, n);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return Error{PIRG_RESOURCE_EXHAUSTED,
-                   "device allocation of " + std::to_string(n) + " bytes failed: " +
-                       hipGetErrorString(e)};
-    }
-    bytes = n;
-    return Ok();
-  }
-  template <typename T>
-  T* as() const {
-    return static_cast<T*>(p);
-  }
-};
-
-struct HostBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  HostBuf() = default;
-  HostBuf(const HostBuf&) = delete;
-  HostBuf& operator=(const HostBuf&) = delete;
-  ~HostBuf() { reset(); }
-  void reset() {
-    if (p != nullptr) (void)hipHostFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  Error alloc(size_t n) {
-    reset();
-    if (n == 0) n = 8;
-    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return Error{PIRG_RESOURCE_EXHAUSTED, "pinned host allocation failed"};
-    }
-    bytes = n;
-    return Ok();
-  }
-  template <typename T>
-  T* as() const {
-    return static_cast<T*>(p);
-  }
-};
-
-// Per-batch working memory, sized for `cap` requests.
-struct Workspace {
-  int cap = 0;
-  DevBuf fd, sd, pk;             // staged requests
-  DevBuf bt_db, bt_pk;           // B-operand tiles
-  DevBuf corr_db, corr_pk;
-  DevBuf db_partial, pk_partial;
-  DevBuf yh_spec, sdb_spec, d_spec;
-  DevBuf fin_partial, ext_partial;
-  DevBuf ct_b, st_a, st_b, out;
-  DevBuf fin_tab, ext_tab;       // IpKernel pointer tables
-  HostBuf h_fd, h_sd, h_pk, h_out;
-};
-
 }  // namespace
 }  // namespace pirg
 
 using namespace pirg;
-
-struct pirg_server {
-  pirg_params P{};
-  int d = 0, log_d = 0, N = 0, log_N = 0, t = 0, ell = 0, k = 0, e = 0;
-  int r = 0, K = 0, M = 0, log_q = 0;
-  u64 qmask = 0;
-  FftDims F{};
-  ChunkRule fin_u{}, fin_v{}, ext_u{}, ext_v{};
-  PairList fin_pairs{}, ext_pairs{};
-  Error deferred;  // error ProcessResponse must return (chunk validation)
-  LimbLayout dbL{}, pkL{};
-  // Batches of at least this many requests use the MFMA-bound WideScan.
-  int wide_min = 10;
-  DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};  // DB / pack scan
-  double db_ms = 0, pack_ms = 0, eval_ms = 0;
-  double db_scan_ms = 0, pk_scan_ms = 0;
-  long long scans = 0;
-  bool timing_pending = false;
-  Workspace ws;
-  int staged = 0;
-
-  ~pirg_server() {
-    if (stream != nullptr) (void)hipStreamSynchronize(stream);
-    for (auto& x : ev) {
-      if (x != nullptr) (void)hipEventDestroy(x);
-    }
-    for (auto& x : kev) {
-      if (x != nullptr) (void)hipEventDestroy(x);
-    }
-    if (stream != nullptr) (void)hipStreamDestroy(stream);
-  }
-};
 
 namespace pirg {
 namespace {
@@ -350,9 +284,12 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
                     const uint64_t* second_dim_a) {
   const int d = S->d;
   S->K = db_cols;
-  PIRG_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
-  for (auto& x : S->ev) PIRG_HIP(hipEventCreate(&x));
-  for (auto& x : S->kev) PIRG_HIP(hipEventCreate(&x));
+  PIRG_HIP(hipGetDevice(&S->device));
+  {
+    Error ie = S->main.Init();
+    if (!ie.ok()) return ie;
+  }
+  PIRG_HIP(ConfigureScanKernels());
   PIRG_HIP(ConfigureFftKernels(S->N));
 
   // DB planes.
@@ -367,10 +304,10 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   for (size_t part = 0; part < db_parts.size(); ++part) {
     const int base = static_cast<int>(part) * rows_per_part;
     er = UploadTiles(db_parts[part], static_cast<size_t>(wpr) * 8, rows_per_part, false,
-                     S->stream, [&](const void* src, int r0, int nr) {
+                     S->main.stream, [&](const void* src, int r0, int nr) {
                        return LaunchDbToTiles(static_cast<const u64*>(src), S->P.db_bytes,
                                               S->M, db_cols, base + r0, nr, S->dbL,
-                                              S->db_tiles.as<int8_t>(), S->stream);
+                                              S->db_tiles.as<int8_t>(), S->main.stream);
                      });
     if (!er.ok()) return er;
   }
@@ -388,11 +325,11 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   const int pk_rows_per_part = S->M / static_cast<int>(pack_parts.size());
   for (size_t part = 0; part < pack_parts.size(); ++part) {
     const int base = static_cast<int>(part) * pk_rows_per_part;
-    er = UploadTiles(pack_parts[part], pk_rowbytes, pk_rows_per_part, false, S->stream,
+    er = UploadTiles(pack_parts[part], pk_rowbytes, pk_rows_per_part, false, S->main.stream,
                      [&](const void* src, int r0, int nr) {
                        return LaunchPackToTiles(src, S->P.mat_bytes, S->M, pk_cols,
                                                 base + r0, nr, S->pkL,
-                                                S->pk_tiles.as<int8_t>(), S->stream);
+                                                S->pk_tiles.as<int8_t>(), S->main.stream);
                      });
     if (!er.ok()) return er;
   }
@@ -433,7 +370,7 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
                          sizeof(double2));
     if (!er.ok()) return er;
     PIRG_HIP(LaunchSpectra(th.as<u64>(), lt * S->k, 1, d, S->fin_u, S->F,
-                           S->tw_fwd.as<double2>(), S->t_spec.as<double2>(), S->stream));
+                           S->tw_fwd.as<double2>(), S->t_spec.as<double2>(), S->main.stream));
     if (S->t > 1) {
       DevBuf sda;
       er = sda.alloc(static_cast<size_t>(2 * S->e) * d * 8);
@@ -444,17 +381,23 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
                              sizeof(double2));
       if (!er.ok()) return er;
       PIRG_HIP(LaunchSpectra(sda.as<u64>(), 2 * S->e, 1, d, S->ext_v, S->F,
-                             S->tw_fwd.as<double2>(), S->sda_spec.as<double2>(), S->stream));
-      PIRG_HIP(hipStreamSynchronize(S->stream));
+                             S->tw_fwd.as<double2>(), S->sda_spec.as<double2>(), S->main.stream));
+      PIRG_HIP(hipStreamSynchronize(S->main.stream));
     }
-    PIRG_HIP(hipStreamSynchronize(S->stream));
+    PIRG_HIP(hipStreamSynchronize(S->main.stream));
   }
   return Ok();
 }
 
-Error EnsureWorkspace(pirg_server* S, int cap) {
-  Workspace& W = S->ws;
-  if (W.cap >= cap) return Ok();
+}  // namespace
+
+size_t ResponseWords(const pirg_server* S) { return static_cast<size_t>(S->ell) * 2 * S->d; }
+
+namespace {
+
+// Device buffer sizes of a workspace for `cap` requests, in the order of
+// Workspace's members fd .. ext_tab.
+std::vector<size_t> WorkspaceSizes(const pirg_server* S, int cap) {
   const size_t d = S->d, N = S->N;
   const int lt = S->ell * S->t;
   const size_t nq = cap;
@@ -469,38 +412,61 @@ Error EnsureWorkspace(pirg_server* S, int cap) {
     part_pk = std::max(part_pk, static_cast<size_t>(pp.ksplit) * n);
   }
   const size_t corr_words = nq + VectorSumScratchWords(cap);
-  struct Item {
-    DevBuf* b;
-    size_t bytes;
+  const int nv_f = S->fin_v.num_chunks;
+  const int nu_e = S->t > 1 ? S->ext_u.num_chunks : 0;
+  const int nv_e = S->t > 1 ? S->ext_v.num_chunks : 0;
+  return {
+      nq * S->K * 8,
+      nq * 2 * S->e * d * 8,
+      nq * 2 * S->k * d * 8,
+      bt_db,
+      bt_pk,
+      corr_words * 8,
+      corr_words * 8,
+      part_db * S->M * 8,
+      part_pk * S->M * 8,
+      nq * S->k * nv_f * N * sizeof(double2),
+      nq * 2 * S->e * nv_e * N * sizeof(double2),
+      nq * S->ell * 2 * S->e * nu_e * N * sizeof(double2),
+      nq * lt * S->fin_pairs.count * d * 8,
+      nq * S->ell * 2 * std::max(1, S->ext_pairs.count) * d * 8,
+      nq * lt * d * 8,
+      nq * S->ell * d * 8,
+      nq * S->ell * d * 8,
+      nq * S->ell * 2 * d * 8,
+      nq * lt * 3 * sizeof(void*),
+      nq * S->ell * 2 * 3 * sizeof(void*),
   };
+}
+
+}  // namespace
+
+size_t WorkspaceBytes(const pirg_server* S, int cap) {
+  size_t total = 0;
+  for (size_t b : WorkspaceSizes(S, cap)) total += b;
+  return total;
+}
+
+Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap) {
+  Workspace& W = slot.ws;
+  if (W.cap >= cap) return Ok();
+  // Captured graphs hold the old buffer addresses.
+  (void)hipStreamSynchronize(slot.stream);
+  slot.DropGraphs();
+  const size_t d = S->d, N = S->N;
+  const int lt = S->ell * S->t;
+  const size_t nq = cap;
   const int nu_f = S->fin_u.num_chunks, nv_f = S->fin_v.num_chunks;
   const int nu_e = S->t > 1 ? S->ext_u.num_chunks : 0;
   const int nv_e = S->t > 1 ? S->ext_v.num_chunks : 0;
-  std::vector<Item> items = {
-      {&W.fd, nq * S->K * 8},
-      {&W.sd, nq * 2 * S->e * d * 8},
-      {&W.pk, nq * 2 * S->k * d * 8},
-      {&W.bt_db, bt_db},
-      {&W.bt_pk, bt_pk},
-      {&W.corr_db, corr_words * 8},
-      {&W.corr_pk, corr_words * 8},
-      {&W.db_partial, part_db * S->M * 8},
-      {&W.pk_partial, part_pk * S->M * 8},
-      {&W.yh_spec, nq * S->k * nv_f * N * sizeof(double2)},
-      {&W.sdb_spec, nq * 2 * S->e * nv_e * N * sizeof(double2)},
-      {&W.d_spec, nq * S->ell * 2 * S->e * nu_e * N * sizeof(double2)},
-      {&W.fin_partial, nq * lt * S->fin_pairs.count * d * 8},
-      {&W.ext_partial, nq * S->ell * 2 * std::max(1, S->ext_pairs.count) * d * 8},
-      {&W.ct_b, nq * lt * d * 8},
-      {&W.st_a, nq * S->ell * d * 8},
-      {&W.st_b, nq * S->ell * d * 8},
-      {&W.out, nq * S->ell * 2 * d * 8},
-      {&W.fin_tab, nq * lt * 3 * sizeof(void*)},
-      {&W.ext_tab, nq * S->ell * 2 * 3 * sizeof(void*)},
-  };
+  const std::vector<size_t> sizes = WorkspaceSizes(S, cap);
+  DevBuf* bufs[] = {&W.fd,       &W.sd,         &W.pk,          &W.bt_db,      &W.bt_pk,
+                    &W.corr_db,  &W.corr_pk,    &W.db_partial,  &W.pk_partial, &W.yh_spec,
+                    &W.sdb_spec, &W.d_spec,     &W.fin_partial, &W.ext_partial, &W.ct_b,
+                    &W.st_a,     &W.st_b,       &W.out,         &W.fin_tab,    &W.ext_tab};
   W.cap = 0;
-  for (auto& it : items) {
-    Error er = it.b->alloc(it.bytes);
+  for (size_t i = 0; i < sizes.size(); ++i) {
+    Error er = bufs[i]->alloc(sizes[i]);
     if (!er.ok()) return er;
   }
   Error er = W.h_fd.alloc(nq * S->K * 8);
@@ -557,9 +523,9 @@ Error ValidateRequest(const pirg_server* S, size_t fd_len, size_t sd_polys,
 
 // Copies validated requests (reqs[idx[i]]) into the pinned staging buffers
 // and on to HBM.
-Error StageRequests(pirg_server* S, const pirg_request_view* reqs,
+Error StageRequests(pirg_server* S, Slot& slot, const pirg_request_view* reqs,
                     const std::vector<int>& idx) {
-  Workspace& W = S->ws;
+  Workspace& W = slot.ws;
   const size_t d = S->d;
   const int nq = static_cast<int>(idx.size());
   u64* hfd = W.h_fd.as<u64>();
@@ -578,44 +544,51 @@ Error StageRequests(pirg_server* S, const pirg_request_view* reqs,
                 static_cast<size_t>(2 * S->k) * d * 8);
   }
   PIRG_HIP(hipMemcpyAsync(W.fd.p, hfd, static_cast<size_t>(nq) * S->K * 8,
-                          hipMemcpyHostToDevice, S->stream));
+                          hipMemcpyHostToDevice, slot.stream));
   if (S->t > 1) {
     PIRG_HIP(hipMemcpyAsync(W.sd.p, hsd, static_cast<size_t>(nq) * 2 * S->e * d * 8,
-                            hipMemcpyHostToDevice, S->stream));
+                            hipMemcpyHostToDevice, slot.stream));
   }
   PIRG_HIP(hipMemcpyAsync(W.pk.p, hpk, static_cast<size_t>(nq) * 2 * S->k * d * 8,
-                          hipMemcpyHostToDevice, S->stream));
+                          hipMemcpyHostToDevice, slot.stream));
   return Ok();
 }
 
-// ProcessResponse for nq staged requests (pir_server.cc:207-305).
-Error RunBatch(pirg_server* S, int nq) {
-  Workspace& W = S->ws;
-  hipStream_t st = S->stream;
+namespace {
+
+// ProcessResponse for nq staged requests (pir_server.cc:207-305), enqueued on
+// the slot's stream. `timed` records the stage and scan-kernel events (it is
+// off while the sequence is captured into a graph).
+Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
+  Workspace& W = slot.ws;
+  hipStream_t st = slot.stream;
   const int d = S->d;
   const int lt = S->ell * S->t;
   unsigned long long* margin = S->margin.as<unsigned long long>();
-  PIRG_HIP(hipEventRecord(S->ev[0], st));
+  auto mark = [&](hipEvent_t e) -> hipError_t {
+    return timed ? hipEventRecord(e, st) : hipSuccess;
+  };
+  PIRG_HIP(mark(slot.ev[0]));
   // DB product (pir_server.cc:215-229).
   const ScanPlan pd = MakeScanPlan(S->dbL, nq, S->log_q, S->wide_min);
   const ScanPlan pp = MakeScanPlan(S->pkL, nq, S->log_q, S->wide_min);
   PIRG_HIP(LaunchQueryDigits(W.fd.as<u64>(), S->K, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
   PIRG_HIP(LaunchVectorSums(W.fd.as<u64>(), nq, S->K, S->K, S->dbL.c_off,
                             W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
-  PIRG_HIP(hipEventRecord(S->kev[0], st));
+  PIRG_HIP(mark(slot.kev[0]));
   PIRG_HIP(LaunchScan(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
                       W.db_partial.as<u64>(), st));
-  PIRG_HIP(hipEventRecord(S->kev[1], st));
-  PIRG_HIP(hipEventRecord(S->ev[1], st));
+  PIRG_HIP(mark(slot.kev[1]));
+  PIRG_HIP(mark(slot.ev[1]));
   // Pack product (pir_server.cc:237-248): y = concat(y_vec_g).
   PIRG_HIP(LaunchQueryDigits(W.pk.as<u64>(), 2 * S->k * d, S->k * d, S->pkL, pp,
                               W.bt_pk.as<int8_t>(), st));
   PIRG_HIP(LaunchVectorSums(W.pk.as<u64>(), nq, 2 * S->k * d, S->k * d, S->pkL.c_off,
                             W.corr_pk.as<u64>() + nq, W.corr_pk.as<u64>(), st));
-  PIRG_HIP(hipEventRecord(S->kev[2], st));
+  PIRG_HIP(mark(slot.kev[2]));
   PIRG_HIP(LaunchScan(S->pk_tiles.as<int8_t>(), S->pkL, pp, W.bt_pk.as<int8_t>(),
                       W.pk_partial.as<u64>(), st));
-  PIRG_HIP(hipEventRecord(S->kev[3], st));
+  PIRG_HIP(mark(slot.kev[3]));
   // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279).
   PIRG_HIP(LaunchSpectra(W.pk.as<u64>() + static_cast<size_t>(S->k) * d, nq * S->k, S->k,
                          2LL * S->k * d, S->fin_v, S->F, S->tw_fwd.as<double2>(),
@@ -652,7 +625,7 @@ Error RunBatch(pirg_server* S, int nq) {
   FA.log_q1 = S->P.log_q1;
   FA.log_q2 = S->P.log_q2;
   PIRG_HIP(LaunchFinalizeCombine(FA, st));
-  PIRG_HIP(hipEventRecord(S->ev[2], st));
+  PIRG_HIP(mark(slot.ev[2]));
   // EvalPoly (pir_server.cc:126-150) and the modulus switch.
   if (S->t > 1) {
     PIRG_HIP(LaunchSpectra(W.sd.as<u64>(), nq * 2 * S->e, 2 * S->e, 2LL * S->e * d,
@@ -698,41 +671,86 @@ Error RunBatch(pirg_server* S, int nq) {
       PIRG_HIP(LaunchExtCombine(EA, st));
     }
   }
-  PIRG_HIP(hipEventRecord(S->ev[3], st));
-  S->timing_pending = true;
+  PIRG_HIP(mark(slot.ev[3]));
   return Ok();
 }
 
-void CollectTiming(pirg_server* S) {
-  if (!S->timing_pending) return;
-  if (hipEventSynchronize(S->ev[3]) != hipSuccess) return;
+}  // namespace
+
+Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph) {
+  if (nq <= 0 || nq > slot.ws.cap) return Invalid("batch size exceeds the slot's workspace.");
+  if (!use_graph) {
+    Error er = EnqueueBatch(S, slot, nq, true);
+    if (!er.ok()) return er;
+    slot.timing_pending = true;
+    slot.graph_run = false;
+    return Ok();
+  }
+  auto it = slot.graphs.find(nq);
+  if (it == slot.graphs.end()) {
+    PIRG_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal));
+    Error er = EnqueueBatch(S, slot, nq, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(slot.stream, &graph);
+    if (!er.ok()) {
+      if (graph != nullptr) (void)hipGraphDestroy(graph);
+      return er;
+    }
+    PIRG_HIP(ce);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    PIRG_HIP(ie);
+    it = slot.graphs.emplace(nq, exec).first;
+  }
+  PIRG_HIP(hipEventRecord(slot.ev[0], slot.stream));
+  PIRG_HIP(hipGraphLaunch(it->second, slot.stream));
+  PIRG_HIP(hipEventRecord(slot.ev[3], slot.stream));
+  slot.timing_pending = true;
+  slot.graph_run = true;
+  return Ok();
+}
+
+void CollectTiming(pirg_server* S, Slot& slot) {
+  if (!slot.timing_pending) return;
+  if (hipEventSynchronize(slot.ev[3]) != hipSuccess) return;
+  slot.timing_pending = false;
+  if (slot.graph_run) return;  // a graph replay has no per-stage events
   float a = 0, b = 0, c = 0;
-  (void)hipEventElapsedTime(&a, S->ev[0], S->ev[1]);
-  (void)hipEventElapsedTime(&b, S->ev[1], S->ev[2]);
-  (void)hipEventElapsedTime(&c, S->ev[2], S->ev[3]);
+  (void)hipEventElapsedTime(&a, slot.ev[0], slot.ev[1]);
+  (void)hipEventElapsedTime(&b, slot.ev[1], slot.ev[2]);
+  (void)hipEventElapsedTime(&c, slot.ev[2], slot.ev[3]);
+  float ka = 0, kb = 0;
+  (void)hipEventElapsedTime(&ka, slot.kev[0], slot.kev[1]);
+  (void)hipEventElapsedTime(&kb, slot.kev[2], slot.kev[3]);
+  std::lock_guard<std::mutex> lock(S->mu);
   S->db_ms += a;
   S->pack_ms += b;
   S->eval_ms += c;
-  float ka = 0, kb = 0;
-  (void)hipEventElapsedTime(&ka, S->kev[0], S->kev[1]);
-  (void)hipEventElapsedTime(&kb, S->kev[2], S->kev[3]);
   S->db_scan_ms += ka;
   S->pk_scan_ms += kb;
   S->scans += 1;
-  S->timing_pending = false;
 }
 
-Error FetchOutputs(pirg_server* S, int nq, uint64_t* const* outs,
+Error EnqueueFetch(pirg_server* S, Slot& slot, int nq) {
+  Workspace& W = slot.ws;
+  PIRG_HIP(hipMemcpyAsync(W.h_out.p, W.out.p, static_cast<size_t>(nq) * ResponseWords(S) * 8,
+                          hipMemcpyDeviceToHost, slot.stream));
+  return Ok();
+}
+
+namespace {
+
+Error FetchOutputs(pirg_server* S, Slot& slot, int nq, uint64_t* const* outs,
                    const std::vector<int>* idx) {
-  Workspace& W = S->ws;
-  const size_t per = static_cast<size_t>(S->ell) * 2 * S->d;
-  PIRG_HIP(hipMemcpyAsync(W.h_out.p, W.out.p, static_cast<size_t>(nq) * per * 8,
-                          hipMemcpyDeviceToHost, S->stream));
-  PIRG_HIP(hipStreamSynchronize(S->stream));
-  CollectTiming(S);
+  Error er = EnqueueFetch(S, slot, nq);
+  if (!er.ok()) return er;
+  PIRG_HIP(hipStreamSynchronize(slot.stream));
+  CollectTiming(S, slot);
+  const size_t per = ResponseWords(S);
   for (int i = 0; i < nq; ++i) {
     const int dst = idx ? (*idx)[i] : i;
-    std::memcpy(outs[dst], W.h_out.as<u64>() + static_cast<size_t>(i) * per, per * 8);
+    std::memcpy(outs[dst], slot.ws.h_out.as<u64>() + static_cast<size_t>(i) * per, per * 8);
   }
   return Ok();
 }
@@ -802,6 +820,7 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
                      uint64_t* out, bool is_u16) {
   Error er = CheckDevice();
   if (!er.ok()) return er;
+  PIRG_HIP(ConfigureScanKernels());
   hipStream_t s = nullptr;
   const int planes = is_u16 ? 2 : 4;
   const u64 c_off = is_u16 ? 32896ULL : 128ULL * (1 + 256 + 65536);
@@ -927,13 +946,13 @@ int pirg_process_batch(pirg_server* S, int nreq, const pirg_request_view* reqs,
   if (!first_bad.ok()) Report(first_bad, err, errlen);
   if (idx.empty()) return 0;
   const int cap = static_cast<int>(idx.size());
-  Error er = EnsureWorkspace(S, cap);
+  Error er = EnsureWorkspace(S, S->main, cap);
   if (!er.ok()) return Report(er, err, errlen);
-  er = StageRequests(S, reqs, idx);
+  er = StageRequests(S, S->main, reqs, idx);
   if (!er.ok()) return Report(er, err, errlen);
-  er = RunBatch(S, cap);
+  er = RunBatch(S, S->main, cap, S->use_graphs);
   if (!er.ok()) return Report(er, err, errlen);
-  er = FetchOutputs(S, cap, outs, &idx);
+  er = FetchOutputs(S, S->main, cap, outs, &idx);
   if (!er.ok()) return Report(er, err, errlen);
   return 0;
 }
@@ -947,11 +966,11 @@ int pirg_batch_stage(pirg_server* S, int nreq, const pirg_request_view* reqs,
     if (!er.ok()) return Report(er, err, errlen);
     idx[i] = i;
   }
-  Error er = EnsureWorkspace(S, nreq);
+  Error er = EnsureWorkspace(S, S->main, nreq);
   if (!er.ok()) return Report(er, err, errlen);
-  er = StageRequests(S, reqs, idx);
+  er = StageRequests(S, S->main, reqs, idx);
   if (!er.ok()) return Report(er, err, errlen);
-  if (hipStreamSynchronize(S->stream) != hipSuccess) {
+  if (hipStreamSynchronize(S->main.stream) != hipSuccess) {
     return Report(Internal("stream synchronize failed"), err, errlen);
   }
   S->staged = nreq;
@@ -962,12 +981,12 @@ int pirg_batch_run(pirg_server* S, int nreq, char* err, size_t errlen) {
   if (nreq <= 0 || nreq > S->staged) {
     return Report(Invalid("batch size exceeds the staged requests."), err, errlen);
   }
-  Error er = RunBatch(S, nreq);
+  Error er = RunBatch(S, S->main, nreq, S->use_graphs);
   if (!er.ok()) return Report(er, err, errlen);
-  if (hipStreamSynchronize(S->stream) != hipSuccess) {
+  if (hipStreamSynchronize(S->main.stream) != hipSuccess) {
     return Report(Internal("stream synchronize failed"), err, errlen);
   }
-  CollectTiming(S);
+  CollectTiming(S, S->main);
   return 0;
 }
 
@@ -976,14 +995,21 @@ int pirg_batch_fetch(pirg_server* S, int nreq, uint64_t* const* outs, char* err,
   if (nreq <= 0 || nreq > S->staged) {
     return Report(Invalid("batch size exceeds the staged requests."), err, errlen);
   }
-  Error er = FetchOutputs(S, nreq, outs, nullptr);
+  Error er = FetchOutputs(S, S->main, nreq, outs, nullptr);
   if (!er.ok()) return Report(er, err, errlen);
+  return 0;
+}
+
+int pirg_server_use_graphs(pirg_server* S, int enable) {
+  if (S == nullptr) return PIRG_INVALID_ARGUMENT;
+  S->use_graphs = enable != 0;
   return 0;
 }
 
 int pirg_server_stage_times(pirg_server* S, double* db_mult_ms, double* packing_ms,
                             double* poly_eval_ms) {
-  CollectTiming(S);
+  CollectTiming(S, S->main);
+  std::lock_guard<std::mutex> lock(S->mu);
   if (db_mult_ms) *db_mult_ms = S->db_ms;
   if (packing_ms) *packing_ms = S->pack_ms;
   if (poly_eval_ms) *poly_eval_ms = S->eval_ms;
@@ -992,7 +1018,8 @@ int pirg_server_stage_times(pirg_server* S, double* db_mult_ms, double* packing_
 
 int pirg_server_scan_times(pirg_server* S, double* db_scan_ms, double* pack_scan_ms,
                            long long* launches, int reset) {
-  CollectTiming(S);
+  CollectTiming(S, S->main);
+  std::lock_guard<std::mutex> lock(S->mu);
   if (db_scan_ms) *db_scan_ms = S->db_scan_ms;
   if (pack_scan_ms) *pack_scan_ms = S->pk_scan_ms;
   if (launches) *launches = S->scans;
@@ -1005,7 +1032,7 @@ int pirg_server_scan_times(pirg_server* S, double* db_scan_ms, double* pack_scan
 
 double pirg_server_fft_margin(pirg_server* S, int reset) {
   unsigned long long bits = 0;
-  (void)hipStreamSynchronize(S->stream);
+  (void)hipStreamSynchronize(S->main.stream);
   (void)hipMemcpy(&bits, S->margin.p, 8, hipMemcpyDeviceToHost);
   if (reset) (void)hipMemset(S->margin.p, 0, 8);
   double v;

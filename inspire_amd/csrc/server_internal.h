@@ -1,0 +1,166 @@
+// Host-side state shared by server.cpp (Create / ProcessResponse / batches)
+// and serve.cpp (the serving loop).
+#ifndef INSPIRE_AMD_SERVER_INTERNAL_H_
+#define INSPIRE_AMD_SERVER_INTERNAL_H_
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pirgpu.h"
+#include "pirg_internal.h"
+
+namespace pirg {
+
+struct Error {
+  int code = 0;
+  std::string msg;
+  bool ok() const { return code == 0; }
+};
+
+inline Error Ok() { return Error{}; }
+inline Error Invalid(const std::string& m) { return Error{PIRG_INVALID_ARGUMENT, m}; }
+inline Error Internal(const std::string& m) { return Error{PIRG_INTERNAL, m}; }
+inline Error HipErr(hipError_t e, const char* what) {
+  return Error{PIRG_INTERNAL, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+int Report(const Error& e, char* err, size_t errlen);
+
+#define PIRG_HIP(expr)                              \
+  do {                                              \
+    hipError_t _e = (expr);                         \
+    if (_e != hipSuccess) return HipErr(_e, #expr); \
+  } while (0)
+
+// Device allocation released with its owner.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset();
+  Error alloc(size_t n);
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Pinned host allocation released with its owner.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf&) = delete;
+  HostBuf& operator=(const HostBuf&) = delete;
+  ~HostBuf() { reset(); }
+  void reset();
+  Error alloc(size_t n);
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Per-batch working memory, sized for `cap` requests.
+struct Workspace {
+  int cap = 0;
+  DevBuf fd, sd, pk;             // staged requests
+  DevBuf bt_db, bt_pk;           // B-operand tiles
+  DevBuf corr_db, corr_pk;
+  DevBuf db_partial, pk_partial;
+  DevBuf yh_spec, sdb_spec, d_spec;
+  DevBuf fin_partial, ext_partial;
+  DevBuf ct_b, st_a, st_b, out;
+  DevBuf fin_tab, ext_tab;       // IpKernel pointer tables
+  HostBuf h_fd, h_sd, h_pk, h_out;
+};
+
+// One execution context: a workspace, its stream, timing events and the
+// hipGraphs captured for it (one per batch size). The server answers direct
+// calls on its main slot; the serving loop owns several.
+struct Slot {
+  Workspace ws;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // stage boundaries
+  hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};  // DB / pack scans
+  hipEvent_t done = nullptr;                                 // serving: batch finished
+  bool timing_pending = false;
+  bool graph_run = false;  // the pending timing came from a graph launch
+  std::map<int, hipGraphExec_t> graphs;
+  Slot() = default;
+  Slot(const Slot&) = delete;
+  Slot& operator=(const Slot&) = delete;
+  ~Slot();
+  Error Init();
+  void DropGraphs();
+};
+
+class ServeLoop;
+
+}  // namespace pirg
+
+struct pirg_server {
+  pirg_params P{};
+  int device = 0;
+  int d = 0, log_d = 0, N = 0, log_N = 0, t = 0, ell = 0, k = 0, e = 0;
+  int r = 0, K = 0, M = 0, log_q = 0;
+  pirg::u64 qmask = 0;
+  pirg::FftDims F{};
+  pirg::ChunkRule fin_u{}, fin_v{}, ext_u{}, ext_v{};
+  pirg::PairList fin_pairs{}, ext_pairs{};
+  pirg::Error deferred;  // error ProcessResponse must return (chunk validation)
+  pirg::LimbLayout dbL{}, pkL{};
+  // Batches of at least this many requests use the MFMA-bound WideScan.
+  int wide_min = 10;
+  pirg::DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
+  pirg::Slot main;
+  int staged = 0;
+  // pirg_process_batch / pirg_batch_run replay hipGraphs instead of eager
+  // launches (the serving loop always does).
+  bool use_graphs = false;
+  // Cumulative stage times (GetDbMultiplyTime / GetPackingTime /
+  // GetPolyEvalTime) and scan-kernel times, guarded by `mu` (the serving
+  // loop adds to them from its own thread).
+  std::mutex mu;
+  double db_ms = 0, pack_ms = 0, eval_ms = 0;
+  double db_scan_ms = 0, pk_scan_ms = 0;
+  long long scans = 0;
+  std::unique_ptr<pirg::ServeLoop> serve;
+  // Defined beside ServeLoop (serve.cpp), where its type is complete.
+  pirg_server();
+  ~pirg_server();
+};
+
+namespace pirg {
+
+// pir_server.cc:161-194 request validation, in the reference's order.
+Error ValidateRequest(const pirg_server* S, size_t fd_len, size_t sd_polys, size_t pk_polys);
+// Bytes of device workspace a slot needs for `cap` requests.
+size_t WorkspaceBytes(const pirg_server* S, int cap);
+Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap);
+// Copies requests reqs[idx[i]] into the slot's pinned buffers and enqueues
+// the host-to-device copies on its stream.
+Error StageRequests(pirg_server* S, Slot& slot, const pirg_request_view* reqs,
+                    const std::vector<int>& idx);
+// Enqueues ProcessResponse for the nq staged requests on the slot's stream,
+// eagerly (with per-stage timing) or as a replay of the slot's hipGraph for
+// nq (captured on first use).
+Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph);
+// Enqueues the device-to-host copy of nq responses into the slot's pinned
+// output buffer.
+Error EnqueueFetch(pirg_server* S, Slot& slot, int nq);
+// Adds the slot's finished batch to the server's stage times.
+void CollectTiming(pirg_server* S, Slot& slot);
+size_t ResponseWords(const pirg_server* S);
+
+}  // namespace pirg
+
+#endif  // INSPIRE_AMD_SERVER_INTERNAL_H_

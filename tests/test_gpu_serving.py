@@ -1,0 +1,164 @@
+"""hipGraph replays and the serving loop (continuous batching), on the GPU.
+
+Parity here is "batched == sequential == oracle": the reference answers one
+request per call, so every answer of a graph replay or of a served batch
+must equal, bit for bit, the same request answered alone, and the oracle's
+ProcessResponse on the same inputs.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import inspire_amd as ia
+import oracle as O
+from helpers import gpu_request, server_from_material
+
+pytestmark = pytest.mark.gpu
+
+P = O.metrics_tool_params(n=4096, log_d=8, t=4, ell=2)
+
+
+@pytest.fixture(scope="module")
+def material():
+    return O.synthetic_material(P, seed=77)
+
+
+@pytest.fixture(scope="module")
+def requests():
+    return [O.synthetic_request(P, seed=300 + i) for i in range(40)]
+
+
+def test_graph_replays_match_eager(material, requests):
+    reqs = requests[:12]
+    with server_from_material(P, material) as srv:
+        eager, st, _ = srv.process_batch([gpu_request(r) for r in reqs])
+        assert st == [0] * len(reqs)
+        srv.use_graphs(True)
+        for rep in range(4):
+            got, st2, _ = srv.process_batch([gpu_request(r) for r in reqs])
+            assert st2 == [0] * len(reqs)
+            for a, b in zip(eager, got):
+                np.testing.assert_array_equal(b.ciphertexts, a.ciphertexts)
+            # Eager work between the replays.
+            srv.use_graphs(False)
+            one = srv.process_response(gpu_request(reqs[rep]))
+            np.testing.assert_array_equal(one.ciphertexts, eager[rep].ciphertexts)
+            srv.use_graphs(True)
+        # Another batch size gets its own graph.
+        got5, _, _ = srv.process_batch([gpu_request(r) for r in reqs[:5]])
+        for a, b in zip(eager[:5], got5):
+            np.testing.assert_array_equal(b.ciphertexts, a.ciphertexts)
+    for i in (0, 7):
+        want, _, _ = O.process_response(P, material, reqs[i])
+        np.testing.assert_array_equal(eager[i].ciphertexts, want)
+
+
+def _drain(srv, n, timeout_ms=60000):
+    got = {}
+    while len(got) < n:
+        r = srv.poll(timeout_ms)
+        assert r is not None, f"no result after {timeout_ms} ms ({len(got)} of {n})"
+        tag, resp, status, lat, msg = r
+        assert status == 0, msg
+        assert lat > 0
+        got[tag] = resp.ciphertexts
+    return got
+
+
+def test_serving_alone_equals_in_the_mix(material, requests):
+    with server_from_material(P, material) as srv:
+        srv.serve_start(max_batch=8, slots=2, max_outstanding=64)
+        try:
+            srv.submit(gpu_request(requests[0]), tag=1000)
+            alone = _drain(srv, 1)[1000]
+            # The same request again, in the middle of a burst submitted from
+            # two threads.
+            order = list(range(1, len(requests)))
+            order.insert(17, 0)
+            halves = [order[0::2], order[1::2]]
+
+            def submitter(ids):
+                for i in ids:
+                    srv.submit(gpu_request(requests[i]), tag=i)
+
+            threads = [threading.Thread(target=submitter, args=(h,)) for h in halves]
+            for th in threads:
+                th.start()
+            for th in threads:
+                th.join()
+            mixed = _drain(srv, len(order))
+            stats = srv.serve_stats()
+        finally:
+            srv.serve_stop()
+    np.testing.assert_array_equal(mixed[0], alone)
+    want, _, _ = O.process_response(P, material, requests[0])
+    np.testing.assert_array_equal(alone, want)
+    for i in (5, 23, 39):
+        want, _, _ = O.process_response(P, material, requests[i])
+        np.testing.assert_array_equal(mixed[i], want)
+    assert stats["answered"] == len(order) + 1
+    assert stats["slots"] >= 1 and stats["max_batch"] >= 1
+
+
+def test_serving_batches_a_burst(material, requests):
+    # One slot: while it answers the first arrivals the rest of the burst
+    # queues and joins the next batches.
+    with server_from_material(P, material) as srv:
+        srv.serve_start(max_batch=16, slots=1, max_outstanding=64)
+        try:
+            for i, r in enumerate(requests[:32]):
+                srv.submit(gpu_request(r), tag=i)
+            got = _drain(srv, 32)
+            stats = srv.serve_stats()
+        finally:
+            srv.serve_stop()
+    assert stats["batches"] < 32 and stats["mean_batch"] > 1.0, stats
+    solo, _, _ = O.process_response(P, material, requests[31])
+    np.testing.assert_array_equal(got[31], solo)
+
+
+def test_serving_rejects_invalid_and_full_queue(material, requests):
+    with server_from_material(P, material) as srv:
+        with pytest.raises(ia.PirError) as ei:
+            srv.submit(gpu_request(requests[0]), tag=1)
+        assert ei.value.code == ia.FAILED_PRECONDITION
+        srv.serve_start(max_batch=4, slots=1, max_outstanding=2)
+        try:
+            bad = O.Request(requests[0].first_dim, requests[0].second_dim,
+                            requests[0].packing_key[:-1])
+            with pytest.raises(ia.PirError) as ei:
+                srv.submit(gpu_request(bad), tag=2)
+            assert ei.value.code == ia.INVALID_ARGUMENT
+            assert ei.value.message == "Packing key must contain 2 * num_digits polynomials."
+            srv.submit(gpu_request(requests[1]), tag=3)
+            srv.submit(gpu_request(requests[2]), tag=4)
+            # Two submitted and not yet polled: the third is refused.
+            with pytest.raises(ia.PirError) as ei:
+                srv.submit(gpu_request(requests[3]), tag=5)
+            assert ei.value.code == ia.RESOURCE_EXHAUSTED
+            first = _drain(srv, 1)
+            srv.submit(gpu_request(requests[3]), tag=5)
+            rest = _drain(srv, 2)
+        finally:
+            srv.serve_stop()
+    got = {**first, **rest}
+    assert sorted(got) == [3, 4, 5]
+    want, _, _ = O.process_response(P, material, requests[3])
+    np.testing.assert_array_equal(got[5], want)
+
+
+def test_direct_calls_while_serving(material, requests):
+    with server_from_material(P, material) as srv:
+        srv.serve_start(max_batch=8, slots=2, max_outstanding=64)
+        try:
+            for i in range(8):
+                srv.submit(gpu_request(requests[i]), tag=i)
+            direct = srv.process_response(gpu_request(requests[9])).ciphertexts
+            served = _drain(srv, 8)
+        finally:
+            srv.serve_stop()
+    want, _, _ = O.process_response(P, material, requests[9])
+    np.testing.assert_array_equal(direct, want)
+    want, _, _ = O.process_response(P, material, requests[4])
+    np.testing.assert_array_equal(served[4], want)
