@@ -187,6 +187,34 @@ int pirg_serve_stats(pirg_server* server, pirg_serve_stats_t* stats, int reset);
  * dropped). pirg_server_destroy stops a running loop itself. */
 int pirg_serve_stop(pirg_server* server);
 
+/* ---- Batched LWE-to-RLWE packing. MatrixPack (crypto/lwes_to_rlwe.cc:481-508,
+ * declared at crypto/lwes_to_rlwe.h:126-133) for njobs independent jobs,
+ * each with its own PreprocessMatrixPackOutput (lwes_to_rlwe.h:42-47), made
+ * resident in HBM:
+ *   pack[j]:    d x k*d matrix (int32 condensed words, or int16 values when
+ *               params->mat_bytes == 2);
+ *   a_tilde[j]: a_tilde_agg, d coefficients;  t_vec_h[j]: k x d.
+ * params supplies log_degree, log_q, pack_log_digit/num_digits, mat_bytes.
+ * The reference packs exactly d LWEs per call (lwes_to_rlwe.cc:428); a job
+ * with m < d LWEs passes its m b-values and is zero-padded. */
+typedef struct pirg_packer pirg_packer;
+int pirg_packer_create(const pirg_params* params, int njobs, const void* const* pack,
+                       const uint64_t* const* a_tilde, const uint64_t* const* t_vec_h,
+                       pirg_packer** out, char* err, size_t errlen);
+/* One MatrixPack per job, all jobs in one launch (a hipGraph, captured on the
+ * first call): b[j] holds m[j] <= d LWE b-values, keys[j] the packing key
+ * y_vec_g || y_vec_h (2k x d; jobs may share one key pointer), out[j]
+ * receives the RLWE ciphertext (a, b) as 2 x d. Host pointers. */
+int pirg_packer_run(pirg_packer* packer, const uint64_t* const* b, const int* m,
+                    const uint64_t* const* keys, uint64_t* const* out, char* err,
+                    size_t errlen);
+/* Graph replay (default) or eager launches, which also time the scan. */
+int pirg_packer_use_graph(pirg_packer* packer, int enable);
+/* Cumulative batch time and (eager runs) pack-scan kernel time, in ms. */
+int pirg_packer_times(pirg_packer* packer, double* total_ms, double* scan_ms,
+                      long long* runs, int reset);
+void pirg_packer_destroy(pirg_packer* packer);
+
 /* PirServer::GetDbMultiplyTime / GetPackingTime / GetPolyEvalTime
  * (pir_server.h:70-72): cumulative GPU time, in milliseconds, of the DB
  * product, the packing (pack product + FinalizeMatrixPack) and EvalPoly

@@ -55,6 +55,7 @@ def lib() -> ctypes.CDLL:
         _lib.pirg_version.restype = ctypes.c_char_p
         _lib.pirg_server_fft_margin.restype = ctypes.c_double
         _lib.pirg_server_destroy.restype = None
+        _lib.pirg_packer_destroy.restype = None
         _lib.pirg_serve_submit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                            ctypes.c_char_p, ctypes.c_size_t]
         _lib.pirg_serve_poll.argtypes = [ctypes.c_void_p, ctypes.c_int, _u64p, _u64p,
@@ -436,6 +437,80 @@ class PirServer:
 
     def serve_stop(self) -> None:
         lib().pirg_serve_stop(self._h)
+
+
+class PirPacker:
+    """Batched MatrixPack (crypto/lwes_to_rlwe.cc:481-508) over independent
+    jobs with resident PreprocessMatrixPackOutput material."""
+
+    def __init__(self, handle, params: PirParams, njobs: int):
+        self._h = handle
+        self.params = params
+        self.njobs = njobs
+
+    @classmethod
+    def create(cls, params: PirParams, pack: Sequence[np.ndarray],
+               a_tilde: Sequence[np.ndarray], t_vec_h: Sequence[np.ndarray]) -> "PirPacker":
+        J = len(pack)
+        if not (len(a_tilde) == len(t_vec_h) == J):
+            raise ValueError("pack, a_tilde and t_vec_h need one entry per job")
+        packs = [np.ascontiguousarray(p) for p in pack]
+        ats = [_u64(a) for a in a_tilde]
+        ths = [_u64_2d(t) for t in t_vec_h]
+        pk_arr = (ctypes.c_void_p * J)(*[p.ctypes.data for p in packs])
+        at_arr = (_u64p * J)(*[a.ctypes.data_as(_u64p) for a in ats])
+        th_arr = (_u64p * J)(*[t.ctypes.data_as(_u64p) for t in ths])
+        cp = params.c()
+        handle = ctypes.c_void_p()
+        err = _err()
+        rc = lib().pirg_packer_create(ctypes.byref(cp), J, pk_arr, at_arr, th_arr,
+                                      ctypes.byref(handle), err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return cls(handle, params, J)
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().pirg_packer_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def use_graph(self, enable: bool = True) -> None:
+        lib().pirg_packer_use_graph(self._h, int(enable))
+
+    def run(self, b: Sequence[np.ndarray], keys: Sequence[np.ndarray]) -> np.ndarray:
+        """MatrixPack for every job: b[j] (m_j <= d LWE b-values), keys[j] the
+        packing key y_vec_g || y_vec_h (2k x d). Returns (njobs, 2, d)."""
+        J, d = self.njobs, self.params.degree
+        bs = [_u64(x) for x in b]
+        ks = [_u64_2d(x) for x in keys]
+        out = np.zeros((J, 2, d), dtype=np.uint64)
+        b_arr = (_u64p * J)(*[x.ctypes.data_as(_u64p) for x in bs])
+        m_arr = (ctypes.c_int * J)(*[x.size for x in bs])
+        k_arr = (_u64p * J)(*[x.ctypes.data_as(_u64p) for x in ks])
+        o_arr = (_u64p * J)(*[out[j].ctypes.data_as(_u64p) for j in range(J)])
+        err = _err()
+        rc = lib().pirg_packer_run(self._h, b_arr, m_arr, k_arr, o_arr, err,
+                                   ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return out
+
+    def times_ms(self, reset: bool = False):
+        """(total batch ms, pack-scan ms of eager runs, runs)."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        lib().pirg_packer_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n),
+                                int(reset))
+        return a.value, b.value, n.value
 
 
 def matrix_multiply_u16(cdata: np.ndarray, cols: int, vec: np.ndarray) -> np.ndarray:

@@ -164,7 +164,7 @@ __global__ void FinalizeCombineKernel(FinalizeArgs A) {
     for (int k = 0; k < A.ksplit_db; ++k) {
       bprime += A.db_partial[((long long)k * A.nq + q) * A.M + row];
     }
-    u64 bpart = A.corr_pk[q];
+    u64 bpart = A.corr_pk[A.corr_rows > 0 ? row / A.corr_rows : q];
     for (int k = 0; k < A.ksplit_pk; ++k) {
       bpart += A.pk_partial[((long long)k * A.nq + q) * A.M + row];
     }

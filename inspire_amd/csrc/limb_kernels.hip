@@ -284,14 +284,17 @@ __device__ __forceinline__ void WaveEpilogue(const v4i (&acc)[NS][NT], u64* red,
 template <int NP, int R, int T, int D>
 __global__ __launch_bounds__(256) void NarrowScanKernel(
     const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps,
-    const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
+    const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg, int group_rows,
     u64* __restrict__ partial) {
   __shared__ u64 red[4][R * 16 * kQMax];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int strip0 = (blockIdx.x * 4 + wave) * R;
   const int ks = blockIdx.y;
-  const int grp = blockIdx.z;
+  // Grouped product: the wave's rows (R strips, inside one block of
+  // group_rows rows) use their block's group and write the single-query
+  // layout.
+  const int grp = group_rows > 0 ? min(strip0 * 16, rows - 1) / group_rows : blockIdx.z;
   const int kb0 = ks * kbps;
   const int nkb = min(kblocks, kb0 + kbps) - kb0;
   const v4i* A = reinterpret_cast<const v4i*>(tiles);
@@ -352,8 +355,8 @@ __global__ __launch_bounds__(256) void NarrowScanKernel(
       compute(a[d], b[d]);
     }
   }
-  WaveEpilogue<R, T>(acc, red[wave], lane, ns, qpg, grp, nq, rows,
-                     (long long)strip0 * 16, ks, partial);
+  WaveEpilogue<R, T>(acc, red[wave], lane, ns, qpg, group_rows > 0 ? 0 : grp,
+                     group_rows > 0 ? 1 : nq, rows, (long long)strip0 * 16, ks, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -515,10 +518,11 @@ constexpr int kNarrowD = 4;
 template <int NP, int T>
 hipError_t LaunchNarrowT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                          const int8_t* btiles, u64* partial, hipStream_t s) {
-  dim3 grid((L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR), P.ksplit, P.ngp);
+  dim3 grid((L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR), P.ksplit,
+            P.group_rows > 0 ? 1 : P.ngp);
   hipLaunchKernelGGL((NarrowScanKernel<NP, kNarrowR, T, kNarrowD>), grid, dim3(256), 0, s,
                      tiles, L.strips, L.rows, L.kblocks, P.kb_per_split, btiles, P.ngp,
-                     P.nq, P.ns, P.qpg, partial);
+                     P.nq, P.ns, P.qpg, P.group_rows, partial);
   return hipGetLastError();
 }
 
@@ -733,6 +737,28 @@ ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_qu
   return P;
 }
 
+ScanPlan MakeGroupedScanPlan(const LimbLayout& L, int ngroups, int group_rows, int out_bits) {
+  ScanPlan P{};
+  P.wide = 0;
+  P.nq = ngroups;
+  P.ns = (out_bits + 7) / 8;
+  P.ctg = 1;
+  P.qpg = 1;
+  P.ngp = ngroups;
+  P.group_rows = group_rows;
+  const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
+  const int row_wgs = (L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR);
+  int ks = (2048 + row_wgs - 1) / row_wgs;
+  const int min_split = (L.kblocks + max_kb - 1) / max_kb;
+  if (ks < min_split) ks = min_split;
+  if (ks > L.kblocks) ks = L.kblocks;
+  if (ks < 1) ks = 1;
+  P.kb_per_split = (L.kblocks + ks - 1) / ks;
+  P.ksplit = (L.kblocks + P.kb_per_split - 1) / P.kb_per_split;
+  P.btile_bytes = static_cast<size_t>(L.kblocks + 1) * P.ngp * P.ctg * kTileBytes;
+  return P;
+}
+
 hipError_t LaunchQueryDigits(const u64* vec, int stride, int len, const LimbLayout& L,
                              const ScanPlan& P, int8_t* btiles, hipStream_t s) {
   // kblocks + 1 blocks: the last one is all zero (its k's are >= len).
@@ -761,6 +787,10 @@ hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& 
                       const int8_t* btiles, u64* partial, hipStream_t s) {
   if (P.nq <= 0) return hipSuccess;
   if (P.qpg < 1 || P.ngp < 1) return hipErrorInvalidValue;
+  if (P.group_rows > 0 && (P.wide || P.group_rows % (16 * kNarrowR) != 0 ||
+                           static_cast<long long>(P.group_rows) * P.ngp < L.rows)) {
+    return hipErrorInvalidValue;
+  }
   switch (L.planes) {
     case 1: return LaunchScanNP<1>(tiles, L, P, btiles, partial, s);
     case 2: return LaunchScanNP<2>(tiles, L, P, btiles, partial, s);

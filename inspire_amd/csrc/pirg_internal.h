@@ -109,11 +109,18 @@ struct ScanPlan {
   int ksplit;        // k-ranges (exact int32 sums and occupancy)
   int kb_per_split;
   size_t btile_bytes;
+  // Grouped product (NarrowScan only): row block b of group_rows rows is
+  // multiplied by query b alone, and partial is [ksplit][rows]. 0: every row
+  // by every query.
+  int group_rows;
 };
 
 // wide_min_queries: batches of at least this many queries use WideScan even
 // when they would fit NarrowScan.
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries);
+// Grouped plan: ngroups blocks of group_rows rows (a multiple of 64), block b
+// times vector b.
+ScanPlan MakeGroupedScanPlan(const LimbLayout& L, int ngroups, int group_rows, int out_bits);
 // Sets the launch attributes the scan kernels need; call once per process
 // (and device) before the first scan, outside any stream capture.
 hipError_t ConfigureScanKernels();
@@ -160,7 +167,8 @@ struct FinalizeArgs {
   const u64* corr_db;      // [nq]
   const u64* pk_partial;   // [ksplit_pk][nq][M]
   int ksplit_pk;
-  const u64* corr_pk;      // [nq]
+  const u64* corr_pk;      // [nq], or [M / corr_rows] when corr_rows > 0
+  int corr_rows;           // > 0: pack correction per block of corr_rows rows
   const u64* fin_partial;  // [nq][ell*t][npairs][d]
   int npairs;
   const u64* a_tilde;      // [ell*t][d]

@@ -95,6 +95,8 @@ namespace {
 
 bool IsPow2(u64 x) { return x != 0 && (x & (x - 1)) == 0; }
 
+}  // namespace
+
 Error CheckDevice() {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
@@ -135,7 +137,6 @@ Error MakeRules(int n_terms, int d, int u_bits, int v_bits, int chunk,
   return Ok();
 }
 
-}  // namespace
 }  // namespace pirg
 
 using namespace pirg;
@@ -213,6 +214,8 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
   return Ok();
 }
 
+}  // namespace
+
 // crypto/fft.h:71-80, from the g++-built table (twiddles.cpp), reordered by
 // stage for the kernels: stage s (1..log2 n) occupies entries
 // [2^(s-1) - 1, 2^s - 1), and entry 2^(s-1) - 1 + k is tw[k * (n >> s)], the
@@ -244,38 +247,7 @@ LimbLayout MakeLayout(int rows, int cols, int planes, u64 c_off) {
   return L;
 }
 
-// Uploads rows [0, rows) of a host matrix in chunks and converts them to
-// tiles. `rowbytes` is the size of one host row.
-template <typename Convert>
-Error UploadTiles(const void* host, size_t rowbytes, int rows, bool host_is_device,
-                  hipStream_t s, Convert conv) {
-  if (host_is_device) {
-    PIRG_HIP(conv(host, 0, rows));
-    PIRG_HIP(hipStreamSynchronize(s));
-    return Ok();
-  }
-  // Chunk size of the staged upload (PIRG_UPLOAD_CHUNK_BYTES overrides it so
-  // tests can exercise the multi-chunk path on small matrices).
-  size_t chunk_bytes = static_cast<size_t>(256) << 20;
-  if (const char* env = std::getenv("PIRG_UPLOAD_CHUNK_BYTES")) {
-    const long long v = std::atoll(env);
-    if (v > 0) chunk_bytes = static_cast<size_t>(v);
-  }
-  int rows_per = static_cast<int>(std::max<size_t>(16, chunk_bytes / std::max<size_t>(rowbytes, 1)));
-  rows_per = (rows_per / 16) * 16;
-  if (rows_per < 16) rows_per = 16;
-  DevBuf stage;
-  Error er = stage.alloc(static_cast<size_t>(std::min(rows_per, rows)) * rowbytes);
-  if (!er.ok()) return er;
-  for (int r0 = 0; r0 < rows; r0 += rows_per) {
-    const int nr = std::min(rows_per, rows - r0);
-    PIRG_HIP(hipMemcpyAsync(stage.p, static_cast<const char*>(host) + static_cast<size_t>(r0) * rowbytes,
-                            static_cast<size_t>(nr) * rowbytes, hipMemcpyHostToDevice, s));
-    PIRG_HIP(conv(stage.p, r0, nr));
-    PIRG_HIP(hipStreamSynchronize(s));
-  }
-  return Ok();
-}
+namespace {
 
 Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts,
                     int db_cols, const std::vector<const void*>& pack_parts,

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -160,6 +162,50 @@ Error EnqueueFetch(pirg_server* S, Slot& slot, int nq);
 // Adds the slot's finished batch to the server's stage times.
 void CollectTiming(pirg_server* S, Slot& slot);
 size_t ResponseWords(const pirg_server* S);
+
+Error CheckDevice();
+// crypto/lwes_to_rlwe.cc:60-74 and crypto/external_product.cc:100-102.
+int ChunkFor(int n_terms, int d);
+// crypto/polynomial_fft.cc:90-107.
+Error MakeRules(int n_terms, int d, int u_bits, int v_bits, int chunk, bool u_signed,
+                ChunkRule* ur, ChunkRule* vr);
+// Stage-ordered FFT twiddle tables (n - 1 complex values each).
+void Twiddles(int n, std::vector<double2>* fwd, std::vector<double2>* bwd);
+LimbLayout MakeLayout(int rows, int cols, int planes, u64 c_off);
+
+// Uploads rows [0, rows) of a host matrix in chunks and converts them to
+// tiles. `rowbytes` is the size of one host row.
+template <typename Convert>
+inline Error UploadTiles(const void* host, size_t rowbytes, int rows, bool host_is_device,
+                  hipStream_t s, Convert conv) {
+  if (host_is_device) {
+    PIRG_HIP(conv(host, 0, rows));
+    PIRG_HIP(hipStreamSynchronize(s));
+    return Ok();
+  }
+  // Chunk size of the staged upload (PIRG_UPLOAD_CHUNK_BYTES overrides it so
+  // tests can exercise the multi-chunk path on small matrices).
+  size_t chunk_bytes = static_cast<size_t>(256) << 20;
+  if (const char* env = std::getenv("PIRG_UPLOAD_CHUNK_BYTES")) {
+    const long long v = std::atoll(env);
+    if (v > 0) chunk_bytes = static_cast<size_t>(v);
+  }
+  int rows_per = static_cast<int>(std::max<size_t>(16, chunk_bytes / std::max<size_t>(rowbytes, 1)));
+  rows_per = (rows_per / 16) * 16;
+  if (rows_per < 16) rows_per = 16;
+  DevBuf stage;
+  Error er = stage.alloc(static_cast<size_t>(std::min(rows_per, rows)) * rowbytes);
+  if (!er.ok()) return er;
+  for (int r0 = 0; r0 < rows; r0 += rows_per) {
+    const int nr = std::min(rows_per, rows - r0);
+    PIRG_HIP(hipMemcpyAsync(stage.p, static_cast<const char*>(host) + static_cast<size_t>(r0) * rowbytes,
+                            static_cast<size_t>(nr) * rowbytes, hipMemcpyHostToDevice, s));
+    PIRG_HIP(conv(stage.p, r0, nr));
+    PIRG_HIP(hipStreamSynchronize(s));
+  }
+  return Ok();
+}
+
 
 }  // namespace pirg
 
