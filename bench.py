@@ -8,12 +8,22 @@ are synthetic but range-faithful (DB uniform in [0, p), pack entries uniform
 in [-2^30, 2^30), t_vec_h signed 19-bit digits, a_tilde / second_dim_a /
 requests uniform mod 2^52), all seeded and resident in HBM before timing.
 
-A step answers one batch of B requests (PirServer::ProcessResponse for each,
-batched); `value` is requests/s of the batched path (B = --batch); the
-single-request path (B = 1) is timed beside it with its p50 / p99 latency.
+A step answers one batch of B requests (PirServer::ProcessResponse for each)
+as one hipGraph launch with the requests already in HBM. The scan
+configuration of every batch width is autotuned first. `value` is the best
+batched requests/s over the widths; the single-request path (B = 1) is timed
+beside it. Eager (non-graph) runs of the same batches time the DB-scan kernel
+with HIP events for the rooflines.
 
-The CPU baseline is the oracle's ProcessResponse restatement (single thread,
-matrix loops auto-vectorised) on the same material, timed on a bounded sample.
+Also reported, each on its own synthetic material:
+  * serving: n = 262144, d = 2048, t = 4, ell = 2; Poisson arrivals at 100 to
+    5000 requests/s through the continuous-batching loop; p50 / p99 latency
+    and sustained requests/s beside the one-at-a-time GPU path;
+  * packing: 512 MatrixPack jobs at d = 2048, q = 2^56 with LWE counts uniform
+    in 16..2048, one graph launch.
+
+The CPU baseline is the oracle's ProcessResponse restatement (one thread,
+matrix loops vectorised) on the same material and request.
 """
 from __future__ import annotations
 
@@ -21,6 +31,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -33,6 +44,10 @@ import inspire_amd as ia  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 INT8_PEAK_TOPS = 5000.0        # MI355X dense INT8 MFMA
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def synthetic_material(P: ia.PirParams, seed: int):
@@ -75,10 +90,23 @@ def synthetic_requests(P: ia.PirParams, n: int, seed: int):
     return out
 
 
-def time_batches(srv: ia.PirServer, B: int, steps: int, warmup: int):
+def server_from(P, mat):
+    return ia.PirServer.create_flat(P, mat["db"], mat["db_cols"], mat["pack"], mat["a_tilde"],
+                                    mat["t_vec_h"], mat["second_dim_a"])
+
+
+def time_width(srv: ia.PirServer, reqs, B: int, steps: int, warmup: int):
+    """Graph-replay throughput of batches of B staged requests, and the DB /
+    pack scan-kernel times of eager runs of the same batches."""
+    srv.batch_stage(reqs[:B])
+    srv.use_graphs(False)
+    srv.scan_times_ms(reset=True)
+    for _ in range(2):
+        srv.batch_run(B)
+    db_ms, pk_ms, n = srv.scan_times_ms(reset=True)
+    srv.use_graphs(True)
     for _ in range(warmup):
         srv.batch_run(B)
-    srv.scan_times_ms(reset=True)
     lat = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -86,11 +114,13 @@ def time_batches(srv: ia.PirServer, B: int, steps: int, warmup: int):
         srv.batch_run(B)
         lat.append((time.perf_counter() - s0) * 1e3)
     total = time.perf_counter() - t0
-    db_ms, pk_ms, n = srv.scan_times_ms(reset=True)
-    return total, np.array(lat), db_ms / max(n, 1), pk_ms / max(n, 1)
+    srv.use_graphs(False)
+    return {"batch": B, "qps": B * steps / total, "ms_per_batch": float(np.mean(lat)),
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "db_scan_ms": db_ms / max(n, 1), "pack_scan_ms": pk_ms / max(n, 1)}
 
 
-def cpu_baseline(P, mat, req, max_queries=2, budget_s=30.0):
+def cpu_baseline(P, mat, req, max_queries=3, budget_s=30.0):
     """The oracle's ProcessResponse (one thread) on full requests of the same
     workload; stops after `max_queries` or the time budget."""
     import oracle as O
@@ -98,7 +128,7 @@ def cpu_baseline(P, mat, req, max_queries=2, budget_s=30.0):
     omat = O.Material(mat["db"], mat["pack"], mat["a_tilde"], mat["t_vec_h"],
                       mat["second_dim_a"], mat["db_cols"])
     oreq = O.Request(req.first_dimension_query, req.second_dimension_query, req.packing_key)
-    times, out = [], None
+    times, out, stages = [], None, None
     start = time.perf_counter()
     for _ in range(max_queries):
         t0 = time.perf_counter()
@@ -109,20 +139,156 @@ def cpu_baseline(P, mat, req, max_queries=2, budget_s=30.0):
     return len(times) / sum(times), len(times), out, stages
 
 
+def run_serving(rates, seconds: float, seed: int = 5):
+    """Poisson arrivals through the serving loop at each rate (requests/s)."""
+    P = ia.metrics_tool_params(n=262144, log_d=11, t=4, ell=2)
+    log("serving material n=262144 d=2048 t=4 ell=2")
+    mat = synthetic_material(P, seed=seed)
+    pool = synthetic_requests(P, 32, seed=seed + 1)
+    out = {"workload": "serving n=262144 d=2048 t=4 ell=2, Poisson arrivals", "rates": []}
+    with server_from(P, mat) as srv:
+        srv.process_response(pool[0])
+        lat = []
+        for i in range(20):
+            s0 = time.perf_counter()
+            srv.process_response(pool[i % len(pool)])
+            lat.append((time.perf_counter() - s0) * 1e3)
+        out["one_at_a_time_ms"] = float(np.median(lat))
+        out["one_at_a_time_qps"] = 1e3 / float(np.median(lat))
+        srv.serve_start(max_batch=64, slots=3, max_outstanding=1 << 16)
+        try:
+            # Warm the graphs of the common batch sizes.
+            for i in range(64):
+                srv.submit(pool[i % len(pool)], tag=i)
+            for _ in range(64):
+                assert srv.poll(60000) is not None
+            rng = np.random.default_rng(seed + 2)
+            for lam in rates:
+                n = int(max(100, lam * seconds))
+                arrivals = np.cumsum(rng.exponential(1.0 / lam, size=n))
+                srv.serve_stats(reset=True)
+                done = []
+
+                def poller():
+                    while len(done) < n:
+                        r = srv.poll(30000)
+                        if r is None:
+                            break
+                        done.append(r)
+
+                th = threading.Thread(target=poller)
+                th.start()
+                t0 = time.perf_counter()
+                for i in range(n):
+                    delay = t0 + arrivals[i] - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    srv.submit(pool[i % len(pool)], tag=i)
+                th.join()
+                wall = time.perf_counter() - t0
+                st = srv.serve_stats()
+                bad = sum(1 for r in done if r[2] != 0)
+                out["rates"].append({"offered_qps": lam, "requests": n,
+                                     "answered": len(done), "failed": bad,
+                                     "sustained_qps": len(done) / wall,
+                                     "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                                     "mean_batch": st["mean_batch"]})
+                log(f"serving {lam} q/s: p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms "
+                    f"mean batch {st['mean_batch']:.1f}")
+            st = srv.serve_stats()
+            out["slots"], out["max_batch"] = st["slots"], st["max_batch"]
+        finally:
+            srv.serve_stop()
+    return out
+
+
+def run_packing(njobs: int, seed: int = 11, verify: int = 2):
+    """MatrixPack for njobs jobs at d = 2048, q = 2^56, one graph launch."""
+    log_d, k = 11, 2
+    d = 1 << log_d
+    P = ia.PirParams(log_degree=log_d, log_q=56, log_q1=56, log_q2=56, variance=11,
+                     plaintext_modulus=1 << 16, pack_log_digit=19, pack_num_digits=k,
+                     eval_log_digit=19, eval_num_digits=3, num_entries=d, interp_degree=1,
+                     entry_size_multiple=1, db_bytes=2, mat_bytes=4)
+    rng = np.random.default_rng(seed)
+    words = d * k * d // 2                  # one job's condensed int32 matrix
+    stride = 1 << 18                        # jobs are distinct windows of one pool
+    log(f"packing material: {njobs} jobs x {words * 8 >> 20} MiB")
+    pool = np.frombuffer(rng.bytes((words + stride * njobs) * 8), dtype=np.uint64)
+    packs = [pool[j * stride:j * stride + words].reshape(d, k * d // 2) for j in range(njobs)]
+    qmask = np.uint64((1 << 56) - 1)
+    a_tilde = [rng.integers(0, 1 << 63, size=d, dtype=np.uint64) & qmask for _ in range(njobs)]
+    h = 1 << 18
+    t_vec_h = [rng.integers(-h, h, size=(k, d)).astype(np.uint64) for _ in range(njobs)]
+    counts = rng.integers(16, d + 1, size=njobs)
+    bs = [rng.integers(0, 1 << 63, size=int(m), dtype=np.uint64) & qmask for m in counts]
+    keys = [rng.integers(0, 1 << 63, size=(2 * k, d), dtype=np.uint64) & qmask
+            for _ in range(njobs)]
+    with ia.PirPacker.create(P, packs, a_tilde, t_vec_h) as pk:
+        pk.use_graph(False)
+        pk.times_ms(reset=True)
+        for _ in range(2):
+            out = pk.run(bs, keys)
+        tot_e, scan_ms, n = pk.times_ms(reset=True)
+        pk.use_graph(True)
+        pk.run(bs, keys)
+        pk.times_ms(reset=True)
+        steps = 5
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            got = pk.run(bs, keys)
+        wall = (time.perf_counter() - t0) / steps
+        tot_g, _, ng = pk.times_ms(reset=True)
+    exact = bool(np.array_equal(got, out))
+    if verify:
+        import oracle as O
+        OP = O.Params(**{f: getattr(P, f) for f in O.Params.__dataclass_fields__})
+        for j in range(verify):
+            bb = np.zeros(d, dtype=np.uint64)
+            bb[:counts[j]] = bs[j]
+            a, b = O.matrix_pack(OP, bb, keys[j][:k], keys[j][k:], packs[j], t_vec_h[j],
+                                 a_tilde[j])
+            exact = exact and bool(np.array_equal(got[j], np.stack([a, b])))
+    pack_bytes = njobs * d * k * d * 4
+    scan = scan_ms / max(n, 1)
+    return {"workload": f"{njobs} MatrixPack jobs, d=2048, q=2^56, LWE counts 16..2048",
+            "jobs": njobs, "graph_ms_per_batch": tot_g / max(ng, 1),
+            "jobs_per_s": njobs / (tot_g / max(ng, 1) * 1e-3),
+            "wall_ms_per_batch_incl_pcie": wall * 1e3,
+            "pack_scan_ms": scan, "pack_scan_gbs": pack_bytes / (scan * 1e-3) / 1e9,
+            "pack_scan_hbm_frac": pack_bytes / (scan * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "bit_identical": exact}
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernels, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--widths", default="1,16,64,256")
     ap.add_argument("--log-degree", type=int, default=12)
     ap.add_argument("--num-entries", type=int, default=1 << 20)
     ap.add_argument("--interp-degree", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-queries", type=int, default=1)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-pack", action="store_true")
+    ap.add_argument("--cpu-queries", type=int, default=3)
+    ap.add_argument("--serve-rates", default="100,500,1000,2000,5000")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--pack-jobs", type=int, default=512)
     ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: 2 single + 2 batched answers and a "
-                         "2 GiB copy-kernel calibration, no CPU baseline")
+                    help="short run for rocprofv3: two single and two 64-wide eager answers, "
+                         "a copy and a read calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the answer path runs on a single MI355X in this "
@@ -135,24 +301,21 @@ def main():
     r = P.num_entries // t
     M = t * d
     K = (r + 31) // 32 * 32
-    print(f"[bench] generating material n={P.num_entries} d={d} t={t} "
-          f"(DB {M * K * 2 / 2**30:.2f} GiB, pack {M * 2 * d * 4 / 2**30:.2f} GiB)",
-          file=sys.stderr, flush=True)
+    widths = [int(x) for x in args.widths.split(",")]
+    log(f"generating material n={P.num_entries} d={d} t={t} "
+        f"(DB {M * K * 2 / 2**30:.2f} GiB, pack {M * 2 * d * 4 / 2**30:.2f} GiB)")
     mat = synthetic_material(P, seed=2026)
     t0 = time.perf_counter()
-    srv = ia.PirServer.create_flat(P, mat["db"], mat["db_cols"], mat["pack"], mat["a_tilde"],
-                                   mat["t_vec_h"], mat["second_dim_a"])
-    print(f"[bench] server resident in HBM ({time.perf_counter() - t0:.1f}s)",
-          file=sys.stderr, flush=True)
-    B = args.batch
-    reqs = synthetic_requests(P, B, seed=7)
+    srv = server_from(P, mat)
+    log(f"server resident in HBM ({time.perf_counter() - t0:.1f}s)")
+    reqs = synthetic_requests(P, max(widths + [64]), seed=7)
     if args.profile:
         srv.batch_stage(reqs[:1])
         for _ in range(2):
             srv.batch_run(1)
-        srv.batch_stage(reqs)
+        srv.batch_stage(reqs[:64])
         for _ in range(2):
-            srv.batch_run(B)
+            srv.batch_run(64)
         copy_ms = ia.copy_probe(2 << 30, repeats=3)
         read_ms = ia.read_probe(4 << 30, repeats=3)
         print(json.dumps({"profile": True, "copy_bytes": 2 << 30, "copy_ms": copy_ms,
@@ -161,35 +324,45 @@ def main():
         srv.close()
         return
 
-    # Single-request path.
-    srv.batch_stage(reqs[:1])
-    s_total, s_lat, s_db_ms, s_pk_ms = time_batches(srv, 1, max(args.steps, 5), args.warmup)
-    single_qps = max(args.steps, 5) / s_total
-    print(f"[bench] single: {np.mean(s_lat):.3f} ms/query, DB scan {s_db_ms:.3f} ms",
-          file=sys.stderr, flush=True)
-    # Batched path.
-    srv.batch_stage(reqs)
-    b_total, b_lat, b_db_ms, b_pk_ms = time_batches(srv, B, args.steps, args.warmup)
-    batch_qps = B * args.steps / b_total
-    print(f"[bench] batch {B}: {np.mean(b_lat):.3f} ms/batch, {batch_qps:.1f} q/s, "
-          f"DB scan {b_db_ms:.3f} ms", file=sys.stderr, flush=True)
+    tune_report = srv.autotune(widths)
+    log("autotune:\n" + tune_report.strip())
+    per_width = {}
+    for B in widths:
+        res = time_width(srv, reqs, B, max(args.steps, 5) if B == 1 else args.steps,
+                         args.warmup)
+        per_width[B] = res
+        log(f"width {B}: {res['qps']:.1f} q/s, {res['ms_per_batch']:.3f} ms/batch, "
+            f"DB scan {res['db_scan_ms']:.3f} ms")
+    single = per_width[1] if 1 in per_width else time_width(srv, reqs, 1, 5, args.warmup)
+    best_B = max((b for b in per_width if b > 1), key=lambda b: per_width[b]["qps"])
+    best = per_width[best_B]
+    read_ms = ia.read_probe(4 << 30, repeats=4)
+    mfma_ms, mfma_tops = ia.mfma_probe(2048, 4096)
 
     db_bytes = M * K * 2
-    nl = 7
     pairs = 13  # (plane, digit) pairs with 8 (p + c) < 52
-    mfma_ops_batch = 2.0 * M * K * B * pairs
-    roofline_single = {
-        "bound": "hbm", "achieved": db_bytes / (s_db_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": db_bytes / (s_db_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "traffic": None}
-    batch_tops = mfma_ops_batch / (b_db_ms * 1e-3) / 1e12
-    batch_gbs = db_bytes / (b_db_ms * 1e-3) / 1e9
-    if batch_tops / INT8_PEAK_TOPS >= batch_gbs / HBM_PEAK_GBS:
-        roofline = {"bound": "mfma", "achieved": batch_tops, "peak": INT8_PEAK_TOPS,
-                    "unit": "TOP/s", "frac": batch_tops / INT8_PEAK_TOPS, "traffic": None}
-    else:
-        roofline = {"bound": "hbm", "achieved": batch_gbs, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": batch_gbs / HBM_PEAK_GBS, "traffic": None}
+    traffic = load_traffic()
+
+    def mfma_line(B, scan_ms):
+        tops = 2.0 * M * K * B * pairs / (scan_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                "frac": tops / INT8_PEAK_TOPS,
+                "traffic": traffic.get(f"db_scan_batch{B}")}
+
+    roofline = mfma_line(best_B, best["db_scan_ms"])
+    gbs = db_bytes / (single["db_scan_ms"] * 1e-3) / 1e9
+    roofline_single = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": gbs / HBM_PEAK_GBS, "traffic": traffic.get("db_scan_single"),
+                       "achievable_read_gbs": (4 << 30) / (read_ms * 1e-3) / 1e9}
+    widths_out = []
+    for B, res in sorted(per_width.items()):
+        line = dict(res)
+        if B > 1:
+            line["db_scan_mfma_frac"] = mfma_line(B, res["db_scan_ms"])["frac"]
+            line["db_scan_hbm_frac"] = db_bytes / (res["db_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        else:
+            line["db_scan_hbm_frac"] = roofline_single["frac"]
+        widths_out.append(line)
 
     cpu = None
     if not args.no_cpu:
@@ -197,40 +370,53 @@ def main():
         gpu_out = srv.process_response(reqs[0]).ciphertexts
         cpu = {"value": qps, "unit": "queries/s", "cores": 1, "kind": "port",
                "sample": f"{nq} full request(s) of the same n=2^20 d=4096 t=8 workload "
-                         f"through the oracle ProcessResponse (stage ms: db "
-                         f"{stages[0]:.0f}, pack {stages[1]:.0f}, eval {stages[2]:.0f})",
+                         f"through the oracle ProcessResponse, one thread (last run's stage "
+                         f"ms: db {stages[0]:.0f}, pack {stages[1]:.0f}, eval {stages[2]:.0f})",
                "bit_identical": bool(np.array_equal(cpu_out, gpu_out))}
+    srv.close()
+    del mat
+
+    serving = None
+    if not args.no_serve:
+        rates = [float(x) for x in args.serve_rates.split(",")]
+        serving = run_serving(rates, args.serve_seconds)
+    packing = None
+    if not args.no_pack:
+        packing = run_packing(args.pack_jobs)
+        log(f"packing: {packing['graph_ms_per_batch']:.3f} ms per {args.pack_jobs} jobs, "
+            f"scan {packing['pack_scan_gbs']:.0f} GB/s, exact {packing['bit_identical']}")
 
     line = {
         "metric": "queries/sec on 1 GPU at n=2^20,d=4096 (batched vs single); p50/p99 answer ms",
-        "value": batch_qps,
+        "value": best["qps"],
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": float(np.mean(b_lat)),
+        "ms_per_step": best["ms_per_batch"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic (seeded, range-faithful material and requests)",
         "config": {"workload": "resident-DB PIR answer, n=2^20 d=4096 t=8 ell=1",
-                   "n": P.num_entries, "d": d, "t": t, "ell": 1, "batch": B,
+                   "n": P.num_entries, "d": d, "t": t, "ell": 1, "batch": best_B,
                    "log_q": P.log_q, "p": P.plaintext_modulus},
-        "single_qps": single_qps,
-        "batched_over_single": batch_qps / single_qps,
-        "p50_ms_single": float(np.percentile(s_lat, 50)),
-        "p99_ms_single": float(np.percentile(s_lat, 99)),
-        "p50_ms_batch": float(np.percentile(b_lat, 50)),
-        "p99_ms_batch": float(np.percentile(b_lat, 99)),
+        "single_qps": single["qps"],
+        "batched_over_single": best["qps"] / single["qps"],
+        "p50_ms_single": single["p50_ms"],
+        "p99_ms_single": single["p99_ms"],
+        "p50_ms_batch": best["p50_ms"],
+        "p99_ms_batch": best["p99_ms"],
         "roofline": roofline,
         "roofline_single_scan": roofline_single,
-        "db_scan_ms_single": s_db_ms,
-        "db_scan_ms_batch": b_db_ms,
-        "pack_scan_ms_single": s_pk_ms,
+        "mfma_probe_tops": mfma_tops,
+        "widths": widths_out,
+        "autotune": tune_report.strip().split("\n"),
+        "serving": serving,
+        "packing": packing,
         "cpu_baseline": cpu,
     }
-    srv.close()
     print(json.dumps(line))
 
 

@@ -147,6 +147,16 @@ int pirg_batch_run(pirg_server* server, int nreq, char* err, size_t errlen);
 int pirg_batch_fetch(pirg_server* server, int nreq, uint64_t* const* outs,
                      char* err, size_t errlen);
 
+/* Autotuner: for each batch size in widths, times every candidate
+ * configuration of the DB and pack scans (narrow scan with 1/2/4 tiles per
+ * group, wide scan with 2/4/7 tiles per wave, two K-splits) on the resident
+ * matrices and keeps the fastest for that batch size; other sizes keep the
+ * built-in choice. `report` (may be NULL) receives one line per scan and
+ * width with every candidate's time. Not while serving. No reference
+ * counterpart. */
+int pirg_server_autotune(pirg_server* server, const int* widths, int nwidths,
+                         char* report, size_t report_len);
+
 /* Makes pirg_process_batch / pirg_batch_run replay one hipGraph per batch
  * size (captured on first use: the whole answer pipeline as one graph launch)
  * instead of launching the kernels one by one. Off by default; graph replays

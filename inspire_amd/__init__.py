@@ -391,6 +391,15 @@ class PirServer:
         """Largest |x - round(x)| the FFT products have rounded so far."""
         return lib().pirg_server_fft_margin(self._h, int(reset))
 
+    def autotune(self, widths: Sequence[int]) -> str:
+        """Times the scan configurations for each batch width and keeps the
+        fastest; returns the per-candidate report."""
+        arr = (ctypes.c_int * max(1, len(widths)))(*widths)
+        rep = ctypes.create_string_buffer(1 << 16)
+        rc = lib().pirg_server_autotune(self._h, arr, len(widths), rep, ctypes.c_size_t(len(rep)))
+        _check(rc, rep)
+        return rep.value.decode()
+
     def use_graphs(self, enable: bool = True) -> None:
         """process_batch / batch_run replay one captured hipGraph per batch size."""
         _check(lib().pirg_server_use_graphs(self._h, int(enable)), _err())

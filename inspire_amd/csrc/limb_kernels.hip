@@ -13,6 +13,9 @@
 // exact int8 x int8 -> int32 product on v_mfma_i32_16x16x64_i8; the pairs
 // are recombined with 64-bit shifts. A k-range of at most 65536 columns keeps
 // every int32 sum exact (|sum| <= 65536 * 2^14 = 2^30).
+#include <algorithm>
+#include <vector>
+
 #include "pirg_internal.h"
 
 namespace pirg {
@@ -690,6 +693,53 @@ int QueriesPerGroup(int tiles, int ns) {
 constexpr int kNarrowTiles[] = {1, 2, 4};
 constexpr int kWideTiles[] = {2, 4, 7};
 
+// Finishes a plan whose kind and tile width are set: groups, K-split and
+// B-tile size. ks_scale multiplies the workgroup target of the K-split.
+void CompletePlan(const LimbLayout& L, int ks_scale, ScanPlan* P) {
+  P->qpg = QueriesPerGroup(P->ctg, P->ns);
+  const int groups = (P->nq + P->qpg - 1) / P->qpg;
+  P->ngp = P->wide ? (groups + 3) / 4 * 4 : groups;
+  // K-split: exact int32 sums need at most 65536 / planes columns per split;
+  // beyond that, enough workgroups to occupy every CU a few times over.
+  const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
+  const int row_wgs = P->wide ? (L.strips + 7) / 8 : (L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR);
+  const int z = P->wide ? P->ngp / 4 : P->ngp;
+  const int target = (P->wide ? 512 : 2048) * std::max(ks_scale, 1);
+  int ks = (target + row_wgs * z - 1) / (row_wgs * z);
+  const int min_split = (L.kblocks + max_kb - 1) / max_kb;
+  if (ks < min_split) ks = min_split;
+  if (ks > L.kblocks) ks = L.kblocks;
+  if (ks < 1) ks = 1;
+  P->kb_per_split = (L.kblocks + ks - 1) / ks;
+  P->ksplit = (L.kblocks + P->kb_per_split - 1) / P->kb_per_split;
+  P->btile_bytes = static_cast<size_t>(L.kblocks + 1) * P->ngp * P->ctg * kTileBytes;
+}
+
+std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits) {
+  std::vector<ScanPlan> out;
+  const int ns = (out_bits + 7) / 8;
+  for (int wide = 0; wide < 2; ++wide) {
+    for (int t : (wide ? std::vector<int>{2, 4, 7} : std::vector<int>{1, 2, 4})) {
+      // A narrow workgroup re-reads A for every group: keep those with few groups.
+      if (!wide && (nq + QueriesPerGroup(t, ns) - 1) / QueriesPerGroup(t, ns) > 2) continue;
+      for (int ks_scale : {1, 2}) {
+        ScanPlan P{};
+        P.wide = wide;
+        P.nq = nq;
+        P.ns = ns;
+        P.ctg = t;
+        CompletePlan(L, ks_scale, &P);
+        bool dup = false;
+        for (const ScanPlan& o : out) {
+          dup = dup || (o.wide == P.wide && o.ctg == P.ctg && o.ksplit == P.ksplit);
+        }
+        if (!dup) out.push_back(P);
+      }
+    }
+  }
+  return out;
+}
+
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries) {
   ScanPlan P{};
   P.nq = nq;
@@ -717,23 +767,7 @@ ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_qu
       }
     }
   }
-  P.qpg = QueriesPerGroup(P.ctg, ns);
-  const int groups = (nq + P.qpg - 1) / P.qpg;
-  P.ngp = P.wide ? (groups + 3) / 4 * 4 : groups;
-  // K-split: exact int32 sums need at most 65536 / planes columns per split;
-  // beyond that, enough workgroups to occupy every CU a few times over.
-  const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
-  const int row_wgs = P.wide ? (L.strips + 7) / 8 : (L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR);
-  const int z = P.wide ? P.ngp / 4 : P.ngp;
-  const int target = P.wide ? 512 : 2048;
-  int ks = (target + row_wgs * z - 1) / (row_wgs * z);
-  const int min_split = (L.kblocks + max_kb - 1) / max_kb;
-  if (ks < min_split) ks = min_split;
-  if (ks > L.kblocks) ks = L.kblocks;
-  if (ks < 1) ks = 1;
-  P.kb_per_split = (L.kblocks + ks - 1) / ks;
-  P.ksplit = (L.kblocks + P.kb_per_split - 1) / P.kb_per_split;
-  P.btile_bytes = static_cast<size_t>(L.kblocks + 1) * P.ngp * P.ctg * kTileBytes;
+  CompletePlan(L, 1, &P);
   return P;
 }
 

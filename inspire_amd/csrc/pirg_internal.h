@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 namespace pirg {
 
@@ -118,6 +119,9 @@ struct ScanPlan {
 // wide_min_queries: batches of at least this many queries use WideScan even
 // when they would fit NarrowScan.
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries);
+// The configurations an autotuner may time for nq queries (kind, tile width,
+// K-split).
+std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits);
 // Grouped plan: ngroups blocks of group_rows rows (a multiple of 64), block b
 // times vector b.
 ScanPlan MakeGroupedScanPlan(const LimbLayout& L, int ngroups, int group_rows, int out_bits);

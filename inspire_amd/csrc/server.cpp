@@ -365,6 +365,13 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
 
 size_t ResponseWords(const pirg_server* S) { return static_cast<size_t>(S->ell) * 2 * S->d; }
 
+ScanPlan PlanFor(const pirg_server* S, bool db, int nq) {
+  const auto& tuned = db ? S->tuned_db : S->tuned_pk;
+  auto it = tuned.find(nq);
+  if (it != tuned.end()) return it->second;
+  return MakeScanPlan(db ? S->dbL : S->pkL, nq, S->log_q, S->wide_min);
+}
+
 namespace {
 
 // Device buffer sizes of a workspace for `cap` requests, in the order of
@@ -376,8 +383,8 @@ std::vector<size_t> WorkspaceSizes(const pirg_server* S, int cap) {
   // Scan buffers: the largest plan over every batch size up to cap.
   size_t bt_db = 0, bt_pk = 0, part_db = 0, part_pk = 0;
   for (int n = 1; n <= cap; ++n) {
-    const ScanPlan pd = MakeScanPlan(S->dbL, n, S->log_q, S->wide_min);
-    const ScanPlan pp = MakeScanPlan(S->pkL, n, S->log_q, S->wide_min);
+    const ScanPlan pd = PlanFor(S, true, n);
+    const ScanPlan pp = PlanFor(S, false, n);
     bt_db = std::max(bt_db, pd.btile_bytes);
     bt_pk = std::max(bt_pk, pp.btile_bytes);
     part_db = std::max(part_db, static_cast<size_t>(pd.ksplit) * n);
@@ -542,8 +549,8 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   };
   PIRG_HIP(mark(slot.ev[0]));
   // DB product (pir_server.cc:215-229).
-  const ScanPlan pd = MakeScanPlan(S->dbL, nq, S->log_q, S->wide_min);
-  const ScanPlan pp = MakeScanPlan(S->pkL, nq, S->log_q, S->wide_min);
+  const ScanPlan pd = PlanFor(S, true, nq);
+  const ScanPlan pp = PlanFor(S, false, nq);
   PIRG_HIP(LaunchQueryDigits(W.fd.as<u64>(), S->K, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
   PIRG_HIP(LaunchVectorSums(W.fd.as<u64>(), nq, S->K, S->K, S->dbL.c_off,
                             W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
@@ -835,7 +842,95 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
 }  // namespace
 }  // namespace pirg
 
+namespace pirg {
+namespace {
+
+std::string DescribePlan(const ScanPlan& P) {
+  char buf[128];
+  std::snprintf(buf, sizeof(buf), "%s tiles=%d ksplit=%d", P.wide ? "wide" : "narrow", P.ctg,
+                P.ksplit);
+  return buf;
+}
+
+// Times every candidate scan configuration for nq queries on the resident
+// matrix and keeps the fastest.
+Error TuneOne(pirg_server* S, bool db, int nq, std::string* report) {
+  const LimbLayout& L = db ? S->dbL : S->pkL;
+  const DevBuf& tiles = db ? S->db_tiles : S->pk_tiles;
+  const std::vector<ScanPlan> cands = ScanPlanCandidates(L, nq, S->log_q);
+  size_t bt_bytes = 0, part_words = 0;
+  for (const ScanPlan& c : cands) {
+    bt_bytes = std::max(bt_bytes, c.btile_bytes);
+    part_words = std::max(part_words, static_cast<size_t>(c.ksplit) * nq * L.rows);
+  }
+  DevBuf bt, part;
+  Error er = bt.alloc(bt_bytes);
+  if (!er.ok()) return er;
+  er = part.alloc(part_words * 8);
+  if (!er.ok()) return er;
+  // Operands with realistic bit patterns: the MFMA clock depends on the data
+  // (all-zero operands run faster than query digits), so the B tiles are
+  // filled with bytes of the resident matrix.
+  hipStream_t st = S->main.stream;
+  const size_t fill = std::min(bt_bytes, tiles.bytes);
+  PIRG_HIP(hipMemcpyAsync(bt.p, tiles.p, fill, hipMemcpyDeviceToDevice, st));
+  if (fill < bt_bytes) {
+    PIRG_HIP(hipMemsetAsync(static_cast<char*>(bt.p) + fill, 0x5b, bt_bytes - fill, st));
+  }
+  double best = 1e30;
+  ScanPlan best_plan = cands.front();
+  char line[160];
+  std::snprintf(line, sizeof(line), "%s nq=%d:", db ? "db" : "pack", nq);
+  *report += line;
+  for (const ScanPlan& c : cands) {
+    PIRG_HIP(LaunchScan(tiles.as<int8_t>(), L, c, bt.as<int8_t>(), part.as<u64>(), st));
+    PIRG_HIP(hipEventRecord(S->main.kev[0], st));
+    const int reps = 3;
+    for (int r = 0; r < reps; ++r) {
+      PIRG_HIP(LaunchScan(tiles.as<int8_t>(), L, c, bt.as<int8_t>(), part.as<u64>(), st));
+    }
+    PIRG_HIP(hipEventRecord(S->main.kev[1], st));
+    PIRG_HIP(hipEventSynchronize(S->main.kev[1]));
+    float ms = 0;
+    PIRG_HIP(hipEventElapsedTime(&ms, S->main.kev[0], S->main.kev[1]));
+    ms /= reps;
+    std::snprintf(line, sizeof(line), " [%s %.3f ms]", DescribePlan(c).c_str(), ms);
+    *report += line;
+    if (ms < best) {
+      best = ms;
+      best_plan = c;
+    }
+  }
+  std::snprintf(line, sizeof(line), " -> %s\n", DescribePlan(best_plan).c_str());
+  *report += line;
+  (db ? S->tuned_db : S->tuned_pk)[nq] = best_plan;
+  return Ok();
+}
+
+}  // namespace
+}  // namespace pirg
+
 extern "C" {
+
+int pirg_server_autotune(pirg_server* S, const int* widths, int nwidths, char* report,
+                         size_t report_len) {
+  if (S == nullptr || widths == nullptr || nwidths < 0) return PIRG_INVALID_ARGUMENT;
+  if (S->serve) return PIRG_FAILED_PRECONDITION;
+  std::string rep;
+  for (int i = 0; i < nwidths; ++i) {
+    if (widths[i] < 1) return PIRG_INVALID_ARGUMENT;
+    for (bool db : {true, false}) {
+      Error er = TuneOne(S, db, widths[i], &rep);
+      if (!er.ok()) return Report(er, report, report_len);
+    }
+  }
+  // Plans changed: graphs and workspace sizes follow on the next batch.
+  (void)hipStreamSynchronize(S->main.stream);
+  S->main.DropGraphs();
+  S->main.ws.cap = 0;
+  if (report != nullptr && report_len > 0) std::snprintf(report, report_len, "%s", rep.c_str());
+  return 0;
+}
 
 int pirg_server_create(const pirg_params* params,
                        const pirg_preprocessed_view* data, pirg_server** out,

@@ -122,6 +122,8 @@ struct pirg_server {
   pirg::LimbLayout dbL{}, pkL{};
   // Batches of at least this many requests use the MFMA-bound WideScan.
   int wide_min = 10;
+  // Scan configurations chosen by pirg_server_autotune, per batch size.
+  std::map<int, pirg::ScanPlan> tuned_db, tuned_pk;
   pirg::DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
   pirg::Slot main;
   int staged = 0;
@@ -162,6 +164,9 @@ Error EnqueueFetch(pirg_server* S, Slot& slot, int nq);
 // Adds the slot's finished batch to the server's stage times.
 void CollectTiming(pirg_server* S, Slot& slot);
 size_t ResponseWords(const pirg_server* S);
+// The scan configuration of the DB (db = true) or pack product for nq
+// queries: the autotuned one when there is one, else the built-in choice.
+ScanPlan PlanFor(const pirg_server* S, bool db, int nq);
 
 Error CheckDevice();
 // crypto/lwes_to_rlwe.cc:60-74 and crypto/external_product.cc:100-102.

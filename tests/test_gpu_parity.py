@@ -214,3 +214,20 @@ def test_full_size_response_matches_oracle():
                                   mat["t_vec_h"], mat["second_dim_a"]) as srv:
         got = srv.process_response(req).ciphertexts
     np.testing.assert_array_equal(got, want)
+
+
+def test_autotuned_plans_match_oracle():
+    # After the autotuner picks scan configurations per width, batches of
+    # those widths (and of others) still answer bit for bit.
+    P = O.metrics_tool_params(n=8192, log_d=9, t=4)
+    mat = O.synthetic_material(P, seed=404)
+    reqs = [O.synthetic_request(P, seed=4040 + i) for i in range(20)]
+    with server_from_material(P, mat) as srv:
+        report = srv.autotune([1, 5, 20])
+        assert report.count("->") == 6, report
+        for n in (1, 5, 7, 20):
+            got, st, _ = srv.process_batch([gpu_request(r) for r in reqs[:n]])
+            assert st == [0] * n
+            for i in (0, n - 1):
+                want, _, _ = O.process_response(P, mat, reqs[i])
+                np.testing.assert_array_equal(got[i].ciphertexts, want)
