@@ -287,8 +287,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--pack-jobs", type=int, default=512)
     ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: two single and two 64-wide eager answers, "
-                         "a copy and a read calibration kernel")
+                    help="short run for rocprofv3: two eager answers each at widths 1, 64 and "
+                         "256, a copy and a read calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the answer path runs on a single MI355X in this "
@@ -310,12 +310,12 @@ def main():
     log(f"server resident in HBM ({time.perf_counter() - t0:.1f}s)")
     reqs = synthetic_requests(P, max(widths + [64]), seed=7)
     if args.profile:
-        srv.batch_stage(reqs[:1])
-        for _ in range(2):
-            srv.batch_run(1)
-        srv.batch_stage(reqs[:64])
-        for _ in range(2):
-            srv.batch_run(64)
+        for B in (1, 64, 256):
+            if B > len(reqs):
+                continue
+            srv.batch_stage(reqs[:B])
+            for _ in range(2):
+                srv.batch_run(B)
         copy_ms = ia.copy_probe(2 << 30, repeats=3)
         read_ms = ia.read_probe(4 << 30, repeats=3)
         print(json.dumps({"profile": True, "copy_bytes": 2 << 30, "copy_ms": copy_ms,

@@ -9,7 +9,7 @@
 //
 // Pinning (what fixes each part independently of this restatement):
 //   * matrix products: closed form sum mod 2^64 and the KATs of
-//     crypto/matrix_test.cc:82-183 (tests/test_oracle_matrix.py);
+//     crypto/matrix_test.cc:82-183 (tests/test_oracle_kats.py);
 //   * FFT products: the exact 128-bit schoolbook checker below
 //     (orc_inner_product_exact) on the mt19937_64 streams of
 //     crypto/polynomial_test.cc:442-612 and the KAT at :297-309;
