@@ -150,7 +150,7 @@ Error Create(const pirg_params* params, int njobs, const void* const* pack,
   const u64 c_off = params->mat_bytes == 4 ? 128ULL * (1 + 256 + 65536) : 128ULL;
   K->L = MakeLayout(J * d, cols, planes, c_off);
   K->plan = MakeGroupedScanPlan(K->L, J, d, K->log_q);
-  er = K->tiles.alloc(static_cast<size_t>(K->L.strips) * K->L.kblocks * planes * kTileBytes);
+  er = K->tiles.alloc(TileBufferBytes(K->L));
   if (!er.ok()) return er;
   const size_t rowbytes = params->mat_bytes == 4 ? static_cast<size_t>(cols) * 4
                                                  : static_cast<size_t>(cols) * 2;

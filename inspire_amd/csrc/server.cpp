@@ -268,8 +268,7 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   const int db_planes = S->P.db_bytes == 2 ? 2 : 1;
   const u64 db_off = S->P.db_bytes == 2 ? 32896ULL : 128ULL;
   S->dbL = MakeLayout(S->M, db_cols, db_planes, db_off);
-  Error er = S->db_tiles.alloc(static_cast<size_t>(S->dbL.strips) * S->dbL.kblocks *
-                               db_planes * kTileBytes);
+  Error er = S->db_tiles.alloc(TileBufferBytes(S->dbL));
   if (!er.ok()) return er;
   const int wpr = S->P.db_bytes == 2 ? db_cols / 4 : db_cols / 8;
   const int rows_per_part = S->M / static_cast<int>(db_parts.size());
@@ -289,8 +288,7 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   const int pk_planes = S->P.mat_bytes == 4 ? 4 : 2;
   const u64 pk_off = S->P.mat_bytes == 4 ? 128ULL * (1 + 256 + 65536) : 128ULL;
   S->pkL = MakeLayout(S->M, pk_cols, pk_planes, pk_off);
-  er = S->pk_tiles.alloc(static_cast<size_t>(S->pkL.strips) * S->pkL.kblocks *
-                         pk_planes * kTileBytes);
+  er = S->pk_tiles.alloc(TileBufferBytes(S->pkL));
   if (!er.ok()) return er;
   const size_t pk_rowbytes = S->P.mat_bytes == 4 ? static_cast<size_t>(pk_cols) * 4
                                                  : static_cast<size_t>(pk_cols) * 2;
@@ -807,7 +805,7 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
   DevBuf tiles, src, v, bt, corr, partial;
   const size_t wpr = is_u16 ? cols / 4 : cols / 2;
   if ((is_u16 && cols % 4) || (!is_u16 && cols % 2)) return Invalid("Vector size mismatch.");
-  if ((er = tiles.alloc(static_cast<size_t>(L.strips) * L.kblocks * planes * kTileBytes)).code) return er;
+  if ((er = tiles.alloc(TileBufferBytes(L))).code) return er;
   if ((er = src.alloc(static_cast<size_t>(rows) * wpr * 8)).code) return er;
   if ((er = v.alloc(static_cast<size_t>(cols) * 8)).code) return er;
   const ScanPlan plan = MakeScanPlan(L, 1, 64, 1 << 30);
