@@ -418,21 +418,46 @@ struct WideShape {
           : static_cast<size_t>(kRedWords) * 8;
 };
 
+// Workgroup order. Workgroups are dispatched to the 8 XCDs round robin
+// (workgroup w to XCD w % 8), each XCD with its own L2. The nz column blocks
+// of one (row block, k-split) tile read the same A tiles, so they are given
+// consecutive slots on one XCD: w = (u * nz + j) * 8 + x, tile = u * 8 + x,
+// column block j. They run side by side and share A through that XCD's L2
+// (A is fetched from HBM about once instead of nz times). Tiles past the
+// last (the grid is padded to whole groups of 8) exit at once.
+struct WideTile {
+  int strip_block, ks, zb;
+};
+
+__device__ __forceinline__ bool DecodeWideTile(int w, int nx, int ntiles, int nz, WideTile* t) {
+  const int x = w & 7;
+  const int rest = w >> 3;
+  const int tile = (rest / nz) * 8 + x;
+  if (tile >= ntiles) return false;
+  t->strip_block = tile % nx;
+  t->ks = tile / nx;
+  t->zb = rest % nz;
+  return true;
+}
+
 template <int NP, int CT>
 __global__ __launch_bounds__(512) void WideScanKernel(
-    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps,
+    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
     const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
     u64* __restrict__ partial) {
   using W = WideShape<NP, CT>;
   constexpr int RS = W::RS, RW = W::RW, AF = W::AF, S = W::S;
   extern __shared__ v4i lds[];
+  const int nx = (strips + RS - 1) / RS;
+  WideTile wt;
+  if (!DecodeWideTile(blockIdx.x, nx, nx * ksplit, ngp / 4, &wt)) return;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int h = wave >> 2;       // strip half
   const int g = wave & 3;        // group within the workgroup
-  const int strip0 = blockIdx.x * RS;
-  const int ks = blockIdx.y;
-  const int grp = blockIdx.z * 4 + g;
+  const int strip0 = wt.strip_block * RS;
+  const int ks = wt.ks;
+  const int grp = wt.zb * 4 + g;
   const int kb0 = ks * kbps;
   const int nkb = min(kblocks, kb0 + kbps) - kb0;
   const v4i* A = reinterpret_cast<const v4i*>(tiles);
@@ -545,10 +570,12 @@ hipError_t LaunchWideT(const int8_t* tiles, const LimbLayout& L, const ScanPlan&
                        const int8_t* btiles, u64* partial, hipStream_t s) {
   using W = WideShape<NP, CT>;
   static_assert(W::kLdsBytes <= 160 * 1024, "WideScan LDS");
-  dim3 grid((L.strips + 7) / 8, P.ksplit, P.ngp / 4);
-  hipLaunchKernelGGL((WideScanKernel<NP, CT>), grid, dim3(512), W::kLdsBytes, s, tiles,
-                     L.strips, L.rows, L.kblocks, P.kb_per_split, btiles, P.ngp, P.nq, P.ns,
-                     P.qpg, partial);
+  const long long ntiles = static_cast<long long>((L.strips + W::RS - 1) / W::RS) * P.ksplit;
+  const long long blocks = (ntiles + 7) / 8 * 8 * (P.ngp / 4);
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((WideScanKernel<NP, CT>), dim3(static_cast<unsigned>(blocks)), dim3(512),
+                     W::kLdsBytes, s, tiles, L.strips, L.rows, L.kblocks, P.kb_per_split,
+                     P.ksplit, btiles, P.ngp, P.nq, P.ns, P.qpg, partial);
   return hipGetLastError();
 }
 
