@@ -136,14 +136,15 @@ __device__ inline int BalancedDigit(u64 y, int c) {
 }
 
 // B operand tiles (plane-0 digits only). Layout [kb][group][tile][64 lanes]
-// [16 bytes]. Group g holds queries [g*qpg, (g+1)*qpg); its column j is
-// (query g*qpg + j / ns, shift s = j % ns) for j < qpg*ns and zero above; the
-// column holds balanced digit s of the query. Lane l of a tile holds column
-// l % 16 for the 16 k's 16*(l/16) .. +15 of the k-block. One thread per
-// (kb, group, tile, lane).
+// [16 bytes]. Group g holds queries [g*qpg, (g+1)*qpg). Query-major: its
+// column j is (query g*qpg + j / ns, shift s = j % ns) for j < qpg*ns and zero
+// above. Digit-major (ctg == ns, qpg == 16): column c of tile s is (query
+// g*16 + c, shift s). The column holds balanced digit s of the query. Lane l
+// of a tile holds column l % 16 for the 16 k's 16*(l/16) .. +15 of the
+// k-block. One thread per (kb, group, tile, lane).
 __global__ void QueryDigitsKernel(const u64* __restrict__ vec, int nq, int stride,
                                   int len, int kblocks, int ns, int qpg, int ngp,
-                                  int ctg, int8_t* __restrict__ btiles) {
+                                  int ctg, int digit_major, int8_t* __restrict__ btiles) {
   const long long total = (long long)kblocks * ngp * ctg * 64;
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
        g += (long long)gridDim.x * blockDim.x) {
@@ -154,9 +155,9 @@ __global__ void QueryDigitsKernel(const u64* __restrict__ vec, int nq, int strid
     const int grp = static_cast<int>(t % ngp);
     const int kb = static_cast<int>(t / ngp);
     const int j = ct * 16 + (lane & 15);
-    const int q = grp * qpg + j / ns;
-    const int s = j % ns;
-    const bool valid = j < qpg * ns && q < nq;
+    const int q = digit_major ? grp * qpg + (lane & 15) : grp * qpg + j / ns;
+    const int s = digit_major ? ct : j % ns;
+    const bool valid = (digit_major || j < qpg * ns) && q < nq;
     const int kbase = kb * 64 + 16 * (lane >> 4);
     int8_t out[16];
 #pragma unroll
@@ -272,6 +273,40 @@ __device__ __forceinline__ void WaveEpilogue(const v4i (&acc)[NS][NT], u64* red,
     const int q = i / kRows;
     const int rl = i % kRows;
     const int qg = grp * qpg + q;
+    const long long row = row0 + rl;
+    if (qg < nq && row < rows) {
+      partial[((long long)ks * nq + qg) * rows + row] = red[rl * kQMax + q];
+    }
+  }
+}
+
+// Digit-major epilogue: acc[r][s] lane l, element v holds digit-s partial of
+// query grp*16 + l%16 for row 4*(l/16) + v of strip r, so a lane combines its
+// own NT tiles (sum of acc << 8s) and the wave stages the values in red
+// (nstrips*16 rows x 16 queries) for row-contiguous stores.
+template <int NS, int NT>
+__device__ __forceinline__ void WaveEpilogueDigitMajor(const v4i (&acc)[NS][NT], u64* red,
+                                                       int lane, int grp, int nq, int rows,
+                                                       long long row0, int ks, u64* partial) {
+  constexpr int kRows = NS * 16;
+  const int ql = lane & 15;
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      u64 sum = 0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        sum += static_cast<u64>(static_cast<i64>(acc[r][t][v])) << (8 * t);
+      }
+      red[(r * 16 + 4 * (lane >> 4) + v) * kQMax + ql] = sum;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int i = lane; i < kRows * 16; i += 64) {
+    const int q = i / kRows;
+    const int rl = i % kRows;
+    const int qg = grp * 16 + q;
     const long long row = row0 + rl;
     if (qg < nq && row < rows) {
       partial[((long long)ks * nq + qg) * rows + row] = red[rl * kQMax + q];
@@ -440,7 +475,7 @@ __device__ __forceinline__ bool DecodeWideTile(int w, int nx, int ntiles, int nz
   return true;
 }
 
-template <int NP, int CT>
+template <int NP, int CT, bool DM>
 __global__ __launch_bounds__(512) void WideScanKernel(
     const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
     const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
@@ -500,6 +535,20 @@ __global__ __launch_bounds__(512) void WideScanKernel(
     v4i b0[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) b0[c] = st[W::kA + (g * CT + c) * 64 + lane];
+    if constexpr (DM) {
+      // Output tile c takes plane p times digit c - p: no data movement.
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          const v4i af = st[((h * RW + r) * NP + p) * 64 + lane];
+#pragma unroll
+          for (int c = p; c < CT; ++c)
+            acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, b0[c - p], acc[r][c], 0, 0, 0);
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       v4i bp[CT];
@@ -540,8 +589,13 @@ __global__ __launch_bounds__(512) void WideScanKernel(
   WaitVmcnt<0>();
   LdsBarrier();
   u64* red = reinterpret_cast<u64*>(lds) + wave * (RW * 16 * kQMax);
-  WaveEpilogue<RW, CT>(acc, red, lane, ns, qpg, grp, nq, rows,
-                       (long long)(strip0 + h * RW) * 16, ks, partial);
+  if constexpr (DM) {
+    WaveEpilogueDigitMajor<RW, CT>(acc, red, lane, grp, nq, rows,
+                                   (long long)(strip0 + h * RW) * 16, ks, partial);
+  } else {
+    WaveEpilogue<RW, CT>(acc, red, lane, ns, qpg, grp, nq, rows,
+                         (long long)(strip0 + h * RW) * 16, ks, partial);
+  }
 }
 
 int GridBlocks(long long total, int threads) {
@@ -565,7 +619,7 @@ hipError_t LaunchNarrowT(const int8_t* tiles, const LimbLayout& L, const ScanPla
   return hipGetLastError();
 }
 
-template <int NP, int CT>
+template <int NP, int CT, bool DM>
 hipError_t LaunchWideT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                        const int8_t* btiles, u64* partial, hipStream_t s) {
   using W = WideShape<NP, CT>;
@@ -573,7 +627,7 @@ hipError_t LaunchWideT(const int8_t* tiles, const LimbLayout& L, const ScanPlan&
   const long long ntiles = static_cast<long long>((L.strips + W::RS - 1) / W::RS) * P.ksplit;
   const long long blocks = (ntiles + 7) / 8 * 8 * (P.ngp / 4);
   if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((WideScanKernel<NP, CT>), dim3(static_cast<unsigned>(blocks)), dim3(512),
+  hipLaunchKernelGGL((WideScanKernel<NP, CT, DM>), dim3(static_cast<unsigned>(blocks)), dim3(512),
                      W::kLdsBytes, s, tiles, L.strips, L.rows, L.kblocks, P.kb_per_split,
                      P.ksplit, btiles, P.ngp, P.nq, P.ns, P.qpg, partial);
   return hipGetLastError();
@@ -591,9 +645,11 @@ hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan
     }
   }
   switch (P.ctg) {
-    case 2: return LaunchWideT<NP, 2>(tiles, L, P, btiles, partial, s);
-    case 4: return LaunchWideT<NP, 4>(tiles, L, P, btiles, partial, s);
-    case 7: return LaunchWideT<NP, 7>(tiles, L, P, btiles, partial, s);
+    case 2: return LaunchWideT<NP, 2, false>(tiles, L, P, btiles, partial, s);
+    case 4: return LaunchWideT<NP, 4, false>(tiles, L, P, btiles, partial, s);
+    case 7:
+      return P.digit_major ? LaunchWideT<NP, 7, true>(tiles, L, P, btiles, partial, s)
+                           : LaunchWideT<NP, 7, false>(tiles, L, P, btiles, partial, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -703,9 +759,9 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
   return hipGetLastError();
 }
 
-template <int NP, int CT>
+template <int NP, int CT, bool DM = false>
 hipError_t ConfigureWide() {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanKernel<NP, CT>),
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanKernel<NP, CT, DM>),
                              hipFuncAttributeMaxDynamicSharedMemorySize,
                              static_cast<int>(WideShape<NP, CT>::kLdsBytes));
 }
@@ -715,7 +771,9 @@ hipError_t ConfigureWide() {
 hipError_t ConfigureScanKernels() {
   const hipError_t es[] = {ConfigureWide<1, 2>(), ConfigureWide<1, 4>(), ConfigureWide<1, 7>(),
                            ConfigureWide<2, 2>(), ConfigureWide<2, 4>(), ConfigureWide<2, 7>(),
-                           ConfigureWide<4, 2>(), ConfigureWide<4, 4>(), ConfigureWide<4, 7>()};
+                           ConfigureWide<4, 2>(), ConfigureWide<4, 4>(), ConfigureWide<4, 7>(),
+                           ConfigureWide<1, 7, true>(), ConfigureWide<2, 7, true>(),
+                           ConfigureWide<4, 7, true>()};
   for (hipError_t e : es) {
     if (e != hipSuccess) return e;
   }
@@ -734,7 +792,7 @@ constexpr int kWideTiles[] = {2, 4, 7};
 // Finishes a plan whose kind and tile width are set: groups, K-split and
 // B-tile size. ks_scale multiplies the workgroup target of the K-split.
 void CompletePlan(const LimbLayout& L, int ks_scale, ScanPlan* P) {
-  P->qpg = QueriesPerGroup(P->ctg, P->ns);
+  P->qpg = P->digit_major ? 16 : QueriesPerGroup(P->ctg, P->ns);
   const int groups = (P->nq + P->qpg - 1) / P->qpg;
   P->ngp = P->wide ? (groups + 3) / 4 * 4 : groups;
   // K-split: exact int32 sums need at most 65536 / planes columns per split;
@@ -753,32 +811,43 @@ void CompletePlan(const LimbLayout& L, int ks_scale, ScanPlan* P) {
   P->btile_bytes = static_cast<size_t>(L.kblocks + 1) * P->ngp * P->ctg * kTileBytes;
 }
 
+// Digit-major WideScan is instantiated for this many shifts per query.
+constexpr int kDigitMajorShifts = 7;
+
 std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits) {
   std::vector<ScanPlan> out;
   const int ns = (out_bits + 7) / 8;
+  auto add = [&](int wide, int t, int dm) {
+    for (int ks_scale : {1, 2}) {
+      ScanPlan P{};
+      P.wide = wide;
+      P.nq = nq;
+      P.ns = ns;
+      P.ctg = t;
+      P.digit_major = dm;
+      CompletePlan(L, ks_scale, &P);
+      bool dup = false;
+      for (const ScanPlan& o : out) {
+        dup = dup || (o.wide == P.wide && o.ctg == P.ctg && o.ksplit == P.ksplit &&
+                      o.digit_major == P.digit_major);
+      }
+      if (!dup) out.push_back(P);
+    }
+  };
   for (int wide = 0; wide < 2; ++wide) {
     for (int t : (wide ? std::vector<int>{2, 4, 7} : std::vector<int>{1, 2, 4})) {
       // A narrow workgroup re-reads A for every group: keep those with few groups.
       if (!wide && (nq + QueriesPerGroup(t, ns) - 1) / QueriesPerGroup(t, ns) > 2) continue;
-      for (int ks_scale : {1, 2}) {
-        ScanPlan P{};
-        P.wide = wide;
-        P.nq = nq;
-        P.ns = ns;
-        P.ctg = t;
-        CompletePlan(L, ks_scale, &P);
-        bool dup = false;
-        for (const ScanPlan& o : out) {
-          dup = dup || (o.wide == P.wide && o.ctg == P.ctg && o.ksplit == P.ksplit);
-        }
-        if (!dup) out.push_back(P);
-      }
+      add(wide, t, 0);
     }
   }
+  // Digit-major groups hold 16 queries: worth trying from two groups on.
+  if (ns == kDigitMajorShifts && nq > 16) add(1, ns, 1);
   return out;
 }
 
-ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries) {
+ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries,
+                      bool allow_digit_major) {
   ScanPlan P{};
   P.nq = nq;
   P.ns = (out_bits + 7) / 8;
@@ -795,6 +864,12 @@ ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_qu
   if (narrow_tiles > 0 && nq < wide_min_queries) {
     P.wide = 0;
     P.ctg = narrow_tiles;
+  } else if (allow_digit_major && ns == kDigitMajorShifts && nq > 32) {
+    // Digit-major: no plane shifts and (2*ns - 1)/(2*ns) of the MFMAs, at a
+    // granularity of 64 queries per column block.
+    P.wide = 1;
+    P.ctg = ns;
+    P.digit_major = 1;
   } else {
     P.wide = 1;
     P.ctg = kWideTiles[2];
@@ -836,7 +911,8 @@ hipError_t LaunchQueryDigits(const u64* vec, int stride, int len, const LimbLayo
   // kblocks + 1 blocks: the last one is all zero (its k's are >= len).
   const long long total = (long long)(L.kblocks + 1) * P.ngp * P.ctg * 64;
   hipLaunchKernelGGL(QueryDigitsKernel, dim3(GridBlocks(total, 256)), dim3(256), 0, s, vec,
-                     P.nq, stride, len, L.kblocks + 1, P.ns, P.qpg, P.ngp, P.ctg, btiles);
+                     P.nq, stride, len, L.kblocks + 1, P.ns, P.qpg, P.ngp, P.ctg,
+                     P.digit_major, btiles);
   return hipGetLastError();
 }
 
@@ -859,6 +935,10 @@ hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& 
                       const int8_t* btiles, u64* partial, hipStream_t s) {
   if (P.nq <= 0) return hipSuccess;
   if (P.qpg < 1 || P.ngp < 1) return hipErrorInvalidValue;
+  if (P.digit_major && (!P.wide || P.ctg != kDigitMajorShifts || P.ns != P.ctg ||
+                        P.qpg != 16 || P.ngp % 4 != 0)) {
+    return hipErrorInvalidValue;
+  }
   if (P.group_rows > 0 && (P.wide || P.group_rows % (16 * kNarrowR) != 0 ||
                            static_cast<long long>(P.group_rows) * P.ngp < L.rows)) {
     return hipErrorInvalidValue;

@@ -124,6 +124,12 @@ struct ScanPlan {
   int ns;            // shifts per query
   int ctg;           // tiles per group
   int qpg;           // queries per group
+  // B column order. 0 (query-major): column j of a group is (query j / ns,
+  // shift j % ns) and plane p's operand is plane 0's shifted p columns.
+  // 1 (digit-major, WideScan with ctg == ns only): tile s of a group holds
+  // digit s of its 16 queries, so plane p's operand for output tile s is
+  // plane 0's tile s - p, unchanged.
+  int digit_major;
   int ngp;           // groups in the B layout (padded for WideScan)
   int ksplit;        // k-ranges (exact int32 sums and occupancy)
   int kb_per_split;
@@ -136,7 +142,8 @@ struct ScanPlan {
 
 // wide_min_queries: batches of at least this many queries use WideScan even
 // when they would fit NarrowScan.
-ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries);
+ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries,
+                      bool allow_digit_major = true);
 // The configurations an autotuner may time for nq queries (kind, tile width,
 // K-split).
 std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits);

@@ -367,7 +367,7 @@ ScanPlan PlanFor(const pirg_server* S, bool db, int nq) {
   const auto& tuned = db ? S->tuned_db : S->tuned_pk;
   auto it = tuned.find(nq);
   if (it != tuned.end()) return it->second;
-  return MakeScanPlan(db ? S->dbL : S->pkL, nq, S->log_q, S->wide_min);
+  return MakeScanPlan(db ? S->dbL : S->pkL, nq, S->log_q, S->wide_min, S->digit_major);
 }
 
 namespace {
@@ -770,6 +770,9 @@ Error CreateFromParts(const pirg_params* params, const pirg_preprocessed_view* v
     const int v = std::atoi(env);
     if (v > 0) S->wide_min = v;
   }
+  // PIRG_DIGIT_MAJOR=0 keeps WideScan on the query-major B layout (tests
+  // cover both layouts at the same batch sizes with it).
+  if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
   er = ValidateView(S.get(), v);
   if (!er.ok()) return er;
   er = CheckDevice();
@@ -845,8 +848,8 @@ namespace {
 
 std::string DescribePlan(const ScanPlan& P) {
   char buf[128];
-  std::snprintf(buf, sizeof(buf), "%s tiles=%d ksplit=%d", P.wide ? "wide" : "narrow", P.ctg,
-                P.ksplit);
+  std::snprintf(buf, sizeof(buf), "%s tiles=%d ksplit=%d",
+                P.digit_major ? "wide-dm" : (P.wide ? "wide" : "narrow"), P.ctg, P.ksplit);
   return buf;
 }
 
