@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <vector>
 
+#include <type_traits>
+
 #include "pirg_internal.h"
 
 namespace pirg {
@@ -413,7 +415,8 @@ __global__ __launch_bounds__(256) void NarrowScanKernel(
 //    issue order).
 //  * The barrier is s_barrier after an LDS-only wait: HIP's __syncthreads
 //    also waits for every outstanding load, which would drain the ring.
-//  * Plane p's B fragments are derived from plane 0's with DPP shifts.
+//  * Query-major B: plane p's fragments are derived from plane 0's with DPP
+//    shifts. Digit-major B: plane p uses digit tile s - p as loaded.
 __device__ __forceinline__ void LdsBarrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
@@ -598,6 +601,128 @@ __global__ __launch_bounds__(512) void WideScanKernel(
   }
 }
 
+// Digit-major WideScan with software-pipelined fragments (NP <= 2): the
+// fragments of step i + 1 are read from LDS into a second register set while
+// step i's MFMAs run, and the step's LDS-DMA issues and fragment reads are
+// spread between its RW x NP MFMA blocks (fenced by sched_barrier) so the
+// memory pipeline drains while the matrix cores work. Same LDS ring, tile
+// order, vmcnt accounting and epilogue as WideScanKernel; the loop runs an
+// even number of steps (a step past the k-range reads the all-zero B block).
+template <int NP, int CT>
+__global__ __launch_bounds__(512) void WideScanPipeKernel(
+    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
+    const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial) {
+  using W = WideShape<NP, CT>;
+  constexpr int RS = W::RS, RW = W::RW, AF = W::AF, S = W::S;
+  extern __shared__ v4i lds[];
+  const int nx = (strips + RS - 1) / RS;
+  WideTile wt;
+  if (!DecodeWideTile(blockIdx.x, nx, nx * ksplit, ngp / 4, &wt)) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int h = wave >> 2;
+  const int g = wave & 3;
+  const int strip0 = wt.strip_block * RS;
+  const int ks = wt.ks;
+  const int grp = wt.zb * 4 + g;
+  const int kb0 = ks * kbps;
+  const int nkb = min(kblocks, kb0 + kbps) - kb0;
+  const v4i* A = reinterpret_cast<const v4i*>(tiles);
+  const v4i* Bg = reinterpret_cast<const v4i*>(btiles) + (long long)grp * CT * 64 + lane;
+  const v4i* Af[AF];
+#pragma unroll
+  for (int jf = 0; jf < AF; ++jf) {
+    const int f = wave * AF + jf;
+    Af[jf] = A + TileIndex(min(strip0 + f / NP, strips - 1), 0, f % NP, kblocks, NP) * 64 + lane;
+  }
+  const int c0 = h == 0 ? 0 : W::BH0;
+  v4i acc[RW][CT];
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[r][c] = v4i{0, 0, 0, 0};
+
+  constexpr int kDma = AF + W::BH0;    // DMA slots of a step (an h = 1 wave skips the last)
+  constexpr int kFrag = CT + RW * NP;  // fragments a wave reads per step
+  constexpr int kBlk = RW * NP;        // MFMA blocks per step, one per (strip, plane)
+  // DMA slot j of step i: A fragment j, then this wave's share of the B tiles.
+  auto dma = [&](int i, int j) {
+    v4i* st = lds + (i % S) * W::kStage;
+    if (j < AF) {
+      const long long kba = kb0 + min(i, nkb - 1);
+      DmaLoadStream16(Af[j] + kba * kStripGroup * NP * 64, st + (wave * AF + j) * 64);
+    } else {
+      const int cc = j - AF;
+      if (h == 0 || cc < W::BH1) {
+        const long long kbb = i < nkb ? kb0 + i : kblocks;
+        const int c = c0 + cc;
+        DmaLoad16(Bg + (kbb * ngp * CT + c) * 64, st + W::kA + (g * CT + c) * 64);
+      }
+    }
+  };
+  auto frag = [&](int i, int j, v4i (&a)[RW][NP], v4i (&b)[CT]) {
+    const v4i* st = lds + (i % S) * W::kStage;
+    if (j < CT) {
+      b[j] = st[W::kA + (g * CT + j) * 64 + lane];
+    } else {
+      const int rp = j - CT;
+      a[rp / NP][rp % NP] = st[((h * RW + rp / NP) * NP + rp % NP) * 64 + lane];
+    }
+  };
+  // Step i on (ca, cb); DMAs of step i + S; fragments of step i + 1 into (na, nb).
+  auto step = [&](int i, v4i (&ca)[RW][NP], v4i (&cb)[CT], v4i (&na)[RW][NP],
+                  v4i (&nb)[CT]) {
+#pragma unroll
+    for (int blk = 0; blk < kBlk; ++blk) {
+      const int r = blk / NP, p = blk % NP;
+#pragma unroll
+      for (int c = p; c < CT; ++c)
+        acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ca[r][p], cb[c - p], acc[r][c], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = blk * kDma / kBlk; j < (blk + 1) * kDma / kBlk; ++j) dma(i + S, j);
+#pragma unroll
+      for (int j = blk * kFrag / kBlk; j < (blk + 1) * kFrag / kBlk; ++j) frag(i + 1, j, na, nb);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  // Own DMAs per step: all but the newest (S - 1 or S - 2) steps' must have landed.
+  auto wait_all_but = [&](auto steps_c) {
+    constexpr int n = decltype(steps_c)::value;
+    if (h == 0) {
+      WaitVmcnt<(AF + W::BH0) * n>();
+    } else {
+      WaitVmcnt<(AF + W::BH1) * n>();
+    }
+  };
+  v4i fa0[RW][NP], fb0[CT], fa1[RW][NP], fb1[CT];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+#pragma unroll
+    for (int j = 0; j < kDma; ++j) dma(s, j);
+  }
+  wait_all_but(std::integral_constant<int, S - 1>{});
+  LdsBarrier();
+#pragma unroll
+  for (int j = 0; j < kFrag; ++j) frag(0, j, fa0, fb0);
+  const int nkb2 = (nkb + 1) & ~1;
+  for (int i = 0; i < nkb2; i += 2) {
+    // Step i+1's DMAs have landed everywhere; step i's fragments are in
+    // registers, so its stage is free for step i + S.
+    wait_all_but(std::integral_constant<int, S - 2>{});
+    LdsBarrier();
+    step(i, fa0, fb0, fa1, fb1);
+    wait_all_but(std::integral_constant<int, S - 2>{});
+    LdsBarrier();
+    step(i + 1, fa1, fb1, fa0, fb0);
+  }
+  WaitVmcnt<0>();
+  LdsBarrier();
+  u64* red = reinterpret_cast<u64*>(lds) + wave * (RW * 16 * kQMax);
+  WaveEpilogueDigitMajor<RW, CT>(acc, red, lane, grp, nq, rows,
+                                 (long long)(strip0 + h * RW) * 16, ks, partial);
+}
+
 int GridBlocks(long long total, int threads) {
   long long b = (total + threads - 1) / threads;
   if (b > 65535LL * 16) b = 65535LL * 16;
@@ -633,6 +758,19 @@ hipError_t LaunchWideT(const int8_t* tiles, const LimbLayout& L, const ScanPlan&
   return hipGetLastError();
 }
 
+template <int NP, int CT>
+hipError_t LaunchWidePipeT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
+                           const int8_t* btiles, u64* partial, hipStream_t s) {
+  using W = WideShape<NP, CT>;
+  const long long ntiles = static_cast<long long>((L.strips + W::RS - 1) / W::RS) * P.ksplit;
+  const long long blocks = (ntiles + 7) / 8 * 8 * (P.ngp / 4);
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((WideScanPipeKernel<NP, CT>), dim3(static_cast<unsigned>(blocks)),
+                     dim3(512), W::kLdsBytes, s, tiles, L.strips, L.rows, L.kblocks,
+                     P.kb_per_split, P.ksplit, btiles, P.ngp, P.nq, partial);
+  return hipGetLastError();
+}
+
 template <int NP>
 hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                         const int8_t* btiles, u64* partial, hipStream_t s) {
@@ -648,8 +786,13 @@ hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan
     case 2: return LaunchWideT<NP, 2, false>(tiles, L, P, btiles, partial, s);
     case 4: return LaunchWideT<NP, 4, false>(tiles, L, P, btiles, partial, s);
     case 7:
-      return P.digit_major ? LaunchWideT<NP, 7, true>(tiles, L, P, btiles, partial, s)
-                           : LaunchWideT<NP, 7, false>(tiles, L, P, btiles, partial, s);
+      if (!P.digit_major) return LaunchWideT<NP, 7, false>(tiles, L, P, btiles, partial, s);
+      // Two register sets of fragments fit beside the accumulators for NP <= 2.
+      if constexpr (NP <= 2) {
+        return LaunchWidePipeT<NP, 7>(tiles, L, P, btiles, partial, s);
+      } else {
+        return LaunchWideT<NP, 7, true>(tiles, L, P, btiles, partial, s);
+      }
     default: return hipErrorInvalidValue;
   }
 }
@@ -766,14 +909,21 @@ hipError_t ConfigureWide() {
                              static_cast<int>(WideShape<NP, CT>::kLdsBytes));
 }
 
+template <int NP, int CT>
+hipError_t ConfigureWidePipe() {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanPipeKernel<NP, CT>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(WideShape<NP, CT>::kLdsBytes));
+}
+
 // The WideScan instantiations need more than the default 64 KiB of dynamic
 // LDS; set once, outside any stream capture.
 hipError_t ConfigureScanKernels() {
   const hipError_t es[] = {ConfigureWide<1, 2>(), ConfigureWide<1, 4>(), ConfigureWide<1, 7>(),
                            ConfigureWide<2, 2>(), ConfigureWide<2, 4>(), ConfigureWide<2, 7>(),
                            ConfigureWide<4, 2>(), ConfigureWide<4, 4>(), ConfigureWide<4, 7>(),
-                           ConfigureWide<1, 7, true>(), ConfigureWide<2, 7, true>(),
-                           ConfigureWide<4, 7, true>()};
+                           ConfigureWide<4, 7, true>(), ConfigureWidePipe<1, 7>(),
+                           ConfigureWidePipe<2, 7>()};
   for (hipError_t e : es) {
     if (e != hipSuccess) return e;
   }
