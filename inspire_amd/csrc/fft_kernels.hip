@@ -81,15 +81,23 @@ __device__ __forceinline__ void MarginMax(double local, unsigned long long* marg
 // (output, chunk pair): acc[j] = sum_i U_i,a[j] * V_i,b[j] in term order, the
 // inverse FFT, real / 2d, round half away from zero, shift by chunk_bits*(a+b)
 // and fold X^d = -1. One workgroup per (output, pair).
+//
+// Job order: the G = outs_per_group * npairs jobs of one group (outputs that
+// read the same u spectra) take consecutive slots on one XCD (workgroup w runs
+// on XCD w % 8), so that group's spectra are fetched into one L2 rather than
+// into all eight; the grid is padded to whole rounds of 8 groups.
 template <int L>
 __global__ __launch_bounds__(Geo<L>::NT) void IpKernel(
-    IpJob jobs, int nterms, int nu, int nv, PairList pairs,
+    IpJob jobs, int n_jobs, int group_jobs, int nterms, int nu, int nv, PairList pairs,
     const double2* __restrict__ st, unsigned long long* margin) {
   extern __shared__ double2 lds[];
   constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
   const int t = threadIdx.x;
-  const int o = blockIdx.x / pairs.count;
-  const int pi = blockIdx.x % pairs.count;
+  const int w = blockIdx.x;
+  const int job = ((w >> 3) / group_jobs * 8 + (w & 7)) * group_jobs + (w >> 3) % group_jobs;
+  if (job >= n_jobs) return;
+  const int o = job / pairs.count;
+  const int pi = job % pairs.count;
   const int a = pairs.a[pi], b = pairs.b[pi], sh = pairs.shift[pi];
   const double2* U = jobs.uptr[o] + t;
   const double2* V = jobs.vptr[o] + t;
@@ -330,12 +338,17 @@ struct SpectraL {
 
 template <int L>
 struct IpL {
-  static hipError_t Run(const IpJob& jobs, int n_out, int nterms, int nu, int nv,
-                        const PairList& pairs, const double2* st,
+  static hipError_t Run(const IpJob& jobs, int n_out, int outs_per_group, int nterms, int nu,
+                        int nv, const PairList& pairs, const double2* st,
                         unsigned long long* margin, hipStream_t s) {
-    hipLaunchKernelGGL(IpKernel<L>, dim3(n_out * pairs.count), dim3(Geo<L>::NT),
-                       Geo<L>::kLds * sizeof(double2), s, jobs, nterms, nu, nv, pairs, st,
-                       margin);
+    const long long n_jobs = static_cast<long long>(n_out) * pairs.count;
+    const long long group_jobs = static_cast<long long>(outs_per_group) * pairs.count;
+    const long long groups = (n_out + outs_per_group - 1) / outs_per_group;
+    const long long blocks = (groups + 7) / 8 * 8 * group_jobs;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(IpKernel<L>, dim3(static_cast<unsigned>(blocks)), dim3(Geo<L>::NT),
+                       Geo<L>::kLds * sizeof(double2), s, jobs, static_cast<int>(n_jobs),
+                       static_cast<int>(group_jobs), nterms, nu, nv, pairs, st, margin);
     return hipGetLastError();
   }
 };
@@ -375,14 +388,15 @@ hipError_t LaunchSpectra(const u64* src, int npolys, int per_group,
 }
 
 hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
-                    u64* const* optr, int n_out, int nterms, int nu, int nv,
-                    const PairList& pairs, const FftDims& F,
+                    u64* const* optr, int n_out, int outs_per_group, int nterms, int nu,
+                    int nv, const PairList& pairs, const FftDims& F,
                     const double2* tw_bwd, unsigned long long* margin,
                     hipStream_t s) {
   if (n_out <= 0 || pairs.count <= 0) return hipSuccess;
-  if (!ValidDims(F)) return hipErrorInvalidValue;
+  if (!ValidDims(F) || outs_per_group <= 0) return hipErrorInvalidValue;
   IpJob jobs{uptr, vptr, optr};
-  return DispatchLogN<IpL>(F.log_N, jobs, n_out, nterms, nu, nv, pairs, tw_bwd, margin, s);
+  return DispatchLogN<IpL>(F.log_N, jobs, n_out, outs_per_group, nterms, nu, nv, pairs,
+                           tw_bwd, margin, s);
 }
 
 hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,

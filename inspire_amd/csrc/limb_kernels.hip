@@ -173,6 +173,44 @@ __global__ void QueryDigitsKernel(const u64* __restrict__ vec, int nq, int strid
   }
 }
 
+// Digit-major B tiles (ctg == ns, qpg == 16), one thread per (kb, group,
+// lane): the lane's query q = group*16 + lane%16 and k's 16*(lane/16) .. +15
+// are read once and all ns balanced digits are produced in one carry pass
+// (the recurrence of BalancedDigit), one 16-byte store per digit tile.
+__global__ void QueryDigitsDigitMajorKernel(const u64* __restrict__ vec, int nq, int stride,
+                                            int len, int kblocks, int ns, int ngp,
+                                            int8_t* __restrict__ btiles) {
+  const long long total = (long long)kblocks * ngp * 64;
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+       g += (long long)gridDim.x * blockDim.x) {
+    const int lane = static_cast<int>(g & 63);
+    const long long t = g >> 6;
+    const int grp = static_cast<int>(t % ngp);
+    const int kb = static_cast<int>(t / ngp);
+    const int q = grp * 16 + (lane & 15);
+    const int kbase = kb * 64 + 16 * (lane >> 4);
+    u64 v[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      const int k = kbase + b;
+      v[b] = (q < nq && k < len) ? vec[(long long)q * stride + k] : 0ULL;
+    }
+    int8_t* dst = btiles + ((((long long)kb * ngp + grp) * ns) * 64 + lane) * 16;
+    for (int sft = 0; sft < ns; ++sft) {
+      int8_t out[16];
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const unsigned byte = static_cast<unsigned>(v[b] & 255u);
+        const bool hi = byte >= 128u;
+        out[b] = static_cast<int8_t>(hi ? static_cast<int>(byte) - 256 : static_cast<int>(byte));
+        v[b] = (v[b] >> 8) + (hi ? 1 : 0);
+      }
+      *reinterpret_cast<int4*>(dst + (long long)sft * 64 * 16) =
+          *reinterpret_cast<const int4*>(out);
+    }
+  }
+}
+
 // corr[q] = c_off * sum_k vec[q][k] mod 2^64, in two passes: partial sums of
 // chunks, then one thread per query.
 __global__ void VectorPartialSumKernel(const u64* __restrict__ vec, int stride,
@@ -1059,6 +1097,12 @@ ScanPlan MakeGroupedScanPlan(const LimbLayout& L, int ngroups, int group_rows, i
 hipError_t LaunchQueryDigits(const u64* vec, int stride, int len, const LimbLayout& L,
                              const ScanPlan& P, int8_t* btiles, hipStream_t s) {
   // kblocks + 1 blocks: the last one is all zero (its k's are >= len).
+  if (P.digit_major) {
+    const long long total = (long long)(L.kblocks + 1) * P.ngp * 64;
+    hipLaunchKernelGGL(QueryDigitsDigitMajorKernel, dim3(GridBlocks(total, 256)), dim3(256), 0,
+                       s, vec, P.nq, stride, len, L.kblocks + 1, P.ns, P.ngp, btiles);
+    return hipGetLastError();
+  }
   const long long total = (long long)(L.kblocks + 1) * P.ngp * P.ctg * 64;
   hipLaunchKernelGGL(QueryDigitsKernel, dim3(GridBlocks(total, 256)), dim3(256), 0, s, vec,
                      P.nq, stride, len, L.kblocks + 1, P.ns, P.qpg, P.ngp, P.ctg,

@@ -82,7 +82,7 @@ Error Enqueue(pirg_packer* K, bool timed) {
   void* const* tab = K->fin_tab.as<void* const>();
   PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(tab),
                     reinterpret_cast<const double2* const*>(tab + J),
-                    reinterpret_cast<u64* const*>(tab + 2 * J), J, k, K->fin_u.num_chunks,
+                    reinterpret_cast<u64* const*>(tab + 2 * J), J, 1, k, K->fin_u.num_chunks,
                     K->fin_v.num_chunks, K->fin_pairs, K->F, K->tw_bwd.as<double2>(),
                     K->margin.as<unsigned long long>(), st));
   // b_agg = LowBits(LowBits(b + b_part) + inner); (a_tilde_agg, b_agg).

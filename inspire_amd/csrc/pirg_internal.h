@@ -227,9 +227,11 @@ struct ExtCombineArgs {
   int nq, ell, t, d, step_i, log_q, log_q1, log_q2;
 };
 
+// outs_per_group: consecutive outputs that share their u spectra (one
+// request's outputs); their workgroups are placed on one XCD.
 hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
-                    u64* const* optr, int n_out, int nterms, int nu, int nv,
-                    const PairList& pairs, const FftDims& F,
+                    u64* const* optr, int n_out, int outs_per_group, int nterms, int nu,
+                    int nv, const PairList& pairs, const FftDims& F,
                     const double2* tw_bwd, unsigned long long* margin,
                     hipStream_t s);
 hipError_t LaunchFinalizeCombine(const FinalizeArgs& A, hipStream_t s);

@@ -576,7 +576,7 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
     PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(tab),
                       reinterpret_cast<const double2* const*>(tab + n_fin_cap),
                       reinterpret_cast<u64* const*>(tab + 2 * n_fin_cap),
-                      nq * lt, S->k, S->fin_u.num_chunks, S->fin_v.num_chunks,
+                      nq * lt, lt, S->k, S->fin_u.num_chunks, S->fin_v.num_chunks,
                       S->fin_pairs, S->F, S->tw_bwd.as<double2>(), margin, st));
   }
   FinalizeArgs FA{};
@@ -626,7 +626,7 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
       PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(et),
                         reinterpret_cast<const double2* const*>(et + n_ext_cap),
                         reinterpret_cast<u64* const*>(et + 2 * n_ext_cap),
-                        nq * S->ell * 2, 2 * S->e, S->ext_u.num_chunks,
+                        nq * S->ell * 2, S->ell * 2, 2 * S->e, S->ext_u.num_chunks,
                         S->ext_v.num_chunks, S->ext_pairs, S->F, S->tw_bwd.as<double2>(),
                         margin, st));
       ExtCombineArgs EA{};
@@ -1167,7 +1167,7 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
     void* const* t = tab.as<void* const>();
     PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(t),
                       reinterpret_cast<const double2* const*>(t + 1),
-                      reinterpret_cast<u64* const*>(t + 2), 1, n,
+                      reinterpret_cast<u64* const*>(t + 2), 1, 1, n,
                       ur.num_chunks, vr.num_chunks, pairs, F, twb.as<double2>(), nullptr, s));
     PIRG_HIP(LaunchSumPairs(part.as<u64>(), pairs.count, d, res.as<u64>(), s));
     PIRG_HIP(hipDeviceSynchronize());
