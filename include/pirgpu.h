@@ -287,6 +287,46 @@ int pirg_mfma_probe(int blocks, int iters, double* ms, double* tops);
 int pirg_fft_twiddles(int n, double* fwd, double* bwd);
 
 /* Version / device info string (for logs). */
+/* ---- Bit-packed formats ----
+ * Polynomial::ToProto (crypto/polynomial.cc:562-590) for npolys polynomials
+ * of len coefficients each (coeffs: npolys x len). out receives each
+ * polynomial's encoded_coeffs back to back, ceil(len * log_modulus / 8)
+ * bytes apiece: the low log_modulus bits of every coefficient, LSB first.
+ * Errors: "Invalid log_modulus." (log_modulus outside [1, 64]). */
+int pirg_poly_to_proto(const uint64_t* coeffs, int npolys, int len, int log_modulus,
+                       uint8_t* out, char* err, size_t errlen);
+/* Polynomial::CreateFromProto (crypto/polynomial.cc:208-252) for npolys
+ * byte strings of poly_bytes bytes each, with the reference's checks and
+ * messages: "Invalid length.", "Invalid log_modulus.", "Empty serialized
+ * polynomial.", "Unexpected serialized size.". out: npolys x len. */
+int pirg_poly_from_proto(const uint8_t* bytes, size_t poly_bytes, int npolys, int len,
+                         int log_modulus, uint64_t* out, char* err, size_t errlen);
+/* The same two conversions on device memory, queued on `stream` (a
+ * hipStream_t; NULL for the default stream). A polynomial's stream is
+ * ceil(len * log_modulus / 64) 64-bit words at a pitch of pitch_words words
+ * (bytes in the little-endian order of ToProto's string); words past a
+ * stream's end are written as zero by the encoder and ignored by the
+ * decoder. No counterpart in the reference: the device forms are for callers
+ * that keep requests and responses in HBM. */
+int pirg_poly_to_proto_device(const uint64_t* d_coeffs, int npolys, int len, int log_modulus,
+                              uint64_t* d_words, long long pitch_words, void* stream,
+                              char* err, size_t errlen);
+int pirg_poly_from_proto_device(const uint64_t* d_words, long long pitch_words, int npolys,
+                                int len, int log_modulus, uint64_t* d_coeffs, void* stream,
+                                char* err, size_t errlen);
+
+/* PirServer::LoadFromBuffer (crypto/pir/server/pir_server.cc:594-793): builds
+ * a server from the byte image PirServer::SaveToFile writes (:433-592) --
+ * combined DB matrix (raw condensed words), combined pack matrix (signed
+ * entries bit-packed at pack_log_digit + log2(d) bits), the preprocessed
+ * outputs and second_dim_a (bit-packed at log q bits). The pack matrix and
+ * the polynomials are unpacked on the GPU straight into the resident
+ * layouts. The image's shapes must match params (INVALID_ARGUMENT otherwise);
+ * a record cut short fails with "Error reading from buffer" (INTERNAL), as
+ * in the reference. */
+int pirg_server_load_from_buffer(const pirg_params* params, const uint8_t* buffer,
+                                 size_t size, pirg_server** out, char* err, size_t errlen);
+
 const char* pirg_version(void);
 
 #ifdef __cplusplus

@@ -299,6 +299,20 @@ class PirServer:
         _check(rc, err)
         return cls(handle, params)
 
+    @classmethod
+    def load_from_buffer(cls, params: PirParams, buffer: bytes) -> "PirServer":
+        """PirServer::LoadFromBuffer (pir_server.cc:594-793) from the byte image
+        SaveToFile writes (pirg_server_load_from_buffer)."""
+        buf = np.frombuffer(buffer, dtype=np.uint8) if len(buffer) else np.zeros(1, np.uint8)
+        handle = ctypes.c_void_p()
+        err = _err()
+        cp = params.c()
+        rc = lib().pirg_server_load_from_buffer(
+            ctypes.byref(cp), buf.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(len(buffer)),
+            ctypes.byref(handle), err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return cls(handle, params)
+
     def close(self) -> None:
         if self._h is not None and self._h.value:
             lib().pirg_server_destroy(self._h)
@@ -520,6 +534,60 @@ class PirPacker:
         lib().pirg_packer_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n),
                                 int(reset))
         return a.value, b.value, n.value
+
+
+def poly_to_proto(coeffs, log_modulus: int) -> List[bytes]:
+    """Polynomial::ToProto(log_modulus).encoded_coeffs (polynomial.cc:562-590)
+    for each row of coeffs (npolys x len), on the GPU (pirg_poly_to_proto)."""
+    c = _u64_2d(coeffs)
+    npolys, length = c.shape
+    nb = (length * log_modulus + 7) // 8 if 0 < log_modulus <= 64 else 0
+    out = np.zeros(max(npolys * nb, 1), dtype=np.uint8)
+    err = _err()
+    rc = lib().pirg_poly_to_proto(c.ctypes.data_as(_u64p), npolys, length, log_modulus,
+                                  out.ctypes.data_as(ctypes.c_void_p), err,
+                                  ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    return [out[i * nb:(i + 1) * nb].tobytes() for i in range(npolys)]
+
+
+def poly_from_proto(encoded: Sequence[bytes], length: int, log_modulus: int) -> np.ndarray:
+    """Polynomial::CreateFromProto (polynomial.cc:208-252) for byte strings of
+    one size, on the GPU (pirg_poly_from_proto): an npolys x length array."""
+    npolys = len(encoded)
+    nb = len(encoded[0]) if npolys else 0
+    if any(len(e) != nb for e in encoded):
+        raise PirError(INVALID_ARGUMENT, "Unexpected serialized size.")
+    blob = np.frombuffer(b"".join(encoded), dtype=np.uint8) if nb else np.zeros(1, np.uint8)
+    out = np.zeros((max(npolys, 1), max(length, 1)), dtype=np.uint64)
+    err = _err()
+    rc = lib().pirg_poly_from_proto(blob.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(nb),
+                                    npolys, length, log_modulus, out.ctypes.data_as(_u64p), err,
+                                    ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    return out[:npolys, :length] if npolys else np.zeros((0, max(length, 0)), np.uint64)
+
+
+def poly_to_proto_device(d_coeffs: int, npolys: int, length: int, log_modulus: int,
+                         d_words: int, pitch_words: int, stream: int = 0) -> None:
+    """pirg_poly_to_proto_device on device addresses (e.g. torch data_ptr())."""
+    err = _err()
+    rc = lib().pirg_poly_to_proto_device(ctypes.c_void_p(d_coeffs), npolys, length, log_modulus,
+                                         ctypes.c_void_p(d_words), ctypes.c_longlong(pitch_words),
+                                         ctypes.c_void_p(stream), err, ctypes.c_size_t(len(err)))
+    _check(rc, err)
+
+
+def poly_from_proto_device(d_words: int, pitch_words: int, npolys: int, length: int,
+                           log_modulus: int, d_coeffs: int, stream: int = 0) -> None:
+    """pirg_poly_from_proto_device on device addresses."""
+    err = _err()
+    rc = lib().pirg_poly_from_proto_device(ctypes.c_void_p(d_words),
+                                           ctypes.c_longlong(pitch_words), npolys, length,
+                                           log_modulus, ctypes.c_void_p(d_coeffs),
+                                           ctypes.c_void_p(stream), err,
+                                           ctypes.c_size_t(len(err)))
+    _check(rc, err)
 
 
 def matrix_multiply_u16(cdata: np.ndarray, cols: int, vec: np.ndarray) -> np.ndarray:

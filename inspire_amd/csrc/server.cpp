@@ -142,7 +142,6 @@ Error MakeRules(int n_terms, int d, int u_bits, int v_bits, int chunk,
 using namespace pirg;
 
 namespace pirg {
-namespace {
 
 // pir_params.cc:29-68 and the GPU path's own limits.
 Error DeriveServer(const pirg_params& P, pirg_server* S) {
@@ -214,8 +213,6 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
   return Ok();
 }
 
-}  // namespace
-
 // crypto/fft.h:71-80, from the g++-built table (twiddles.cpp), reordered by
 // stage for the kernels: stage s (1..log2 n) occupies entries
 // [2^(s-1) - 1, 2^s - 1), and entry 2^(s-1) - 1 + k is tw[k * (n >> s)], the
@@ -249,11 +246,55 @@ LimbLayout MakeLayout(int rows, int cols, int planes, u64 c_off) {
 
 namespace {
 
+// Pack rows [0, M) from a bit-packed image: row chunks of the stream are
+// staged, unpacked (sign-extended) into int32/int16 rows on the GPU and
+// converted to tiles.
+Error UploadPackedPack(pirg_server* S, const PackedPack& pk, int pk_cols) {
+  const int M = S->M;
+  const int eb = S->P.mat_bytes == 4 ? 4 : 2;
+  const size_t row_bits = static_cast<size_t>(pk_cols) * pk.bits;
+  size_t chunk_bytes = static_cast<size_t>(256) << 20;  // decoded rows per chunk
+  if (const char* env = std::getenv("PIRG_UPLOAD_CHUNK_BYTES")) {
+    const long long v = std::atoll(env);
+    if (v > 0) chunk_bytes = static_cast<size_t>(v);
+  }
+  int rows_per = static_cast<int>(
+      std::max<size_t>(16, chunk_bytes / (static_cast<size_t>(pk_cols) * eb)));
+  rows_per = std::max(16, rows_per / 16 * 16);
+  const int chunk_rows = std::min(rows_per, M);
+  const size_t max_bytes = (static_cast<size_t>(chunk_rows) * row_bits + 7) / 8 + 8;
+  DevBuf words, rowsbuf;
+  Error er = words.alloc((max_bytes + 7) / 8 * 8 + 8);
+  if (!er.ok()) return er;
+  er = rowsbuf.alloc(static_cast<size_t>(chunk_rows) * pk_cols * eb);
+  if (!er.ok()) return er;
+  hipStream_t st = S->main.stream;
+  for (int r0 = 0; r0 < M; r0 += rows_per) {
+    const int nr = std::min(rows_per, M - r0);
+    const size_t bit_a = static_cast<size_t>(r0) * row_bits;
+    const size_t bit_b = static_cast<size_t>(r0 + nr) * row_bits;
+    const size_t byte_a = bit_a / 8;
+    const size_t byte_b = (bit_b + 7) / 8;
+    if (byte_b > pk.nbytes) return Invalid("pack matrix bytes are short.");
+    PIRG_HIP(hipMemsetAsync(words.p, 0, words.bytes, st));
+    PIRG_HIP(hipMemcpyAsync(words.p, pk.bytes + byte_a, byte_b - byte_a, hipMemcpyHostToDevice,
+                            st));
+    PIRG_HIP(LaunchBitUnpack(words.as<u64>(), 0, static_cast<long long>(bit_a % 8), 1,
+                             static_cast<long long>(nr) * pk_cols, pk.bits, true, eb, rowsbuf.p,
+                             st));
+    PIRG_HIP(LaunchPackToTiles(rowsbuf.p, S->P.mat_bytes, M, pk_cols, r0, nr, S->pkL,
+                               S->pk_tiles.as<int8_t>(), st));
+    PIRG_HIP(hipStreamSynchronize(st));
+  }
+  return Ok();
+}
+
+// Polynomial sources may be host or device pointers (hipMemcpyDefault).
 Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts,
                     int db_cols, const std::vector<const void*>& pack_parts,
                     const std::vector<const uint64_t*>& a_parts,
                     const std::vector<const uint64_t*>& th_parts,
-                    const uint64_t* second_dim_a) {
+                    const uint64_t* second_dim_a, const PackedPack* packed = nullptr) {
   const int d = S->d;
   S->K = db_cols;
   PIRG_HIP(hipGetDevice(&S->device));
@@ -292,8 +333,13 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   if (!er.ok()) return er;
   const size_t pk_rowbytes = S->P.mat_bytes == 4 ? static_cast<size_t>(pk_cols) * 4
                                                  : static_cast<size_t>(pk_cols) * 2;
-  const int pk_rows_per_part = S->M / static_cast<int>(pack_parts.size());
-  for (size_t part = 0; part < pack_parts.size(); ++part) {
+  if (packed != nullptr) {
+    er = UploadPackedPack(S, *packed, pk_cols);
+    if (!er.ok()) return er;
+  }
+  const int pk_rows_per_part =
+      packed != nullptr ? 0 : S->M / static_cast<int>(pack_parts.size());
+  for (size_t part = 0; packed == nullptr && part < pack_parts.size(); ++part) {
     const int base = static_cast<int>(part) * pk_rows_per_part;
     er = UploadTiles(pack_parts[part], pk_rowbytes, pk_rows_per_part, false, S->main.stream,
                      [&](const void* src, int r0, int nr) {
@@ -310,14 +356,14 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   if (!er.ok()) return er;
   for (int oi = 0; oi < lt; ++oi) {
     PIRG_HIP(hipMemcpy(S->a_tilde.as<u64>() + static_cast<size_t>(oi) * d, a_parts[oi],
-                       static_cast<size_t>(d) * 8, hipMemcpyHostToDevice));
+                       static_cast<size_t>(d) * 8, hipMemcpyDefault));
   }
   DevBuf th;
   er = th.alloc(static_cast<size_t>(lt) * S->k * d * 8);
   if (!er.ok()) return er;
   for (int oi = 0; oi < lt; ++oi) {
     PIRG_HIP(hipMemcpy(th.as<u64>() + static_cast<size_t>(oi) * S->k * d, th_parts[oi],
-                       static_cast<size_t>(S->k) * d * 8, hipMemcpyHostToDevice));
+                       static_cast<size_t>(S->k) * d * 8, hipMemcpyDefault));
   }
 
   // Twiddles.
@@ -346,7 +392,7 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
       er = sda.alloc(static_cast<size_t>(2 * S->e) * d * 8);
       if (!er.ok()) return er;
       PIRG_HIP(hipMemcpy(sda.p, second_dim_a, static_cast<size_t>(2 * S->e) * d * 8,
-                         hipMemcpyHostToDevice));
+                         hipMemcpyDefault));
       er = S->sda_spec.alloc(static_cast<size_t>(2 * S->e) * S->ext_v.num_chunks * S->N *
                              sizeof(double2));
       if (!er.ok()) return er;
@@ -787,6 +833,50 @@ Error CreateFromParts(const pirg_params* params, const pirg_preprocessed_view* v
   *out = std::move(S);
   return Ok();
 }
+
+}  // namespace
+
+Error CreateFromImage(const pirg_params* params, const uint64_t* db_cdata, int db_cols,
+                      const PackedPack& pack, const uint64_t* a_tilde, const uint64_t* t_vec_h,
+                      const uint64_t* second_dim_a, std::unique_ptr<pirg_server>* out) {
+  std::unique_ptr<pirg_server> S(new pirg_server());
+  Error er = DeriveServer(*params, S.get());
+  if (!er.ok()) return er;
+  if (const char* env = std::getenv("PIRG_WIDE_MIN")) {
+    const int v = std::atoi(env);
+    if (v > 0) S->wide_min = v;
+  }
+  if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
+  // The image holds the combined matrices: view them as ell shards.
+  const int ell = S->ell, t = S->t, d = S->d, lt = ell * t;
+  std::vector<int> rows(ell, t * d), cols(ell, db_cols), olen(ell, t);
+  const size_t wpr = S->P.db_bytes == 2 ? db_cols / 4 : db_cols / 8;
+  std::vector<const uint64_t*> db(ell);
+  for (int s = 0; s < ell; ++s) db[s] = db_cdata + static_cast<size_t>(s) * t * d * wpr;
+  pirg_preprocessed_view v{};
+  v.num_db_matrices = ell;
+  v.db_cdata = db.data();
+  v.db_rows = rows.data();
+  v.db_cols = cols.data();
+  v.num_outputs = ell;
+  v.outputs_len = olen.data();
+  v.second_dim_a_len = t > 1 ? 2 * S->e : 0;
+  er = ValidateView(S.get(), &v);
+  if (!er.ok()) return er;
+  er = CheckDevice();
+  if (!er.ok()) return er;
+  std::vector<const uint64_t*> at(lt), th(lt);
+  for (int oi = 0; oi < lt; ++oi) {
+    at[oi] = a_tilde + static_cast<size_t>(oi) * d;
+    th[oi] = t_vec_h + static_cast<size_t>(oi) * S->k * d;
+  }
+  er = BuildResident(S.get(), db, db_cols, {}, at, th, second_dim_a, &pack);
+  if (!er.ok()) return er;
+  *out = std::move(S);
+  return Ok();
+}
+
+namespace {
 
 thread_local std::string g_version;
 

@@ -172,6 +172,8 @@ ScanPlan PlanFor(const pirg_server* S, bool db, int nq);
 
 Error CheckDevice();
 // crypto/lwes_to_rlwe.cc:60-74 and crypto/external_product.cc:100-102.
+// Sizes, chunk rules and checks of a server for params (pir_params.cc:29-68).
+Error DeriveServer(const pirg_params& P, pirg_server* S);
 int ChunkFor(int n_terms, int d);
 // crypto/polynomial_fft.cc:90-107.
 Error MakeRules(int n_terms, int d, int u_bits, int v_bits, int chunk, bool u_signed,
@@ -179,6 +181,30 @@ Error MakeRules(int n_terms, int d, int u_bits, int v_bits, int chunk, bool u_si
 // Stage-ordered FFT twiddle tables (n - 1 complex values each).
 void Twiddles(int n, std::vector<double2>* fwd, std::vector<double2>* bwd);
 LimbLayout MakeLayout(int rows, int cols, int planes, u64 c_off);
+
+// Bit-packed streams (codec.hip): value i of stream p is bits
+// [bit0 + i*bits, +bits) of the little-endian words at words + p*pitch.
+hipError_t LaunchBitUnpack(const u64* words, long long pitch, long long bit0, int nstreams,
+                           long long len, int bits, bool sign, int out_bytes, void* out,
+                           hipStream_t s);
+hipError_t LaunchBitPack(const u64* vals, int nstreams, long long len, int bits, u64* words,
+                         long long pitch, hipStream_t s);
+long long StreamWords(long long len, int bits);
+
+// The pack matrix of a server image: signed entries, row-major, bit-packed
+// at `bits` bits (host memory).
+struct PackedPack {
+  const uint8_t* bytes;
+  size_t nbytes;
+  int bits;
+};
+
+// Server from image parts: the combined DB's condensed words (host), the
+// bit-packed pack matrix, and the polynomials already in device memory
+// (a_tilde: lt x d, t_vec_h: lt x k x d, second_dim_a: 2e x d or null).
+Error CreateFromImage(const pirg_params* params, const uint64_t* db_cdata, int db_cols,
+                      const PackedPack& pack, const uint64_t* a_tilde, const uint64_t* t_vec_h,
+                      const uint64_t* second_dim_a, std::unique_ptr<pirg_server>* out);
 
 // Uploads rows [0, rows) of a host matrix in chunks and converts them to
 // tiles. `rowbytes` is the size of one host row.

@@ -585,3 +585,85 @@ def fill_u64(seed: int, n: int, mask: int = (1 << 64) - 1) -> np.ndarray:
                        ctypes.c_uint64(mask))
     return out
 
+
+
+# ---- bit-packed formats (crypto/polynomial.cc, crypto/pir/server/pir_server.cc) ----
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+def poly_to_proto(coeffs, log_modulus: int) -> bytes:
+    """Polynomial::ToProto(log_modulus).encoded_coeffs (polynomial.cc:562-590)."""
+    c = _u64(coeffs)
+    cap = (c.size * 64 + 7) // 8 + 1
+    out = np.zeros(cap, dtype=np.uint8)
+    n = ctypes.c_size_t(0)
+    _call(lib().orc_poly_to_proto, _p(c), ctypes.c_int(c.size), ctypes.c_int(log_modulus),
+          _p(out, _u8p), ctypes.c_size_t(cap), ctypes.byref(n))
+    return out[:n.value].tobytes()
+
+
+def poly_from_proto(data: bytes, length: int, log_modulus: int) -> np.ndarray:
+    """Polynomial::CreateFromProto (polynomial.cc:208-252)."""
+    b = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+    out = np.zeros(max(length, 1), dtype=np.uint64)
+    _call(lib().orc_poly_from_proto, _p(np.ascontiguousarray(b), _u8p),
+          ctypes.c_size_t(len(data)), ctypes.c_int(length), ctypes.c_int(log_modulus), _p(out))
+    return out[:length]
+
+
+def bit_pack(values, bits: int, is_signed: bool = False, value_bits: int = 64) -> bytes:
+    """BitPackSigned / BitPackCoefficients (pir_server.cc:309-399)."""
+    v = _u64(np.asarray(values).astype(np.int64).view(np.uint64) if is_signed else values)
+    lib().orc_bit_pack.restype = ctypes.c_size_t
+    n = lib().orc_bit_pack(_p(v), ctypes.c_size_t(v.size), ctypes.c_int(bits),
+                           ctypes.c_int(int(is_signed)), ctypes.c_int(value_bits), None)
+    out = np.zeros(max(n, 1), dtype=np.uint8)
+    lib().orc_bit_pack(_p(v), ctypes.c_size_t(v.size), ctypes.c_int(bits),
+                       ctypes.c_int(int(is_signed)), ctypes.c_int(value_bits), _p(out, _u8p))
+    return out[:n].tobytes()
+
+
+def bit_unpack(data: bytes, n: int, bits: int, is_signed: bool = False,
+               value_bits: int = 64) -> np.ndarray:
+    """BitUnpackSigned / BitUnpackCoefficients (pir_server.cc:338-429); signed
+    values come back as int64."""
+    b = np.frombuffer(data, dtype=np.uint8).copy() if len(data) else np.zeros(1, np.uint8)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    lib().orc_bit_unpack.restype = None
+    lib().orc_bit_unpack(_p(b, _u8p), ctypes.c_size_t(n), ctypes.c_int(bits),
+                         ctypes.c_int(int(is_signed)), ctypes.c_int(value_bits), _p(out))
+    return out[:n].view(np.int64) if is_signed else out[:n]
+
+
+def save_server(P: Params, mat: Material) -> bytes:
+    """PirServer::SaveToFile's byte layout (pir_server.cc:433-592) for the
+    combined material."""
+    cp = P.c()
+    f = lib().orc_save_server
+    f.restype = ctypes.c_size_t
+    pack = np.ascontiguousarray(mat.pack)
+    args = (ctypes.byref(cp), _p(_u64(mat.db)), ctypes.c_int(mat.db_cols),
+            pack.ctypes.data_as(ctypes.c_void_p), _p(_u64(mat.a_tilde)),
+            _p(_u64(mat.t_vec_h)), _p(_u64(mat.second_dim_a)))
+    n = f(*args, None)
+    out = np.zeros(n, dtype=np.uint8)
+    f(*args, _p(out, _u8p))
+    return out.tobytes()
+
+
+def load_server(P: Params, data: bytes) -> Material:
+    """PirServer::LoadFromBuffer (pir_server.cc:594-793) back into the combined
+    material."""
+    s = sizes(P)
+    db = np.zeros((s.M, s.db_words_per_row), dtype=np.uint64)
+    pack = _alloc_pack(P, s.M)
+    a_tilde = np.zeros((s.ell * s.t, s.d), dtype=np.uint64)
+    t_vec_h = np.zeros((s.ell * s.t, s.k, s.d), dtype=np.uint64)
+    sda = np.zeros((2 * s.e, s.d), dtype=np.uint64)
+    b = np.frombuffer(data, dtype=np.uint8)
+    cp = P.c()
+    _call(lib().orc_load_server, ctypes.byref(cp), _p(np.ascontiguousarray(b), _u8p),
+          ctypes.c_size_t(len(data)), _p(db), pack.ctypes.data_as(ctypes.c_void_p),
+          _p(a_tilde), _p(t_vec_h), _p(sda))
+    return Material(db, pack, a_tilde, t_vec_h, sda, s.K)

@@ -1290,6 +1290,109 @@ ClientState MakeClient(const Derived& D, uint64_t secret_seed) {
   return c;
 }
 
+// ---- Bit-packed formats ----------------------------------------------------
+// pir_server.cc:309-336 (BitPackSigned) and :369-399 (BitPackCoefficients),
+// polynomial.cc:562-590 (ToProto): bits of value i occupy stream bits
+// [i*bits, (i+1)*bits), LSB first within each byte. A signed value is first
+// reinterpreted as the unsigned type of its own width (value_bits).
+u64 FieldOf(u64 v, int is_signed, int value_bits, int bits) {
+  if (is_signed && value_bits < 64) v &= (1ULL << value_bits) - 1;
+  return bits >= 64 ? v : (v & ((1ULL << bits) - 1));
+}
+
+std::vector<uint8_t> BitPack(const u64* vals, size_t n, int bits, int is_signed,
+                             int value_bits) {
+  if (n == 0 || bits <= 0) return {};
+  // BitPackCoefficients copies whole values when bits >= the type width; the
+  // bitwise loop below produces the same little-endian bytes for 64.
+  const int b = bits > 64 ? 64 : bits;
+  std::vector<uint8_t> out((n * static_cast<size_t>(b) + 7) / 8, 0);
+  u64 pos = 0;
+  for (size_t i = 0; i < n; ++i) {
+    u64 v = FieldOf(vals[i], is_signed, value_bits, b);
+    int left = b;
+    while (left > 0) {
+      const size_t byte = pos / 8;
+      const int off = static_cast<int>(pos % 8);
+      const int take = std::min(left, 8 - off);
+      out[byte] |= static_cast<uint8_t>((v & ((1u << take) - 1)) << off);
+      v = take >= 64 ? 0 : (v >> take);
+      pos += take;
+      left -= take;
+    }
+  }
+  return out;
+}
+
+// pir_server.cc:338-367 (BitUnpackSigned), :401-429 (BitUnpackCoefficients).
+void BitUnpack(const uint8_t* data, size_t n, int bits, int is_signed, int value_bits,
+               u64* out) {
+  if (n == 0 || bits <= 0) {
+    for (size_t i = 0; i < n; ++i) out[i] = 0;
+    return;
+  }
+  const int b = bits > 64 ? 64 : bits;
+  u64 pos = 0;
+  for (size_t i = 0; i < n; ++i) {
+    u64 v = 0;
+    int left = b, shift = 0;
+    while (left > 0) {
+      const size_t byte = pos / 8;
+      const int off = static_cast<int>(pos % 8);
+      const int take = std::min(left, 8 - off);
+      const u64 chunk = (data[byte] >> off) & ((1u << take) - 1);
+      v |= chunk << shift;
+      shift += take;
+      pos += take;
+      left -= take;
+    }
+    if (is_signed && b < 64 && ((v >> (b - 1)) & 1)) v |= ~((1ULL << b) - 1);
+    if (value_bits < 64) v &= (1ULL << value_bits) - 1;  // static_cast to the value type
+    if (is_signed && value_bits < 64 && ((v >> (value_bits - 1)) & 1)) {
+      v |= ~((1ULL << value_bits) - 1);  // kept sign-extended in 64 bits
+    }
+    out[i] = v;
+  }
+}
+
+// Byte writer / bounded reader of the server image (pir_server.cc:433-793).
+struct ImageWriter {
+  uint8_t* out;  // null: count only
+  size_t n = 0;
+  void Put(const void* p, size_t len) {
+    if (out != nullptr && len > 0) std::memcpy(out + n, p, len);
+    n += len;
+  }
+  template <typename T>
+  void Val(T v) { Put(&v, sizeof(v)); }
+};
+
+// BufferReader of LoadFromBuffer: a read past the end leaves the offset and
+// marks the reader bad; the loader then fails only when the offset is not at
+// the end (pir_server.cc:778-780).
+struct ImageReader {
+  const uint8_t* buf;
+  size_t size;
+  size_t off = 0;
+  bool ok = true;
+  void Get(void* p, size_t len) {
+    if (off + len > size) {
+      ok = false;
+      return;
+    }
+    if (len > 0) std::memcpy(p, buf + off, len);
+    off += len;
+  }
+  template <typename T>
+  T Val() {
+    T v{};
+    Get(&v, sizeof(v));
+    return v;
+  }
+};
+
+int ImageLogModulus(const Derived& D) { return D.log_q >= 64 ? 64 : D.log_q; }
+
 }  // namespace
 
 // ===========================================================================
@@ -1974,6 +2077,194 @@ void orc_fill_mod(uint64_t seed, uint64_t* out, size_t n, uint64_t modulus) {
 void orc_fill_db_u16(uint64_t seed, uint16_t* out, size_t n, uint64_t p) {
   SplitMix64 g(seed);
   for (size_t i = 0; i < n; ++i) out[i] = static_cast<uint16_t>(g.Next8() % p);
+}
+
+
+// polynomial.cc:562-590.
+int orc_poly_to_proto(const uint64_t* coeffs, int len, int log_modulus,
+                      uint8_t* out, size_t out_cap, size_t* out_len, char* err,
+                      size_t errlen) {
+  ORC_GUARD_BEGIN
+  if (log_modulus <= 0 || log_modulus > 64) InvalidArg("Invalid log_modulus.");
+  std::vector<uint8_t> b = BitPack(coeffs, static_cast<size_t>(len), log_modulus, 0, 64);
+  if (b.size() > out_cap) InvalidArg("output buffer too small.");
+  if (!b.empty()) std::memcpy(out, b.data(), b.size());
+  *out_len = b.size();
+  ORC_GUARD_END(err, errlen)
+}
+
+// polynomial.cc:208-252.
+int orc_poly_from_proto(const uint8_t* bytes, size_t nbytes, int len,
+                        int log_modulus, uint64_t* out, char* err,
+                        size_t errlen) {
+  ORC_GUARD_BEGIN
+  if (len <= 0 || !IsPow2(static_cast<u64>(len))) InvalidArg("Invalid length.");
+  if (log_modulus <= 0 || log_modulus > 64) InvalidArg("Invalid log_modulus.");
+  if (nbytes == 0) InvalidArg("Empty serialized polynomial.");
+  const size_t expected = (static_cast<size_t>(len) * log_modulus + 7) / 8;
+  if (nbytes != expected) InvalidArg("Unexpected serialized size.");
+  BitUnpack(bytes, static_cast<size_t>(len), log_modulus, 0, 64, out);
+  ORC_GUARD_END(err, errlen)
+}
+
+size_t orc_bit_pack(const uint64_t* values, size_t n, int bits, int is_signed,
+                    int value_bits, uint8_t* out) {
+  std::vector<uint8_t> b = BitPack(values, n, bits, is_signed, value_bits);
+  if (out != nullptr && !b.empty()) std::memcpy(out, b.data(), b.size());
+  return b.size();
+}
+
+void orc_bit_unpack(const uint8_t* data, size_t n, int bits, int is_signed,
+                    int value_bits, uint64_t* out) {
+  BitUnpack(data, n, bits, is_signed, value_bits, out);
+}
+
+// pir_server.cc:433-592 on the combined material of a server built by
+// PirServer::Create: DB condensed (u16/u8), pack int32 condensed or int16
+// entries (not condensed: Matrix::CreateCondensed has no int16 form).
+size_t orc_save_server(const orc_params* P, const uint64_t* db_cdata,
+                       int db_cols, const void* pack, const uint64_t* a_tilde,
+                       const uint64_t* t_vec_h, const uint64_t* second_dim_a,
+                       uint8_t* out) {
+  const Derived D = Derive(P);
+  const int d = D.d, M = D.M, k = D.k, lt = D.ell * D.t;
+  ImageWriter w{out};
+  // 1. combined DB matrix (condensed words).
+  const int per = P->db_bytes == 2 ? 4 : 8;
+  const int32_t db_phys = (db_cols + per - 1) / per;
+  w.Val<int32_t>(M);
+  w.Val<int32_t>(db_phys);
+  w.Val<uint8_t>(1);
+  w.Val<int32_t>(db_cols);
+  const uint64_t db_elems = static_cast<uint64_t>(M) * db_phys;
+  w.Val<uint64_t>(db_elems);
+  w.Put(db_cdata, db_elems * 8);
+  // 2. combined pack matrix, bit-packed entries.
+  const int32_t pcols = k * d;
+  const bool i32 = P->mat_bytes == 4;
+  w.Val<int32_t>(M);
+  w.Val<int32_t>(pcols);
+  w.Val<int32_t>(i32 ? (pcols + 1) / 2 : pcols);
+  w.Val<uint8_t>(i32 ? 1 : 0);
+  w.Val<int32_t>(pcols);
+  const int32_t bits = P->pack_log_digit + BitWidth(static_cast<u64>(d - 1));
+  w.Val<int32_t>(bits);
+  std::vector<u64> entries(static_cast<size_t>(M) * pcols);
+  for (size_t i = 0; i < entries.size(); ++i) {
+    entries[i] = i32 ? static_cast<u64>(static_cast<i64>(static_cast<const int32_t*>(pack)[i]))
+                     : static_cast<u64>(static_cast<i64>(static_cast<const int16_t*>(pack)[i]));
+  }
+  std::vector<uint8_t> pk = BitPack(entries.data(), entries.size(), bits, 1, i32 ? 32 : 16);
+  w.Val<uint64_t>(pk.size());
+  w.Put(pk.data(), pk.size());
+  // 3. preprocessed outputs.
+  const int32_t logm = ImageLogModulus(D);
+  w.Val<int32_t>(logm);
+  w.Val<int32_t>(D.ell);
+  for (int sh = 0; sh < D.ell; ++sh) {
+    w.Val<int32_t>(D.t);
+    for (int i = 0; i < D.t; ++i) {
+      const int oi = sh * D.t + i;
+      w.Val<uint64_t>(d);
+      std::vector<uint8_t> a = BitPack(a_tilde + static_cast<size_t>(oi) * d, d, logm, 0, 64);
+      w.Val<uint64_t>(a.size());
+      w.Put(a.data(), a.size());
+      w.Val<uint64_t>(k);
+      for (int j = 0; j < k; ++j) {
+        w.Val<uint64_t>(d);
+        std::vector<uint8_t> t =
+            BitPack(t_vec_h + (static_cast<size_t>(oi) * k + j) * d, d, logm, 0, 64);
+        w.Val<uint64_t>(t.size());
+        w.Put(t.data(), t.size());
+      }
+    }
+  }
+  (void)lt;
+  // 4. second-dimension a components (none when t == 1).
+  const uint64_t nsda = D.t > 1 ? 2 * D.e : 0;
+  w.Val<uint64_t>(nsda);
+  for (uint64_t j = 0; j < nsda; ++j) {
+    w.Val<uint64_t>(d);
+    std::vector<uint8_t> b = BitPack(second_dim_a + j * d, d, logm, 0, 64);
+    w.Val<uint64_t>(b.size());
+    w.Put(b.data(), b.size());
+  }
+  return w.n;
+}
+
+// pir_server.cc:594-793, checked against P's shapes.
+int orc_load_server(const orc_params* P, const uint8_t* buf, size_t size,
+                    uint64_t* db_cdata, void* pack, uint64_t* a_tilde,
+                    uint64_t* t_vec_h, uint64_t* second_dim_a, char* err,
+                    size_t errlen) {
+  ORC_GUARD_BEGIN
+  const Derived D = Derive(P);
+  const int d = D.d, M = D.M, k = D.k;
+  ImageReader r{buf, size};
+  const int32_t db_rows = r.Val<int32_t>();
+  const int32_t db_phys = r.Val<int32_t>();
+  const uint8_t db_cond = r.Val<uint8_t>();
+  const int32_t db_orig = r.Val<int32_t>();
+  const uint64_t db_elems = r.Val<uint64_t>();
+  (void)db_cond;
+  (void)db_orig;
+  if (db_rows != M || db_elems != static_cast<uint64_t>(M) * db_phys) {
+    InvalidArg("DB matrix shape does not match the parameters.");
+  }
+  r.Get(db_cdata, db_elems * 8);
+  const int32_t prows = r.Val<int32_t>();
+  const int32_t pcols = r.Val<int32_t>();
+  (void)r.Val<int32_t>();
+  (void)r.Val<uint8_t>();
+  (void)r.Val<int32_t>();
+  const int32_t bits = r.Val<int32_t>();
+  const uint64_t plen = r.Val<uint64_t>();
+  if (prows != M || pcols != k * d) InvalidArg("pack matrix shape does not match the parameters.");
+  const size_t nent = static_cast<size_t>(prows) * pcols;
+  if (plen < (nent * static_cast<size_t>(bits) + 7) / 8 || r.off + plen > size) {
+    InvalidArg("pack matrix bytes are short.");
+  }
+  std::vector<u64> ent(nent);
+  BitUnpack(buf + r.off, nent, bits, 1, P->mat_bytes == 4 ? 32 : 16, ent.data());
+  r.off += plen;
+  for (size_t i = 0; i < nent; ++i) {
+    if (P->mat_bytes == 4) {
+      static_cast<int32_t*>(pack)[i] = static_cast<int32_t>(ent[i]);
+    } else {
+      static_cast<int16_t*>(pack)[i] = static_cast<int16_t>(ent[i]);
+    }
+  }
+  const int32_t logm = r.Val<int32_t>();
+  const int32_t nsh = r.Val<int32_t>();
+  if (nsh != D.ell) InvalidArg("preprocessed_outputs size mismatch.");
+  auto poly = [&](uint64_t* dst) {
+    const uint64_t len = r.Val<uint64_t>();
+    const uint64_t nb = r.Val<uint64_t>();
+    if (len != static_cast<uint64_t>(d) || r.off + nb > size ||
+        nb < (len * static_cast<uint64_t>(logm) + 7) / 8) {
+      InvalidArg("polynomial record does not match the parameters.");
+    }
+    BitUnpack(buf + r.off, len, logm, 0, 64, dst);
+    r.off += nb;
+  };
+  for (int sh = 0; sh < nsh; ++sh) {
+    const int32_t ns = r.Val<int32_t>();
+    if (ns != D.t) InvalidArg("Preprocessed outputs must be of length t.");
+    for (int i = 0; i < ns; ++i) {
+      const int oi = sh * D.t + i;
+      poly(a_tilde + static_cast<size_t>(oi) * d);
+      const uint64_t tv = r.Val<uint64_t>();
+      if (tv != static_cast<uint64_t>(k)) InvalidArg("t_vec_h must hold k polynomials.");
+      for (int j = 0; j < k; ++j) poly(t_vec_h + (static_cast<size_t>(oi) * k + j) * d);
+    }
+  }
+  const uint64_t nsda = r.Val<uint64_t>();
+  if (nsda != static_cast<uint64_t>(D.t > 1 ? 2 * D.e : 0)) {
+    InvalidArg("second_dim_query_a must have length 2 * PolyEvalGadgetParams().num_digits.");
+  }
+  for (uint64_t j = 0; j < nsda; ++j) poly(second_dim_a + j * d);
+  if (!r.ok && r.off < size) Fail(13, "Error reading from buffer");
+  ORC_GUARD_END(err, errlen)
 }
 
 }  // extern "C"

@@ -217,6 +217,41 @@ int orc_client_decrypt(const orc_params* P, uint64_t secret_seed,
 int orc_client_secret_key(const orc_params* P, uint64_t secret_seed,
                           uint64_t* out);
 
+/* ---- Bit-packed formats ----
+ * Polynomial::ToProto / CreateFromProto (crypto/polynomial.cc:562-590,
+ * 208-252): the low log_modulus bits of each coefficient, LSB first, one byte
+ * string per polynomial. BitPackCoefficients / BitUnpackCoefficients and
+ * BitPackSigned / BitUnpackSigned (crypto/pir/server/pir_server.cc:309-429)
+ * use the same bit order; the signed unpack sign-extends. */
+int orc_poly_to_proto(const uint64_t* coeffs, int len, int log_modulus,
+                      uint8_t* out, size_t out_cap, size_t* out_len, char* err,
+                      size_t errlen);
+int orc_poly_from_proto(const uint8_t* bytes, size_t nbytes, int len,
+                        int log_modulus, uint64_t* out, char* err,
+                        size_t errlen);
+/* values: n entries; is_signed selects BitPackSigned (entries are the
+ * sign-extended values of a type of value_bits bits) or BitPackCoefficients
+ * (value_bits = 64 for uint64 coefficients). Returns the byte count. */
+size_t orc_bit_pack(const uint64_t* values, size_t n, int bits, int is_signed,
+                    int value_bits, uint8_t* out);
+void orc_bit_unpack(const uint8_t* data, size_t n, int bits, int is_signed,
+                    int value_bits, uint64_t* out);
+
+/* PirServer::SaveToFile's byte layout (pir_server.cc:433-592) for the combined
+ * material (db: M x db_words condensed words; pack: int32 condensed words or
+ * int16 entries; polys as in orc_process_response). out == NULL returns the
+ * size only. */
+size_t orc_save_server(const orc_params* P, const uint64_t* db_cdata,
+                       int db_cols, const void* pack, const uint64_t* a_tilde,
+                       const uint64_t* t_vec_h, const uint64_t* second_dim_a,
+                       uint8_t* out);
+/* PirServer::LoadFromBuffer (pir_server.cc:594-793) back into the combined
+ * material; fails on a layout that does not match P. */
+int orc_load_server(const orc_params* P, const uint8_t* buf, size_t size,
+                    uint64_t* db_cdata, void* pack, uint64_t* a_tilde,
+                    uint64_t* t_vec_h, uint64_t* second_dim_a, char* err,
+                    size_t errlen);
+
 /* ---- Seeded generators (documented SplitMix64 stream) ---- */
 void orc_fill_u64(uint64_t seed, uint64_t* out, size_t n, uint64_t mask);
 void orc_fill_mod(uint64_t seed, uint64_t* out, size_t n, uint64_t modulus);

@@ -143,3 +143,38 @@ def test_fft_twiddles_match_oracle(n):
         for got, want in [(a[2 * i], math.cos(ang)), (a[2 * i + 1], math.sin(ang)),
                           (b[2 * i], math.cos(-ang)), (b[2 * i + 1], math.sin(-ang))]:
             assert abs(got - want) <= 2 * math.ulp(max(abs(want), 1e-300))
+
+
+@pytest.mark.parametrize("length,logm,nbytes,msg", [
+    (3, 8, 3, "Invalid length."),
+    (4, 0, 4, "Invalid log_modulus."),
+    (4, 8, 0, "Empty serialized polynomial."),
+    (4, 8, 5, "Unexpected serialized size."),
+])
+def test_poly_from_proto_checks_before_the_device(length, logm, nbytes, msg):
+    # CreateFromProto's messages (crypto/polynomial.cc:211-226), no GPU needed.
+    with pytest.raises(ia.PirError) as ei:
+        ia.poly_from_proto([bytes(nbytes)], length, logm)
+    assert ei.value.code == ia.INVALID_ARGUMENT and ei.value.message == msg
+
+
+def test_poly_to_proto_rejects_bad_log_modulus():
+    with pytest.raises(ia.PirError) as ei:
+        ia.poly_to_proto(np.zeros((1, 4), np.uint64), 0)
+    assert ei.value.message == "Invalid log_modulus."
+
+
+def test_load_from_buffer_reads_and_checks_before_the_device():
+    P = O.metrics_tool_params(n=2048, log_d=8, t=2)
+    img = O.save_server(P, O.synthetic_material(P, seed=1))
+    gp = to_gpu_params(P)
+    # Cut inside the pack record: "Error reading from buffer" (INTERNAL), as
+    # LoadFromBuffer returns when it stops short of the end.
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.load_from_buffer(gp, img[:len(img) // 2])
+    assert ei.value.code == 13 and ei.value.message == "Error reading from buffer"
+    # An image of other parameters.
+    P2 = O.metrics_tool_params(n=2048, log_d=8, t=4)
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.load_from_buffer(to_gpu_params(P2), img)
+    assert ei.value.code == ia.INVALID_ARGUMENT
