@@ -202,6 +202,132 @@ def run_serving(rates, seconds: float, seed: int = 5):
     return out
 
 
+def server_image(P: ia.PirParams, mat) -> bytes:
+    """PirServer::SaveToFile's byte image (crypto/pir/server/pir_server.cc:433-592)
+    of the bench material, with the bit-packed records made by the GPU encoder
+    (pirg_poly_to_proto: the same LSB-first layout as BitPackCoefficients, and
+    as BitPackSigned once an entry is cut to its low bits)."""
+    d, t, ell = P.degree, P.interp_degree, P.entry_size_multiple
+    k, e = P.pack_num_digits, P.eval_num_digits
+    M, K = ell * t * d, mat["db_cols"]
+    logm = min(P.log_q, 64)
+    i32 = np.int32
+    parts = [np.array([M, K // 4], dtype=i32).tobytes(), b"\x01",
+             np.array([K], dtype=i32).tobytes(), np.array([M * (K // 4)], np.uint64).tobytes(),
+             mat["db"].reshape(-1).view(np.uint8)]
+    bits = P.pack_log_digit + (d - 1).bit_length()
+    entries = mat["pack"].view(np.int32).reshape(-1)
+    fields = (entries.astype(np.int64).view(np.uint64) & np.uint64((1 << bits) - 1))
+    packed = ia.poly_to_proto(fields.reshape(1, -1), bits)[0]
+    del fields
+    parts += [np.array([M, k * d, k * d // 2], dtype=i32).tobytes(), b"\x01",
+              np.array([k * d, bits], dtype=i32).tobytes(),
+              np.array([len(packed)], np.uint64).tobytes(), packed,
+              np.array([logm, ell], dtype=i32).tobytes()]
+    enc_a = ia.poly_to_proto(mat["a_tilde"], logm)
+    enc_t = ia.poly_to_proto(mat["t_vec_h"].reshape(-1, d), logm)
+    rec = lambda b: np.array([d, len(b)], np.uint64).tobytes() + b  # noqa: E731
+    for sh in range(ell):
+        parts.append(np.array([t], dtype=i32).tobytes())
+        for i in range(t):
+            oi = sh * t + i
+            parts += [rec(enc_a[oi]), np.array([k], np.uint64).tobytes()]
+            parts += [rec(enc_t[oi * k + j]) for j in range(k)]
+    sda = ia.poly_to_proto(mat["second_dim_a"], logm) if t > 1 else []
+    parts.append(np.array([len(sda)], np.uint64).tobytes())
+    parts += [rec(b) for b in sda]
+    return b"".join(parts)
+
+
+def run_load(P, mat, req, want, cpu: bool):
+    """LoadFromBuffer of the full server image: host bytes -> resident HBM
+    layouts (pack matrix and polynomials unpacked on the GPU)."""
+    t0 = time.perf_counter()
+    image = server_image(P, mat)
+    log(f"server image: {len(image) / 2**30:.2f} GiB ({time.perf_counter() - t0:.1f}s to write)")
+    best = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        srv = ia.PirServer.load_from_buffer(P, image)
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(srv.process_response(req).ciphertexts, want))
+        srv.close()
+        best = dt if best is None else min(best, dt)
+    out = {"workload": "PirServer::LoadFromBuffer of the n=2^20 d=4096 t=8 image",
+           "image_bytes": len(image), "load_s": best,
+           "load_gbs_incl_pcie": len(image) / best / 1e9, "answers_identical": same}
+    if cpu:
+        import oracle as O
+        OP = O.Params(**{f: getattr(P, f) for f in O.Params.__dataclass_fields__})
+        t0 = time.perf_counter()
+        back = O.load_server(OP, image)
+        ct = time.perf_counter() - t0
+        del back
+        out["cpu_baseline"] = {"value": len(image) / ct / 1e9, "unit": "GB/s", "cores": 1,
+                               "kind": "port", "seconds": ct,
+                               "sample": "the same image through the oracle's LoadFromBuffer, "
+                                         "one thread"}
+    del image
+    return out
+
+
+def run_codec(steps: int = 20, cpu: bool = True):
+    """Polynomial::ToProto / CreateFromProto of 256 first-dimension queries
+    (131072 coefficients at 52 bits each) on device memory."""
+    from inspire_amd.hiprt import DeviceBuffer, Event, synchronize
+    nq, length, bits = 256, 131072, 52
+    n = nq * length
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 1 << bits, size=n, dtype=np.uint64)
+    pitch = (length * bits + 63) // 64
+    with DeviceBuffer(n * 8) as dx, DeviceBuffer(nq * pitch * 8) as dw, \
+            DeviceBuffer(n * 8) as dy:
+        dx.upload(x)
+        ia.poly_to_proto_device(dx.addr, nq, length, bits, dw.addr, pitch)
+        ia.poly_from_proto_device(dw.addr, pitch, nq, length, bits, dy.addr)
+        synchronize()
+        ev = [Event() for _ in range(3)]
+        ev[0].record()
+        for _ in range(steps):
+            ia.poly_to_proto_device(dx.addr, nq, length, bits, dw.addr, pitch)
+        ev[1].record()
+        for _ in range(steps):
+            ia.poly_from_proto_device(dw.addr, pitch, nq, length, bits, dy.addr)
+        ev[2].record()
+        enc_ms = ev[0].elapsed_ms(ev[1]) / steps
+        dec_ms = ev[1].elapsed_ms(ev[2]) / steps
+        for e in ev:
+            e.close()
+        back = dy.download(np.uint64, n)
+        words = dw.download(np.uint64, nq * pitch).reshape(nq, pitch)
+    exact = bool(np.array_equal(back, x))
+    nbytes = n * 8 + nq * pitch * 8
+    out = {"workload": "256 first-dimension queries, 131072 coefficients at 52 bits",
+           "encode_ms": enc_ms, "decode_ms": dec_ms,
+           "encode_gbs": nbytes / (enc_ms * 1e-3) / 1e9,
+           "decode_gbs": nbytes / (dec_ms * 1e-3) / 1e9,
+           "roofline": {"bound": "hbm", "achieved": nbytes / (dec_ms * 1e-3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": nbytes / (dec_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "traffic": None},
+           "round_trip_exact": exact}
+    if cpu:
+        import oracle as O
+        enc0 = words[0].tobytes()[:(length * bits + 7) // 8]
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 2.0:
+            O.poly_from_proto(enc0, length, bits)
+            reps += 1
+        ct = (time.perf_counter() - t0) / reps
+        out["bit_identical_to_oracle"] = bool(O.poly_to_proto(x[:length], bits) == enc0)
+        out["cpu_baseline"] = {"value": (length * 8 + len(enc0)) / ct / 1e9, "unit": "GB/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "CreateFromProto of one 131072-coefficient query "
+                                         "through the oracle, repeated for 2 s"}
+    return out
+
+
 def run_packing(njobs: int, seed: int = 11, verify: int = 2):
     """MatrixPack for njobs jobs at d = 2048, q = 2^56, one graph launch."""
     log_d, k = 11, 2
@@ -282,6 +408,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-pack", action="store_true")
+    ap.add_argument("--no-load", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=3)
     ap.add_argument("--serve-rates", default="100,500,1000,2000,5000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -316,6 +443,7 @@ def main():
             srv.batch_stage(reqs[:B])
             for _ in range(2):
                 srv.batch_run(B)
+        run_codec(steps=2, cpu=False)
         copy_ms = ia.copy_probe(2 << 30, repeats=3)
         read_ms = ia.read_probe(4 << 30, repeats=3)
         print(json.dumps({"profile": True, "copy_bytes": 2 << 30, "copy_ms": copy_ms,
@@ -373,8 +501,17 @@ def main():
                          f"through the oracle ProcessResponse, one thread (last run's stage "
                          f"ms: db {stages[0]:.0f}, pack {stages[1]:.0f}, eval {stages[2]:.0f})",
                "bit_identical": bool(np.array_equal(cpu_out, gpu_out))}
+    want0 = srv.process_response(reqs[0]).ciphertexts
     srv.close()
+    load = None
+    if not args.no_load:
+        load = run_load(P, mat, reqs[0], want0, cpu=not args.no_cpu)
+        log(f"load from image: {load['load_s']:.2f} s ({load['load_gbs_incl_pcie']:.1f} GB/s "
+            f"incl. PCIe), answers identical {load['answers_identical']}")
     del mat
+    codec = run_codec(cpu=not args.no_cpu)
+    log(f"codec: encode {codec['encode_gbs']:.0f} GB/s, decode {codec['decode_gbs']:.0f} GB/s, "
+        f"exact {codec['round_trip_exact']}")
 
     serving = None
     if not args.no_serve:
@@ -415,6 +552,8 @@ def main():
         "autotune": tune_report.strip().split("\n"),
         "serving": serving,
         "packing": packing,
+        "load": load,
+        "codec": codec,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
