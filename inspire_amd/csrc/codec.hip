@@ -7,10 +7,11 @@
 //
 // On the device a stream is a run of 64-bit words (little-endian bytes, so
 // bit j of the stream is bit j%64 of word j/64). Several equal streams (one
-// per polynomial) sit at a fixed word pitch. Unpacking reads the one or two
-// words a value straddles; packing gives each output word to one thread,
-// which ORs in the (at most 64) values that overlap it. Both are bound by
-// HBM bandwidth: 8 bytes per value on one side, bits/8 on the other.
+// per polynomial) sit at a fixed word pitch. A workgroup owns a tile of 2048
+// values: it stages the tile's words (unpacking) or values (packing) in LDS
+// with coalesced loads, then each value reads the one or two words it
+// straddles, or each output word ORs in the values that overlap it. Both are
+// bound by HBM bandwidth: 8 bytes per value on one side, bits/8 on the other.
 #include <cstring>
 #include <vector>
 
@@ -24,66 +25,94 @@ __device__ __forceinline__ u64 FieldMask(int bits) {
   return bits >= 64 ? ~0ULL : ((1ULL << bits) - 1);
 }
 
+// Tiles of kTile values per workgroup (kTile / 256 per thread). A stream's
+// tile t covers values [t*kTile, (t+1)*kTile) and, when the stream starts at
+// bit 0, exactly the words [t*kTile*bits/64, (t+1)*kTile*bits/64).
+constexpr int kTile = 2048;
+constexpr int kTileThreads = 256;
+
 // out[p][i] = value i of stream p (streams at `pitch` words, stream p's
 // values from bit bit0 on). out_bytes: 8 (uint64), 4 (int32), 2 (int16).
-// A value reads a second word only when its bits reach into it, so no word
-// past the stream's last bit is touched.
-__global__ void BitUnpackKernel(const u64* __restrict__ words, long long pitch, long long bit0,
-                                int nstreams, long long len, int bits, int sign, int out_bytes,
-                                void* __restrict__ out) {
-  const long long total = static_cast<long long>(nstreams) * len;
+// The tile's words are staged in LDS with coalesced loads; each value then
+// reads its one or two words there. A value reads a second word only when
+// its bits reach into it, so no word past the stream's last bit is touched.
+// Grid: (tiles of a stream, streams).
+__global__ __launch_bounds__(kTileThreads) void BitUnpackKernel(
+    const u64* __restrict__ words, long long pitch, long long bit0, long long len, int bits,
+    int sign, int out_bytes, void* __restrict__ out) {
+  __shared__ u64 w[kTile + 2];
+  const long long p = blockIdx.y;
+  const long long v0 = static_cast<long long>(blockIdx.x) * kTile;
+  const int cnt = static_cast<int>(len - v0 < kTile ? len - v0 : kTile);
+  const unsigned long long first_bit = static_cast<unsigned long long>(bit0) +
+                                       static_cast<unsigned long long>(v0) * bits;
+  const unsigned long long last_bit = first_bit + static_cast<unsigned long long>(cnt) * bits - 1;
+  const long long wa = static_cast<long long>(first_bit >> 6);
+  const int nw = static_cast<int>((last_bit >> 6) - (first_bit >> 6)) + 1;
+  const u64* src = words + p * pitch + wa;
+  for (int i = threadIdx.x; i < nw; i += kTileThreads) w[i] = src[i];
+  __syncthreads();
   const u64 mask = FieldMask(bits);
-  for (long long g = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; g < total;
-       g += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const long long p = g / len;
-    const long long i = g - p * len;
-    const unsigned long long pos = static_cast<unsigned long long>(bit0) +
-                                   static_cast<unsigned long long>(i) * bits;
-    const u64* w = words + p * pitch + static_cast<long long>(pos >> 6);
-    const int s = static_cast<int>(pos & 63);
-    u64 v = w[0] >> s;
-    if (s + bits > 64) v |= w[1] << (64 - s);
-    v &= mask;
-    if (sign && bits < 64 && ((v >> (bits - 1)) & 1ULL)) v |= ~mask;
+  const int base = static_cast<int>(first_bit & 63);
+  const long long o = p * len + v0;
+  for (int v = threadIdx.x; v < cnt; v += kTileThreads) {
+    const unsigned pos = static_cast<unsigned>(base) + static_cast<unsigned>(v) * bits;
+    const int s = pos & 63;
+    const int j = pos >> 6;
+    u64 x = w[j] >> s;
+    if (s + bits > 64) x |= w[j + 1] << (64 - s);
+    x &= mask;
+    if (sign && bits < 64 && ((x >> (bits - 1)) & 1ULL)) x |= ~mask;
     if (out_bytes == 8) {
-      static_cast<u64*>(out)[g] = v;
+      static_cast<u64*>(out)[o + v] = x;
     } else if (out_bytes == 4) {
-      static_cast<int32_t*>(out)[g] = static_cast<int32_t>(v);
+      static_cast<int32_t*>(out)[o + v] = static_cast<int32_t>(x);
     } else {
-      static_cast<int16_t*>(out)[g] = static_cast<int16_t>(v);
+      static_cast<int16_t*>(out)[o + v] = static_cast<int16_t>(x);
     }
   }
 }
 
-// words[p][w] = bits [64w, 64w + 64) of stream p, the low `bits` bits of
-// vals[p][0..len) packed from bit 0; words past the stream's end are zero.
-__global__ void BitPackKernel(const u64* __restrict__ vals, int nstreams, long long len,
-                              int bits, u64* __restrict__ words, long long pitch) {
-  const long long total = static_cast<long long>(nstreams) * pitch;
+// words[p][w] = bits [64w, 64w + 64) of stream p: the low `bits` bits of
+// vals[p][0..len) packed from bit 0; words from the stream's end up to the
+// pitch are written as zero. The tile's values are staged in LDS with
+// coalesced loads; each thread then ORs together the values overlapping
+// each of its output words. Grid: (tiles of a stream, streams).
+__global__ __launch_bounds__(kTileThreads) void BitPackKernel(
+    const u64* __restrict__ vals, long long len, int bits, u64* __restrict__ words,
+    long long pitch) {
+  __shared__ u64 v[kTile];
+  const long long p = blockIdx.y;
+  const long long v0 = static_cast<long long>(blockIdx.x) * kTile;
+  const int cnt = static_cast<int>(len - v0 < kTile ? len - v0 : kTile);
+  const u64* src = vals + p * len + v0;
   const u64 mask = FieldMask(bits);
-  for (long long g = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; g < total;
-       g += static_cast<long long>(gridDim.x) * blockDim.x) {
-    const long long p = g / pitch;
-    const long long w = g - p * pitch;
-    const long long lo = w * 64;
-    const long long i0 = lo / bits;
-    long long i1 = (lo + 63) / bits;
-    if (i1 > len - 1) i1 = len - 1;
-    const u64* src = vals + p * len;
+  for (int i = threadIdx.x; i < cnt; i += kTileThreads) v[i] = src[i] & mask;
+  __syncthreads();
+  // This tile's words: [v0*bits/64, ceil((v0 + cnt)*bits/64)); kTile*bits is
+  // a multiple of 64, so tiles own disjoint whole words.
+  const long long w0 = v0 * bits / 64;
+  const int nw = static_cast<int>((static_cast<long long>(cnt) * bits + 63) / 64);
+  u64* dst = words + p * pitch + w0;
+  for (int k = threadIdx.x; k < nw; k += kTileThreads) {
+    const int lo = k * 64;
+    const int i0 = lo / bits;
+    int i1 = (lo + 63) / bits;
+    if (i1 > cnt - 1) i1 = cnt - 1;
     u64 acc = 0;
-    for (long long i = i0; i <= i1; ++i) {
-      const u64 v = src[i] & mask;
-      const long long sh = i * bits - lo;  // in (-bits, 64)
-      acc |= sh >= 0 ? (v << sh) : (v >> (-sh));
+    for (int i = i0; i <= i1; ++i) {
+      const int sh = i * bits - lo;  // in (-bits, 64)
+      acc |= sh >= 0 ? (v[i] << sh) : (v[i] >> (-sh));
     }
-    words[g] = acc;
+    dst[k] = acc;
   }
-}
-
-int CodecBlocks(long long total) {
-  long long b = (total + 255) / 256;
-  if (b > 65535LL * 16) b = 65535LL * 16;
-  return static_cast<int>(b < 1 ? 1 : b);
+  // Words past the stream, up to the pitch: zero (written by the last tile).
+  if (v0 + cnt == len) {
+    const long long used = (len * bits + 63) / 64;
+    for (long long k = used + threadIdx.x; k < pitch; k += kTileThreads) {
+      words[p * pitch + k] = 0;
+    }
+  }
 }
 
 }  // namespace
@@ -92,11 +121,14 @@ hipError_t LaunchBitUnpack(const u64* words, long long pitch, long long bit0, in
                            long long len, int bits, bool sign, int out_bytes, void* out,
                            hipStream_t s) {
   if (nstreams <= 0 || len <= 0) return hipSuccess;
-  if (bits < 1 || bits > 64 || (out_bytes != 8 && out_bytes != 4 && out_bytes != 2)) {
+  if (bits < 1 || bits > 64 || (out_bytes != 8 && out_bytes != 4 && out_bytes != 2) ||
+      nstreams > 65535) {
     return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(BitUnpackKernel, dim3(CodecBlocks(static_cast<long long>(nstreams) * len)),
-                     dim3(256), 0, s, words, pitch, bit0, nstreams, len, bits, sign ? 1 : 0,
+  const long long tiles = (len + kTile - 1) / kTile;
+  if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(BitUnpackKernel, dim3(static_cast<unsigned>(tiles), nstreams),
+                     dim3(kTileThreads), 0, s, words, pitch, bit0, len, bits, sign ? 1 : 0,
                      out_bytes, out);
   return hipGetLastError();
 }
@@ -104,9 +136,13 @@ hipError_t LaunchBitUnpack(const u64* words, long long pitch, long long bit0, in
 hipError_t LaunchBitPack(const u64* vals, int nstreams, long long len, int bits, u64* words,
                          long long pitch, hipStream_t s) {
   if (nstreams <= 0 || len <= 0) return hipSuccess;
-  if (bits < 1 || bits > 64 || pitch * 64 < len * bits) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(BitPackKernel, dim3(CodecBlocks(static_cast<long long>(nstreams) * pitch)),
-                     dim3(256), 0, s, vals, nstreams, len, bits, words, pitch);
+  if (bits < 1 || bits > 64 || pitch * 64 < len * bits || nstreams > 65535) {
+    return hipErrorInvalidValue;
+  }
+  const long long tiles = (len + kTile - 1) / kTile;
+  if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(BitPackKernel, dim3(static_cast<unsigned>(tiles), nstreams),
+                     dim3(kTileThreads), 0, s, vals, len, bits, words, pitch);
   return hipGetLastError();
 }
 
