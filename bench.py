@@ -309,7 +309,7 @@ def run_codec(steps: int = 20, cpu: bool = True):
            "roofline": {"bound": "hbm", "achieved": nbytes / (dec_ms * 1e-3) / 1e9,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": nbytes / (dec_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                        "traffic": None},
+                        "traffic": load_traffic().get("codec_decode")},
            "round_trip_exact": exact}
     if cpu:
         import oracle as O

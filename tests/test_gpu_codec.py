@@ -113,3 +113,16 @@ def test_decryptable_answer_from_loaded_image(name):
     msg = O.client_decrypt(P, 700 + idx, got)
     np.testing.assert_array_equal(msg.astype(np.uint16),
                                   db[idx * s.d * s.ell:(idx + 1) * s.d * s.ell])
+
+
+def test_bench_image_writer_matches_oracle_savetofile():
+    # The bench writes the full-size image with the GPU encoder; at a small
+    # size its bytes must equal the oracle's SaveToFile restatement.
+    import bench
+    P = ia.metrics_tool_params(n=4096, log_d=9, t=2)
+    mat = bench.synthetic_material(P, seed=3)
+    img = bench.server_image(P, mat)
+    OP = O.Params(**{f: getattr(P, f) for f in O.Params.__dataclass_fields__})
+    om = O.Material(mat["db"], mat["pack"], mat["a_tilde"], mat["t_vec_h"],
+                    mat["second_dim_a"], mat["db_cols"])
+    assert img == O.save_server(OP, om)
