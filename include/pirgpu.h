@@ -327,6 +327,60 @@ int pirg_poly_from_proto_device(const uint64_t* d_words, long long pitch_words, 
 int pirg_server_load_from_buffer(const pirg_params* params, const uint8_t* buffer,
                                  size_t size, pirg_server** out, char* err, size_t errlen);
 
+/* ---- vPIR online path (crypto/vpir/server/vpir_server.h:47-121) ----
+ * VpirServer's answer is PirServer's with one interpolation slice (t = 1)
+ * and no modulus switch: b = FinalizeMatrixPack(DB*q, pack*y_g, y_h) per
+ * shard, or LowBits(DB*q + pack*y_g) with local finalize. The proof (z-PIR)
+ * material is offline and not part of this path. */
+typedef struct pirg_vpir_server pirg_vpir_server;
+
+/* VpirServer::Create (vpir_server.cc:45-99) on the online material: db
+ * (ell*d rows x N columns, condensed words, crypto/matrix.cc:76-160), the
+ * combined pack matrix (ell*d x k*d: int32 condensed words or int16 entries),
+ * and the num_outputs = ell preprocessed outputs' a_tilde_agg (ell x d) and
+ * t_vec_h (ell x k x d). params: interp_degree must be 1; log_q1, log_q2 and
+ * the eval gadget are unused. local_finalize: VpirParams::LocalFinalize.
+ * Errors (vpir_server.cc:58-69): "DB matrix must have ell * d rows.",
+ * "DB matrix must have N columns.", "preprocessed_outputs size mismatch." */
+int pirg_vpir_server_create(const pirg_params* params, int local_finalize,
+                            const uint64_t* db_cdata, int db_rows, int db_cols,
+                            const void* pack, int num_outputs, const uint64_t* a_tilde,
+                            const uint64_t* t_vec_h, pirg_vpir_server** out, char* err,
+                            size_t errlen);
+void pirg_vpir_server_destroy(pirg_vpir_server* server);
+
+/* VpirRequest (crypto/vpir/vpir_params.h:160-164): first_dimension_query as
+ * first_dim_polys polynomials of d coefficients (contiguous), packing_key as
+ * packing_key_polys polynomials (k = y_vec_g, then k of y_vec_h unless local
+ * finalize). */
+typedef struct {
+  const uint64_t* first_dim;
+  size_t first_dim_polys;
+  const uint64_t* packing_key;
+  size_t packing_key_polys;
+} pirg_vpir_request_view;
+
+/* VpirServer::ProcessResponse (vpir_server.cc:101-195): out receives the ell
+ * b_responses (ell x d). Errors, in the reference's order: "Packing key must
+ * contain <n> polynomials.", "Flattened first dimension query must match DB
+ * matrix columns (N)." */
+int pirg_vpir_process(pirg_vpir_server* server, const pirg_vpir_request_view* req,
+                      uint64_t* out, char* err, size_t errlen);
+/* Many requests in one batched pass (per-request statuses, as
+ * pirg_process_batch). No reference counterpart. */
+int pirg_vpir_process_batch(pirg_vpir_server* server, int nreq,
+                            const pirg_vpir_request_view* reqs, uint64_t* const* outs,
+                            int* statuses, char* err, size_t errlen);
+/* Device-resident batches (as pirg_batch_stage/run/fetch) and the scan
+ * autotuner (as pirg_server_autotune) of the underlying server. */
+int pirg_vpir_batch_stage(pirg_vpir_server* server, int nreq,
+                          const pirg_vpir_request_view* reqs, char* err, size_t errlen);
+int pirg_vpir_batch_run(pirg_vpir_server* server, int nreq, char* err, size_t errlen);
+int pirg_vpir_batch_fetch(pirg_vpir_server* server, int nreq, uint64_t* const* outs,
+                          char* err, size_t errlen);
+int pirg_vpir_autotune(pirg_vpir_server* server, const int* widths, int nwidths,
+                       char* report, size_t report_len, char* err, size_t errlen);
+
 const char* pirg_version(void);
 
 #ifdef __cplusplus

@@ -56,6 +56,7 @@ def lib() -> ctypes.CDLL:
         _lib.pirg_server_fft_margin.restype = ctypes.c_double
         _lib.pirg_server_destroy.restype = None
         _lib.pirg_packer_destroy.restype = None
+        _lib.pirg_vpir_server_destroy.restype = None
         _lib.pirg_serve_submit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                            ctypes.c_char_p, ctypes.c_size_t]
         _lib.pirg_serve_poll.argtypes = [ctypes.c_void_p, ctypes.c_int, _u64p, _u64p,
@@ -534,6 +535,117 @@ class PirPacker:
         lib().pirg_packer_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n),
                                 int(reset))
         return a.value, b.value, n.value
+
+
+class _CVpirRequest(ctypes.Structure):
+    _fields_ = [("first_dim", ctypes.c_void_p), ("first_dim_polys", ctypes.c_size_t),
+                ("packing_key", ctypes.c_void_p), ("packing_key_polys", ctypes.c_size_t)]
+
+
+class VpirServer:
+    """VpirServer's online path (crypto/vpir/server/vpir_server.h:47-121) over
+    libpirgpu (pirg_vpir_*)."""
+
+    def __init__(self, handle, params: PirParams):
+        self._h = handle
+        self.params = params
+
+    @classmethod
+    def create(cls, params: PirParams, local_finalize: bool, db: np.ndarray, db_rows: int,
+               db_cols: int, pack: np.ndarray, a_tilde: np.ndarray,
+               t_vec_h: np.ndarray) -> "VpirServer":
+        db = _u64(db)
+        pack = np.ascontiguousarray(pack)
+        a_tilde = _u64(a_tilde)
+        t_vec_h = _u64(t_vec_h)
+        handle = ctypes.c_void_p()
+        err = _err()
+        cp = params.c()
+        rc = lib().pirg_vpir_server_create(
+            ctypes.byref(cp), int(local_finalize), db.ctypes.data_as(_u64p), db_rows, db_cols,
+            pack.ctypes.data_as(ctypes.c_void_p), a_tilde.shape[0] if a_tilde.ndim > 1 else 1,
+            a_tilde.ctypes.data_as(_u64p), t_vec_h.ctypes.data_as(_u64p), ctypes.byref(handle),
+            err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return cls(handle, params)
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().pirg_vpir_server_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _views(self, requests):
+        d = self.params.degree
+        keep, views = [], (_CVpirRequest * max(len(requests), 1))()
+        for i, (fd, pk) in enumerate(requests):
+            fd = _u64(fd).reshape(-1)
+            pk = _u64(pk).reshape(-1)
+            keep += [fd, pk]
+            views[i] = _CVpirRequest(fd.ctypes.data, fd.size // d, pk.ctypes.data, pk.size // d)
+        return keep, views
+
+    def process_batch(self, requests):
+        """[(first_dim, packing_key)] -> ([ell x d b arrays], statuses, message)."""
+        p = self.params
+        n = len(requests)
+        keep, views = self._views(requests)
+        outs = [np.zeros((p.entry_size_multiple, p.degree), dtype=np.uint64) for _ in range(n)]
+        ptrs = (_u64p * max(n, 1))(*[o.ctypes.data_as(_u64p) for o in outs])
+        st = (ctypes.c_int * max(n, 1))()
+        err = _err()
+        rc = lib().pirg_vpir_process_batch(self._h, n, views, ptrs, st, err,
+                                           ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return outs, list(st)[:n], err.value.decode()
+
+    def process_response(self, first_dim, packing_key) -> np.ndarray:
+        keep, views = self._views([(first_dim, packing_key)])
+        p = self.params
+        out = np.zeros((p.entry_size_multiple, p.degree), dtype=np.uint64)
+        err = _err()
+        rc = lib().pirg_vpir_process(self._h, views, out.ctypes.data_as(_u64p), err,
+                                     ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return out
+
+    def batch_stage(self, requests) -> None:
+        keep, views = self._views(requests)
+        err = _err()
+        _check(lib().pirg_vpir_batch_stage(self._h, len(requests), views, err,
+                                           ctypes.c_size_t(len(err))), err)
+
+    def batch_run(self, n: int) -> None:
+        err = _err()
+        _check(lib().pirg_vpir_batch_run(self._h, n, err, ctypes.c_size_t(len(err))), err)
+
+    def batch_fetch(self, n: int):
+        p = self.params
+        outs = [np.zeros((p.entry_size_multiple, p.degree), dtype=np.uint64) for _ in range(n)]
+        ptrs = (_u64p * max(n, 1))(*[o.ctypes.data_as(_u64p) for o in outs])
+        err = _err()
+        _check(lib().pirg_vpir_batch_fetch(self._h, n, ptrs, err, ctypes.c_size_t(len(err))),
+               err)
+        return outs
+
+    def autotune(self, widths: Sequence[int]) -> str:
+        w = (ctypes.c_int * len(widths))(*widths)
+        rep = ctypes.create_string_buffer(1 << 16)
+        err = _err()
+        _check(lib().pirg_vpir_autotune(self._h, w, len(widths), rep, ctypes.c_size_t(1 << 16),
+                                        err, ctypes.c_size_t(len(err))), err)
+        return rep.value.decode()
 
 
 def poly_to_proto(coeffs, log_modulus: int) -> List[bytes]:

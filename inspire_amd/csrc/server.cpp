@@ -612,11 +612,12 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   PIRG_HIP(LaunchScan(S->pk_tiles.as<int8_t>(), S->pkL, pp, W.bt_pk.as<int8_t>(),
                       W.pk_partial.as<u64>(), st));
   PIRG_HIP(mark(slot.kev[3]));
-  // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279).
-  PIRG_HIP(LaunchSpectra(W.pk.as<u64>() + static_cast<size_t>(S->k) * d, nq * S->k, S->k,
-                         2LL * S->k * d, S->fin_v, S->F, S->tw_fwd.as<double2>(),
-                         W.yh_spec.as<double2>(), st));
-  {
+  // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279). A
+  // local-finalize server (vPIR, vpir_server.cc:148-160) adds no inner product.
+  if (!S->local_finalize) {
+    PIRG_HIP(LaunchSpectra(W.pk.as<u64>() + static_cast<size_t>(S->k) * d, nq * S->k, S->k,
+                           2LL * S->k * d, S->fin_v, S->F, S->tw_fwd.as<double2>(),
+                           W.yh_spec.as<double2>(), st));
     const size_t n_fin_cap = static_cast<size_t>(W.cap) * lt;
     void* const* tab = W.fin_tab.as<void* const>();
     PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(tab),
@@ -633,7 +634,7 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   FA.ksplit_pk = pp.ksplit;
   FA.corr_pk = W.corr_pk.as<u64>();
   FA.fin_partial = W.fin_partial.as<u64>();
-  FA.npairs = S->fin_pairs.count;
+  FA.npairs = S->local_finalize ? 0 : S->fin_pairs.count;
   FA.a_tilde = S->a_tilde.as<u64>();
   FA.ct_b = W.ct_b.as<u64>();
   FA.st_a = W.st_a.as<u64>();

@@ -132,6 +132,8 @@ struct pirg_server {
   // pirg_process_batch / pirg_batch_run replay hipGraphs instead of eager
   // launches (the serving loop always does).
   bool use_graphs = false;
+  // FinalizeMatrixPack without the t_vec_h inner product (vPIR's local finalize).
+  bool local_finalize = false;
   // Cumulative stage times (GetDbMultiplyTime / GetPackingTime /
   // GetPolyEvalTime) and scan-kernel times, guarded by `mu` (the serving
   // loop adds to them from its own thread).

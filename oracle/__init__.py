@@ -667,3 +667,29 @@ def load_server(P: Params, data: bytes) -> Material:
           ctypes.c_size_t(len(data)), _p(db), pack.ctypes.data_as(ctypes.c_void_p),
           _p(a_tilde), _p(t_vec_h), _p(sda))
     return Material(db, pack, a_tilde, t_vec_h, sda, s.K)
+
+
+# ---- vPIR online path (crypto/vpir/server/vpir_server.cc) ----
+
+def vpir_process(P: Params, local_finalize: bool, db, db_cols: int, pack, a_tilde, t_vec_h,
+                 first_dim, packing_key, matmul: bool = False) -> np.ndarray:
+    """VpirServer::ProcessResponse (vpir_server.cc:101-195), or with matmul
+    ProcessResponseWithMatMul (:197-287); returns ell x d b polynomials."""
+    s = sizes(P)
+    d, ell = s.d, s.ell
+    fd = _u64(first_dim).reshape(-1)
+    pk = _u64(packing_key).reshape(-1)
+    out = np.zeros((ell, d), dtype=np.uint64)
+    cp = P.c()
+    pack = np.ascontiguousarray(pack)
+    if matmul:
+        _call(lib().orc_vpir_process_matmul, ctypes.byref(cp), ctypes.c_int(int(local_finalize)),
+              _p(_u64(db)), ctypes.c_int(db_cols), pack.ctypes.data_as(ctypes.c_void_p),
+              _p(_u64(t_vec_h)), _p(fd), ctypes.c_int(fd.size // d), _p(pk),
+              ctypes.c_int(pk.size // d), _p(out))
+    else:
+        _call(lib().orc_vpir_process, ctypes.byref(cp), ctypes.c_int(int(local_finalize)),
+              _p(_u64(db)), ctypes.c_int(db_cols), pack.ctypes.data_as(ctypes.c_void_p),
+              _p(_u64(a_tilde)), _p(_u64(t_vec_h)), _p(fd), ctypes.c_int(fd.size // d), _p(pk),
+              ctypes.c_int(pk.size // d), _p(out))
+    return out

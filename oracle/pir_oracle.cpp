@@ -2267,4 +2267,125 @@ int orc_load_server(const orc_params* P, const uint8_t* buf, size_t size,
   ORC_GUARD_END(err, errlen)
 }
 
+
+// vpir_server.cc:101-195. The DB has ell*d rows and N = db_cols columns; the
+// first dimension is N/d flattened polynomials.
+int orc_vpir_process(const orc_params* P, int local_finalize,
+                     const uint64_t* db_cdata, int db_cols, const void* pack,
+                     const uint64_t* a_tilde, const uint64_t* t_vec_h,
+                     const uint64_t* first_dim, int fd_polys,
+                     const uint64_t* packing_key, int pk_polys, uint64_t* out,
+                     char* err, size_t errlen) {
+  ORC_GUARD_BEGIN
+  (void)a_tilde;
+  if (P->interp_degree != 1) InvalidArg("vPIR parameters have interpolation degree 1.");
+  const int d = 1 << P->log_degree, ell = P->entry_size_multiple;
+  const int k = P->pack_num_digits, rows = ell * d;
+  const int expected = local_finalize ? k : 2 * k;
+  if (pk_polys != expected) {
+    InvalidArg("Packing key must contain " + std::to_string(expected) + " polynomials.");
+  }
+  if (static_cast<long long>(fd_polys) * d != db_cols) {
+    InvalidArg("Flattened first dimension query must match DB matrix columns (N).");
+  }
+  // Part 1: b' = DB * q.
+  std::vector<u64> b_prime(rows);
+  if (P->db_bytes == 1) {
+    CondensedMulU8Rows(db_cdata, db_cols, 0, rows, first_dim, b_prime.data());
+  } else {
+    CondensedMulU16Rows(db_cdata, db_cols, 0, rows, first_dim, b_prime.data());
+  }
+  // Part 2: pack * y_vec_g.
+  PackView pv;
+  if (P->mat_bytes == 4) {
+    pv.i32_cdata = static_cast<const u64*>(pack);
+  } else {
+    pv.i16 = static_cast<const int16_t*>(pack);
+  }
+  std::vector<u64> b_part(rows);
+  pv.Multiply(0, rows, k * d, packing_key, b_part.data());
+  // Part 3: complete packing.
+  std::vector<Poly> y_vec_h;
+  if (!local_finalize) y_vec_h = SplitPolys(packing_key + static_cast<size_t>(k) * d, k, d);
+  double me = 0.0;
+  for (int s = 0; s < ell; ++s) {
+    const size_t off = static_cast<size_t>(s) * d;
+    Poly b(d);
+    if (local_finalize) {
+      for (int m = 0; m < d; ++m) b[m] = b_prime[off + m] + b_part[off + m];
+      b = LowBits(b, P->log_q);
+    } else {
+      FinalizeMatrixPack(P, d, b_prime.data() + off, b_part.data() + off, y_vec_h,
+                         SplitPolys(t_vec_h + off * k, k, d), &b, &me);
+    }
+    std::copy(b.begin(), b.end(), out + off);
+  }
+  ORC_GUARD_END(err, errlen)
+}
+
+// vpir_server.cc:197-287 and CreateHPrimeMatrix (:320-375): row k*d + r,
+// column j*d + c of H' is t_vec_h[k][j][(r - c) mod d], negated when r < c,
+// read as a signed log_digit-bit value.
+int orc_vpir_process_matmul(const orc_params* P, int local_finalize,
+                            const uint64_t* db_cdata, int db_cols,
+                            const void* pack, const uint64_t* t_vec_h,
+                            const uint64_t* first_dim, int fd_polys,
+                            const uint64_t* packing_key, int pk_polys,
+                            uint64_t* out, char* err, size_t errlen) {
+  ORC_GUARD_BEGIN
+  const int d = 1 << P->log_degree, ell = P->entry_size_multiple;
+  const int k = P->pack_num_digits, rows = ell * d;
+  const int expected = local_finalize ? k : 2 * k;
+  if (pk_polys < expected) InvalidArg("Invalid packing key in request.");
+  if (static_cast<long long>(fd_polys) * d != db_cols) {
+    InvalidArg("Flattened query size must match combined DB matrix columns.");
+  }
+  std::vector<u64> b_prime(rows);
+  if (P->db_bytes == 1) {
+    CondensedMulU8Rows(db_cdata, db_cols, 0, rows, first_dim, b_prime.data());
+  } else {
+    CondensedMulU16Rows(db_cdata, db_cols, 0, rows, first_dim, b_prime.data());
+  }
+  PackView pv;
+  if (P->mat_bytes == 4) {
+    pv.i32_cdata = static_cast<const u64*>(pack);
+  } else {
+    pv.i16 = static_cast<const int16_t*>(pack);
+  }
+  std::vector<u64> b_part(rows);
+  pv.Multiply(0, rows, k * d, packing_key, b_part.data());
+  std::vector<u64> h(rows, 0);
+  if (!local_finalize) {
+    const int shift = 64 - P->pack_log_digit;
+    const u64* y_h = packing_key + static_cast<size_t>(k) * d;
+    for (int s = 0; s < ell; ++s) {
+      for (int r = 0; r < d; ++r) {
+        u64 acc = 0;
+        for (int j = 0; j < k; ++j) {
+          const u64* tv = t_vec_h + (static_cast<size_t>(s) * k + j) * d;
+          for (int c = 0; c < d; ++c) {
+            const bool neg = r < c;
+            const int m = neg ? r + d - c : r - c;
+            i64 v = static_cast<i64>(tv[m] << shift) >> shift;
+            if (neg) v = -v;
+            // H' entries are MatCoeffType; the product accumulates in CoeffType.
+            acc += static_cast<u64>(v) * y_h[static_cast<size_t>(j) * d + c];
+          }
+        }
+        h[static_cast<size_t>(s) * d + r] = acc;
+      }
+    }
+  }
+  for (int s = 0; s < ell; ++s) {
+    Poly b(d);
+    for (int m = 0; m < d; ++m) {
+      const size_t i = static_cast<size_t>(s) * d + m;
+      b[m] = b_prime[i] + b_part[i] + h[i];
+    }
+    b = LowBits(b, P->log_q);
+    std::copy(b.begin(), b.end(), out + static_cast<size_t>(s) * d);
+  }
+  ORC_GUARD_END(err, errlen)
+}
+
 }  // extern "C"

@@ -252,6 +252,27 @@ int orc_load_server(const orc_params* P, const uint8_t* buf, size_t size,
                     uint64_t* t_vec_h, uint64_t* second_dim_a, char* err,
                     size_t errlen);
 
+/* ---- vPIR online path ----
+ * VpirServer::ProcessResponse (crypto/vpir/server/vpir_server.cc:101-195):
+ * db (ell*d x N condensed), combined pack (ell*d x k*d), a_tilde (ell x d),
+ * t_vec_h (ell x k x d); first_dim: fd_polys x d; packing_key: pk_polys x d.
+ * out: ell x d b polynomials. P->interp_degree must be 1. */
+int orc_vpir_process(const orc_params* P, int local_finalize,
+                     const uint64_t* db_cdata, int db_cols, const void* pack,
+                     const uint64_t* a_tilde, const uint64_t* t_vec_h,
+                     const uint64_t* first_dim, int fd_polys,
+                     const uint64_t* packing_key, int pk_polys, uint64_t* out,
+                     char* err, size_t errlen);
+/* ProcessResponseWithMatMul (vpir_server.cc:197-287) with the H' matrix of
+ * CreateHPrimeMatrix (:320-375): the t_vec_h inner product as an exact
+ * negacyclic matrix product. */
+int orc_vpir_process_matmul(const orc_params* P, int local_finalize,
+                            const uint64_t* db_cdata, int db_cols,
+                            const void* pack, const uint64_t* t_vec_h,
+                            const uint64_t* first_dim, int fd_polys,
+                            const uint64_t* packing_key, int pk_polys,
+                            uint64_t* out, char* err, size_t errlen);
+
 /* ---- Seeded generators (documented SplitMix64 stream) ---- */
 void orc_fill_u64(uint64_t seed, uint64_t* out, size_t n, uint64_t mask);
 void orc_fill_mod(uint64_t seed, uint64_t* out, size_t n, uint64_t modulus);
