@@ -328,6 +328,91 @@ def run_codec(steps: int = 20, cpu: bool = True):
     return out
 
 
+def vpir_metrics_params(ell: int = 256) -> ia.PirParams:
+    """vpir_metrics_tool.cc's online phase (lines 64-89, 104-135): d = 2048,
+    n = 2048, 256 shards, q = 2^32 (uint32 modulus 0), uint8 DB, p = 256,
+    int32 pack matrix with gadget (8, 4)."""
+    return ia.PirParams(log_degree=11, log_q=32, log_q1=32, log_q2=32, variance=11,
+                        plaintext_modulus=256, pack_log_digit=8, pack_num_digits=4,
+                        eval_log_digit=1, eval_num_digits=1, num_entries=2048,
+                        interp_degree=1, entry_size_multiple=ell, db_bytes=1, mat_bytes=4)
+
+
+def run_vpir(widths, steps: int, warmup: int, cpu: bool, local: bool = False):
+    """VpirServer::ProcessResponse at the metrics tool's online parameters:
+    single-request and batched throughput, bit-checked against the oracle."""
+    P = vpir_metrics_params()
+    d, ell, k, n = P.degree, P.entry_size_multiple, P.pack_num_digits, P.num_entries
+    M = ell * d
+    rng = np.random.default_rng(77)
+    log(f"vPIR material: DB {M * n / 2**30:.1f} GiB (uint8), pack {M * k * d * 4 / 2**30:.1f} GiB")
+    db = np.frombuffer(rng.bytes(M * n), dtype=np.uint64).reshape(M, n // 8)
+    bound = 1 << (P.pack_log_digit + P.log_degree - 1)
+    pack32 = (np.frombuffer(rng.bytes(M * k * d * 4), dtype=np.int32) >> 13).reshape(
+        M, k * d)
+    assert -bound <= int(pack32.min()) and int(pack32.max()) < bound
+    pack = pack32.view(np.uint64).reshape(M, k * d // 2)
+    qm = np.uint64((1 << 32) - 1)
+    a_tilde = rng.integers(0, 1 << 32, size=(ell, d), dtype=np.uint64)
+    t_vec_h = rng.integers(-128, 128, size=(ell, k, d)).astype(np.uint64)
+    nk = k if local else 2 * k
+    reqs = [(rng.integers(0, 1 << 32, size=n, dtype=np.uint64),
+             rng.integers(0, 1 << 32, size=(nk, d), dtype=np.uint64) & qm)
+            for _ in range(max(widths))]
+    t0 = time.perf_counter()
+    srv = ia.VpirServer.create(P, local, db, M, n, pack, a_tilde, t_vec_h)
+    log(f"vPIR server resident ({time.perf_counter() - t0:.1f}s)")
+    out = {"workload": "vpir_metrics_tool online: d=2048 n=2048 ell=256 q=2^32 u8 DB, "
+                       "int32 pack (8, 4), " + ("local finalize" if local else "FFT finalize"),
+           "widths": []}
+    try:
+        srv.autotune([w for w in widths])
+        for B in widths:
+            srv.batch_stage(reqs[:B])
+            srv.use_graphs(False)
+            srv.scan_times_ms(reset=True)
+            for _ in range(2):
+                srv.batch_run(B)
+            db_ms, pk_ms, nl = srv.scan_times_ms(reset=True)
+            srv.use_graphs(True)
+            for _ in range(warmup):
+                srv.batch_run(B)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                srv.batch_run(B)
+            dt = (time.perf_counter() - t0) / steps
+            srv.use_graphs(False)
+            pack_bytes = M * k * d * 4
+            line = {"batch": B, "qps": B / dt, "ms_per_batch": dt * 1e3,
+                    "db_scan_ms": db_ms / max(nl, 1), "pack_scan_ms": pk_ms / max(nl, 1)}
+            if B == 1:
+                gbs = pack_bytes / (line["pack_scan_ms"] * 1e-3) / 1e9
+                line["pack_scan_roofline"] = {"bound": "hbm", "achieved": gbs,
+                                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                              "frac": gbs / HBM_PEAK_GBS}
+            out["widths"].append(line)
+            log(f"vPIR width {B}: {line['qps']:.1f} q/s, {line['ms_per_batch']:.3f} ms/batch, "
+                f"pack scan {line['pack_scan_ms']:.3f} ms")
+        got = srv.process_response(*reqs[0])
+    finally:
+        srv.close()
+    single = next(w for w in out["widths"] if w["batch"] == 1)
+    best = max(out["widths"], key=lambda w: w["qps"])
+    out.update({"single_qps": single["qps"], "batched_qps": best["qps"],
+                "best_batch": best["batch"], "batched_over_single": best["qps"] / single["qps"]})
+    if cpu:
+        import oracle as O
+        OP = O.Params(**{f: getattr(P, f) for f in O.Params.__dataclass_fields__})
+        t0 = time.perf_counter()
+        want = O.vpir_process(OP, local, db, n, pack, a_tilde, t_vec_h, *reqs[0])
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / ct, "unit": "queries/s", "cores": 1,
+                               "kind": "port", "sample": "one request through the oracle's "
+                                                         "VpirServer::ProcessResponse"}
+        out["bit_identical"] = bool(np.array_equal(got, want))
+    return out
+
+
 def run_packing(njobs: int, seed: int = 11, verify: int = 2):
     """MatrixPack for njobs jobs at d = 2048, q = 2^56, one graph launch."""
     log_d, k = 11, 2
@@ -409,6 +494,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-pack", action="store_true")
     ap.add_argument("--no-load", action="store_true")
+    ap.add_argument("--no-vpir", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=3)
     ap.add_argument("--serve-rates", default="100,500,1000,2000,5000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -513,6 +599,12 @@ def main():
     log(f"codec: encode {codec['encode_gbs']:.0f} GB/s, decode {codec['decode_gbs']:.0f} GB/s, "
         f"exact {codec['round_trip_exact']}")
 
+    vpir = None
+    if not args.no_vpir:
+        vpir = run_vpir([1, 16, 64, 256], args.steps, args.warmup, cpu=not args.no_cpu)
+        log(f"vPIR: single {vpir['single_qps']:.0f} q/s, batched {vpir['batched_qps']:.0f} q/s "
+            f"(width {vpir['best_batch']}), exact {vpir.get('bit_identical')}")
+
     serving = None
     if not args.no_serve:
         rates = [float(x) for x in args.serve_rates.split(",")]
@@ -553,6 +645,7 @@ def main():
         "serving": serving,
         "packing": packing,
         "load": load,
+        "vpir": vpir,
         "codec": codec,
         "cpu_baseline": cpu,
     }

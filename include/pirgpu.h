@@ -380,6 +380,10 @@ int pirg_vpir_batch_fetch(pirg_vpir_server* server, int nreq, uint64_t* const* o
                           char* err, size_t errlen);
 int pirg_vpir_autotune(pirg_vpir_server* server, const int* widths, int nwidths,
                        char* report, size_t report_len, char* err, size_t errlen);
+/* As pirg_server_use_graphs / pirg_server_scan_times, for the underlying server. */
+int pirg_vpir_use_graphs(pirg_vpir_server* server, int enable);
+int pirg_vpir_scan_times(pirg_vpir_server* server, double* db_scan_ms, double* pack_scan_ms,
+                         long long* launches, int reset);
 
 const char* pirg_version(void);
 

@@ -639,6 +639,15 @@ class VpirServer:
                err)
         return outs
 
+    def use_graphs(self, enable: bool = True) -> None:
+        lib().pirg_vpir_use_graphs(self._h, int(enable))
+
+    def scan_times_ms(self, reset: bool = False):
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        lib().pirg_vpir_scan_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n),
+                                   int(reset))
+        return a.value, b.value, n.value
+
     def autotune(self, widths: Sequence[int]) -> str:
         w = (ctypes.c_int * len(widths))(*widths)
         rep = ctypes.create_string_buffer(1 << 16)

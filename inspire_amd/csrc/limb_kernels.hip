@@ -822,7 +822,13 @@ hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan
   }
   switch (P.ctg) {
     case 2: return LaunchWideT<NP, 2, false>(tiles, L, P, btiles, partial, s);
-    case 4: return LaunchWideT<NP, 4, false>(tiles, L, P, btiles, partial, s);
+    case 4:
+      if (!P.digit_major) return LaunchWideT<NP, 4, false>(tiles, L, P, btiles, partial, s);
+      if constexpr (NP <= 2) {
+        return LaunchWidePipeT<NP, 4>(tiles, L, P, btiles, partial, s);
+      } else {
+        return LaunchWideT<NP, 4, true>(tiles, L, P, btiles, partial, s);
+      }
     case 7:
       if (!P.digit_major) return LaunchWideT<NP, 7, false>(tiles, L, P, btiles, partial, s);
       // Two register sets of fragments fit beside the accumulators for NP <= 2.
@@ -961,7 +967,8 @@ hipError_t ConfigureScanKernels() {
                            ConfigureWide<2, 2>(), ConfigureWide<2, 4>(), ConfigureWide<2, 7>(),
                            ConfigureWide<4, 2>(), ConfigureWide<4, 4>(), ConfigureWide<4, 7>(),
                            ConfigureWide<4, 7, true>(), ConfigureWidePipe<1, 7>(),
-                           ConfigureWidePipe<2, 7>()};
+                           ConfigureWidePipe<2, 7>(), ConfigureWide<4, 4, true>(),
+                           ConfigureWidePipe<1, 4>(), ConfigureWidePipe<2, 4>()};
   for (hipError_t e : es) {
     if (e != hipSuccess) return e;
   }
@@ -999,8 +1006,9 @@ void CompletePlan(const LimbLayout& L, int ks_scale, ScanPlan* P) {
   P->btile_bytes = static_cast<size_t>(L.kblocks + 1) * P->ngp * P->ctg * kTileBytes;
 }
 
-// Digit-major WideScan is instantiated for this many shifts per query.
-constexpr int kDigitMajorShifts = 7;
+// Digit-major WideScan is instantiated for 7 shifts per query (log q in
+// (48, 56]) and for 4 (log q in (24, 32]).
+inline bool DigitMajorShifts(int ns) { return ns == 7 || ns == 4; }
 
 std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits) {
   std::vector<ScanPlan> out;
@@ -1030,7 +1038,7 @@ std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bi
     }
   }
   // Digit-major groups hold 16 queries: worth trying from two groups on.
-  if (ns == kDigitMajorShifts && nq > 16) add(1, ns, 1);
+  if (DigitMajorShifts(ns) && nq > 16) add(1, ns, 1);
   return out;
 }
 
@@ -1052,7 +1060,7 @@ ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_qu
   if (narrow_tiles > 0 && nq < wide_min_queries) {
     P.wide = 0;
     P.ctg = narrow_tiles;
-  } else if (allow_digit_major && ns == kDigitMajorShifts && nq > 32) {
+  } else if (allow_digit_major && DigitMajorShifts(ns) && nq > 32) {
     // Digit-major: no plane shifts and (2*ns - 1)/(2*ns) of the MFMAs, at a
     // granularity of 64 queries per column block.
     P.wide = 1;
@@ -1129,7 +1137,7 @@ hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& 
                       const int8_t* btiles, u64* partial, hipStream_t s) {
   if (P.nq <= 0) return hipSuccess;
   if (P.qpg < 1 || P.ngp < 1) return hipErrorInvalidValue;
-  if (P.digit_major && (!P.wide || P.ctg != kDigitMajorShifts || P.ns != P.ctg ||
+  if (P.digit_major && (!P.wide || !DigitMajorShifts(P.ctg) || P.ns != P.ctg ||
                         P.qpg != 16 || P.ngp % 4 != 0)) {
     return hipErrorInvalidValue;
   }

@@ -216,4 +216,15 @@ int pirg_vpir_autotune(pirg_vpir_server* V, const int* widths, int nwidths, char
   return rc;
 }
 
+int pirg_vpir_use_graphs(pirg_vpir_server* V, int enable) {
+  return V == nullptr ? 3 : pirg_server_use_graphs(V->inner, enable);
+}
+
+int pirg_vpir_scan_times(pirg_vpir_server* V, double* db_scan_ms, double* pack_scan_ms,
+                         long long* launches, int reset) {
+  return V == nullptr ? 3
+                      : pirg_server_scan_times(V->inner, db_scan_ms, pack_scan_ms, launches,
+                                               reset);
+}
+
 }  // extern "C"

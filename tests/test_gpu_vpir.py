@@ -52,9 +52,15 @@ def test_vpir_answers_match_oracle(name, local):
                                       err_msg=f"request {i}")
 
 
-@pytest.mark.parametrize("nq", [1, 20, 70])
-def test_vpir_device_batches_and_replays(nq):
-    P = metrics_like(ell=2)
+@pytest.mark.parametrize("cfg,nq", [("u8", 1), ("u8", 20), ("u8", 70), ("u16", 40),
+                                    ("u16", 129)])
+def test_vpir_device_batches_and_replays(cfg, nq):
+    # log q = 32: 4-digit queries, digit-major wide scans above 32 queries
+    # (uint8 DB: 1 plane, uint16: 2 planes, int32 pack: 4 planes).
+    if cfg == "u8":
+        P = metrics_like(ell=2)
+    else:
+        P = vpir_params(32, 4, ell=2, log_d=8, n=2048, pack_log_digit=8, pack_num_digits=4)
     srv, mat = make(P, False, seed=5)
     reqs = [request(P, 500 + i, False) for i in range(nq)]
     with srv:
