@@ -327,6 +327,22 @@ int pirg_poly_from_proto_device(const uint64_t* d_words, long long pitch_words, 
 int pirg_server_load_from_buffer(const pirg_params* params, const uint8_t* buffer,
                                  size_t size, pirg_server** out, char* err, size_t errlen);
 
+/* DeserializePirPreprocessedData + PirServer::Create
+ * (crypto/pir/server/pir_preprocessed_data.h:93-148, pir_server.cc:52-124) on
+ * the bytes of a serialized proto::PirPreprocessedData
+ * (crypto/proto/pir_server.proto:54-66), i.e. the contents of the file
+ * SavePirPreprocessedDataToFile writes. Matrices are taken as stored;
+ * polynomials are decoded on the GPU. Errors: "Failed to parse
+ * PirPreprocessedData from file.", Matrix::FromProto's and CreateFromProto's
+ * messages, then Create's. */
+int pirg_server_create_from_proto(const pirg_params* params, const uint8_t* bytes, size_t size,
+                                  pirg_server** out, char* err, size_t errlen);
+/* LoadPirPreprocessedDataFromDirectory + PirServer::Create
+ * (pir_preprocessed_data.h:226-320): metadata.bin and shard_<i>.bin under
+ * dir_path, for i < entry_size_multiple. */
+int pirg_server_create_from_directory(const pirg_params* params, const char* dir_path,
+                                      pirg_server** out, char* err, size_t errlen);
+
 /* ---- vPIR online path (crypto/vpir/server/vpir_server.h:47-121) ----
  * VpirServer's answer is PirServer's with one interpolation slice (t = 1)
  * and no modulus switch: b = FinalizeMatrixPack(DB*q, pack*y_g, y_h) per

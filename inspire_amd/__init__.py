@@ -301,6 +301,33 @@ class PirServer:
         return cls(handle, params)
 
     @classmethod
+    def create_from_proto(cls, params: PirParams, data: bytes) -> "PirServer":
+        """PirServer::Create on a serialized proto::PirPreprocessedData
+        (pirg_server_create_from_proto)."""
+        buf = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+        handle = ctypes.c_void_p()
+        err = _err()
+        cp = params.c()
+        rc = lib().pirg_server_create_from_proto(
+            ctypes.byref(cp), buf.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(len(data)),
+            ctypes.byref(handle), err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return cls(handle, params)
+
+    @classmethod
+    def create_from_directory(cls, params: PirParams, path: str) -> "PirServer":
+        """PirServer::Create on SavePirPreprocessedDataToDirectory's files
+        (pirg_server_create_from_directory)."""
+        handle = ctypes.c_void_p()
+        err = _err()
+        cp = params.c()
+        rc = lib().pirg_server_create_from_directory(
+            ctypes.byref(cp), path.encode(), ctypes.byref(handle), err,
+            ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return cls(handle, params)
+
+    @classmethod
     def load_from_buffer(cls, params: PirParams, buffer: bytes) -> "PirServer":
         """PirServer::LoadFromBuffer (pir_server.cc:594-793) from the byte image
         SaveToFile writes (pirg_server_load_from_buffer)."""

@@ -837,6 +837,11 @@ Error CreateFromParts(const pirg_params* params, const pirg_preprocessed_view* v
 
 }  // namespace
 
+Error CreateFromView(const pirg_params* params, const pirg_preprocessed_view* v,
+                     std::unique_ptr<pirg_server>* out) {
+  return CreateFromParts(params, v, out);
+}
+
 Error CreateFromImage(const pirg_params* params, const uint64_t* db_cdata, int db_cols,
                       const PackedPack& pack, const uint64_t* a_tilde, const uint64_t* t_vec_h,
                       const uint64_t* second_dim_a, std::unique_ptr<pirg_server>* out) {

@@ -201,6 +201,10 @@ struct PackedPack {
   int bits;
 };
 
+// PirServer::Create on a view (polynomial pointers may be device pointers).
+Error CreateFromView(const pirg_params* params, const pirg_preprocessed_view* v,
+                     std::unique_ptr<pirg_server>* out);
+
 // Server from image parts: the combined DB's condensed words (host), the
 // bit-packed pack matrix, and the polynomials already in device memory
 // (a_tilde: lt x d, t_vec_h: lt x k x d, second_dim_a: 2e x d or null).

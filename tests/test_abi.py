@@ -178,3 +178,30 @@ def test_load_from_buffer_reads_and_checks_before_the_device():
     with pytest.raises(ia.PirError) as ei:
         ia.PirServer.load_from_buffer(to_gpu_params(P2), img)
     assert ei.value.code == ia.INVALID_ARGUMENT
+
+
+def test_proto_loader_checks_before_the_device(tmp_path):
+    import pir_proto
+    P = O.metrics_tool_params(n=2048, log_d=8, t=2)
+    gp = to_gpu_params(P)
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.create_from_proto(gp, b"\xff\xff\xff")
+    assert ei.value.message == "Failed to parse PirPreprocessedData from file."
+    mat = O.synthetic_material(P, seed=1)
+    msg = pir_proto.M["PirPreprocessedData"]()
+    msg.ParseFromString(pir_proto.serialize_data(P, mat))
+    # CreateFromProto's size check on a_tilde_agg (polynomial.cc:221-226).
+    msg.preprocessed_outputs[0].outputs[0].a_tilde_agg.encoded_coeffs = b"\x01\x02"
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.create_from_proto(gp, msg.SerializeToString())
+    assert ei.value.message == "Unexpected serialized size."
+    # Matrix::FromProto (crypto/matrix.cc:504-507).
+    msg.ParseFromString(pir_proto.serialize_data(P, mat))
+    m = msg.preprocessed_outputs[0].outputs[1].matrix
+    m.data = m.condensed_data
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.create_from_proto(gp, msg.SerializeToString())
+    assert ei.value.message == "Condensed matrix cannot have normal data set."
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.create_from_directory(gp, str(tmp_path / "missing"))
+    assert ei.value.code == 5
