@@ -401,6 +401,59 @@ int pirg_vpir_use_graphs(pirg_vpir_server* server, int enable);
 int pirg_vpir_scan_times(pirg_vpir_server* server, double* db_scan_ms, double* pack_scan_ms,
                          long long* launches, int reset);
 
+/* ---- Offline preprocessing (crypto/pir/server/preprocessing_server.h:68-273) ----
+ * The reference's PreprocessingServer samples the first-dimension, second-
+ * dimension and packing a-components from its ChaCha PRNG (:98-111); here the
+ * caller passes them. Everything else is computed on the GPU, bit-identical to
+ * the reference. */
+
+/* Interpolate (crypto/db_row_interpolation.cc:115-228) of nrows rows of t*d
+ * DB values each (uint16 or uint8 per db_bytes, host memory); out has the
+ * same shape. t == 1 copies the rows. Errors as the reference: "ring_degree
+ * must be a power of two.", "interpolation_degree must be a power of two.",
+ * "interpolation_degree must be less than or equal to 2 * ring_degree.",
+ * "plaintext_modulus must be odd." */
+int pirg_interpolate(const void* rows, int db_bytes, long long nrows, int d, int t,
+                     uint32_t plaintext_modulus, void* out, char* err, size_t errlen);
+
+/* ComputeHintMatrix<uint64_t, S, uint64_t> (crypto/hint_computation.cc:78-160):
+ * db is rows x cols (S = uint16 / uint8 per db_bytes), a_components n x d,
+ * out rows x d (host memory). Errors: "Database matrix cannot be empty.",
+ * "a_components cannot be empty.", "Database columns must match d * n where d
+ * is the polynomial length and n is the number of polynomials." */
+int pirg_compute_hint_matrix(const void* db, int db_bytes, int rows, int cols,
+                             const uint64_t* a_components, int n, int d, uint64_t* out,
+                             char* err, size_t errlen);
+
+/* PreprocessMatrixPack (crypto/lwes_to_rlwe.cc:421-446) for nblocks
+ * independent blocks at once: block b's LWE a-vectors are rows
+ * [b*d, b*d + d) of a_is (d words each); w is w_vec_g then w_vec_h (2k x d).
+ * Outputs: pack rows [b*d, b*d + d) (d x k*d, int32 condensed words or int16
+ * values per mat_bytes), a_tilde nblocks x d, t_vec_h nblocks x k x d. */
+int pirg_preprocess_matrix_pack(const pirg_params* params, int nblocks, const uint64_t* a_is,
+                                const uint64_t* w, void* pack, uint64_t* a_tilde,
+                                uint64_t* t_vec_h, char* err, size_t errlen);
+
+/* PreprocessingServer::Preprocess: db_raw holds num_entries * d * ell values
+ * (record-major, uint16 / uint8), first_dim_a ceil((n/t)/d) x d, pack_a
+ * 2k x d (w_vec_g then w_vec_h). Outputs (host, each may be NULL) in the
+ * layouts pirg_server_create_flat takes: db_cdata M x K/4 (or K/8) words with
+ * *db_cols = K, pack, a_tilde, t_vec_h. stage_ms (4 entries, may be NULL)
+ * receives the interpolation, hint, pack-preprocessing and total GPU times
+ * (GetInterpolateTime / GetComputeHintsTime / GetPreprocessPackTime,
+ * preprocessing_server.h:165-168). */
+int pirg_preprocess(const pirg_params* params, const void* db_raw, const uint64_t* first_dim_a,
+                    const uint64_t* pack_a, uint64_t* db_cdata, int* db_cols, void* pack,
+                    uint64_t* a_tilde, uint64_t* t_vec_h, double* stage_ms, char* err,
+                    size_t errlen);
+
+/* Preprocess, then PirServer::Create on the device-resident result (no host
+ * round trip of the material). second_dim_a: 2e x d, or NULL when t == 1. */
+int pirg_server_create_preprocessed(const pirg_params* params, const void* db_raw,
+                                    const uint64_t* first_dim_a, const uint64_t* second_dim_a,
+                                    const uint64_t* pack_a, double* stage_ms, pirg_server** out,
+                                    char* err, size_t errlen);
+
 const char* pirg_version(void);
 
 #ifdef __cplusplus

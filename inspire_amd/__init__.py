@@ -341,6 +341,30 @@ class PirServer:
         _check(rc, err)
         return cls(handle, params)
 
+    @classmethod
+    def create_preprocessed(cls, params: PirParams, db_raw: np.ndarray,
+                            first_dim_a: np.ndarray, second_dim_a: Optional[np.ndarray],
+                            pack_a: np.ndarray) -> "PirServer":
+        """PreprocessingServer::Preprocess then PirServer::Create, both on the
+        GPU (pirg_server_create_preprocessed); the material stays in HBM.
+        The stage times of the preprocessing are in `preprocess_ms`."""
+        raw = np.ascontiguousarray(db_raw, dtype=np.uint16 if params.db_bytes == 2 else np.uint8)
+        fa, pa = _u64(first_dim_a), _u64(pack_a)
+        sda = _u64(second_dim_a) if second_dim_a is not None else None
+        ms = (ctypes.c_double * 4)()
+        handle = ctypes.c_void_p()
+        err = _err()
+        cp = params.c()
+        rc = lib().pirg_server_create_preprocessed(
+            ctypes.byref(cp), raw.ctypes.data_as(ctypes.c_void_p), fa.ctypes.data_as(_u64p),
+            sda.ctypes.data_as(_u64p) if sda is not None else None, pa.ctypes.data_as(_u64p),
+            ms, ctypes.byref(handle), err, ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        srv = cls(handle, params)
+        srv.preprocess_ms = dict(zip(("interpolate", "hints", "preprocess_pack", "total"),
+                                     list(ms)))
+        return srv
+
     def close(self) -> None:
         if self._h is not None and self._h.value:
             lib().pirg_server_destroy(self._h)
@@ -778,6 +802,104 @@ def inner_product_fft(u, v, u_bits=64, v_bits=64, chunk_bits=20, u_signed=False)
                                       out.ctypes.data_as(_u64p), err, ctypes.c_size_t(len(err)))
     _check(rc, err)
     return out
+
+
+# ---- Offline preprocessing (crypto/pir/server/preprocessing_server.h) ----
+
+def interpolate(rows, d: int, t: int, plaintext_modulus: int, db_bytes: int = 2) -> np.ndarray:
+    """Interpolate (crypto/db_row_interpolation.cc:115-228) of every row (t*d
+    values) on the GPU."""
+    dt = np.uint16 if db_bytes == 2 else np.uint8
+    rows = np.ascontiguousarray(rows, dtype=dt)
+    nrows = rows.size // (t * d)
+    out = np.empty_like(rows)
+    err = _err()
+    rc = lib().pirg_interpolate(rows.ctypes.data_as(ctypes.c_void_p), db_bytes,
+                                ctypes.c_longlong(nrows), d, t,
+                                ctypes.c_uint32(plaintext_modulus),
+                                out.ctypes.data_as(ctypes.c_void_p), err,
+                                ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    return out
+
+
+def compute_hint_matrix(db, a_components, db_bytes: int = 2) -> np.ndarray:
+    """ComputeHintMatrix (crypto/hint_computation.cc:78-160) on the GPU: db is
+    rows x cols, a_components n x d; returns rows x d."""
+    dt = np.uint16 if db_bytes == 2 else np.uint8
+    db = np.ascontiguousarray(db, dtype=dt)
+    a = _u64_2d(a_components)
+    rows = db.shape[0] if db.ndim == 2 else 0
+    cols = db.shape[1] if db.ndim == 2 else 0
+    n, d = a.shape
+    out = np.empty((max(rows, 1), d), dtype=np.uint64)
+    err = _err()
+    rc = lib().pirg_compute_hint_matrix(db.ctypes.data_as(ctypes.c_void_p), db_bytes, rows, cols,
+                                        a.ctypes.data_as(_u64p), n, d,
+                                        out.ctypes.data_as(_u64p), err,
+                                        ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    return out[:rows]
+
+
+def preprocess_matrix_pack(params: PirParams, a_is, w):
+    """PreprocessMatrixPack (crypto/lwes_to_rlwe.cc:421-446) on the GPU for
+    nblocks blocks: a_is is (nblocks*d) x d, w is 2k x d. Returns (pack,
+    a_tilde, t_vec_h): pack is (nblocks*d) x k*d/2 condensed words (mat_bytes
+    4) or (nblocks*d) x k*d int16."""
+    d, k = params.degree, params.pack_num_digits
+    a_is = _u64_2d(a_is)
+    w = _u64_2d(w)
+    nb = a_is.shape[0] // d
+    if params.mat_bytes == 4:
+        pack = np.empty((nb * d, k * d // 2), dtype=np.uint64)
+    else:
+        pack = np.empty((nb * d, k * d), dtype=np.int16)
+    a_tilde = np.empty((nb, d), dtype=np.uint64)
+    t_vec_h = np.empty((nb, k, d), dtype=np.uint64)
+    err = _err()
+    cp = params.c()
+    rc = lib().pirg_preprocess_matrix_pack(
+        ctypes.byref(cp), nb, a_is.ctypes.data_as(_u64p), w.ctypes.data_as(_u64p),
+        pack.ctypes.data_as(ctypes.c_void_p), a_tilde.ctypes.data_as(_u64p),
+        t_vec_h.ctypes.data_as(_u64p), err, ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    return pack, a_tilde, t_vec_h
+
+
+def preprocess(params: PirParams, db_raw, first_dim_a, pack_a) -> dict:
+    """PreprocessingServer::Preprocess on the GPU (pirg_preprocess) with the
+    given a-components. Returns the material in the pirg_server_create_flat
+    layouts (db, db_cols, pack, a_tilde, t_vec_h) and the stage times."""
+    d, t, ell, k = params.degree, params.interp_degree, params.entry_size_multiple, \
+        params.pack_num_digits
+    r = params.num_entries // t
+    simd = 64 if params.db_bytes == 1 else 32
+    K = (r + simd - 1) // simd * simd
+    M = ell * t * d
+    raw = np.ascontiguousarray(db_raw, dtype=np.uint16 if params.db_bytes == 2 else np.uint8)
+    fa, pa = _u64_2d(first_dim_a), _u64_2d(pack_a)
+    db = np.empty((M, K * params.db_bytes // 8), dtype=np.uint64)
+    if params.mat_bytes == 4:
+        pack = np.empty((M, k * d // 2), dtype=np.uint64)
+    else:
+        pack = np.empty((M, k * d), dtype=np.int16)
+    a_tilde = np.empty((ell * t, d), dtype=np.uint64)
+    t_vec_h = np.empty((ell * t, k, d), dtype=np.uint64)
+    ms = (ctypes.c_double * 4)()
+    cols = ctypes.c_int()
+    err = _err()
+    cp = params.c()
+    rc = lib().pirg_preprocess(
+        ctypes.byref(cp), raw.ctypes.data_as(ctypes.c_void_p), fa.ctypes.data_as(_u64p),
+        pa.ctypes.data_as(_u64p), db.ctypes.data_as(_u64p), ctypes.byref(cols),
+        pack.ctypes.data_as(ctypes.c_void_p), a_tilde.ctypes.data_as(_u64p),
+        t_vec_h.ctypes.data_as(_u64p), ms, err, ctypes.c_size_t(len(err)))
+    _check(rc, err)
+    assert cols.value == K
+    return dict(db=db, db_cols=K, pack=pack, a_tilde=a_tilde, t_vec_h=t_vec_h,
+                stage_ms=dict(zip(("interpolate", "hints", "preprocess_pack", "total"),
+                                  list(ms))))
 
 
 def copy_probe(nbytes: int, repeats: int = 10) -> float:

@@ -21,7 +21,8 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libpirgpu.so")
 
 SOURCES = ["limb_kernels.hip", "fft_kernels.hip", "codec.hip", "server.cpp", "serve.cpp",
-           "pack.cpp", "loader.cpp", "proto_loader.cpp", "vpir.cpp"]
+           "pack.cpp", "loader.cpp", "proto_loader.cpp", "vpir.cpp", "preprocess_kernels.hip",
+           "preprocess.cpp"]
 # Per-source flags. The scan kernels keep up to 224 int32 accumulators per
 # lane; LLVM's default AGPR form for their MFMAs makes the register allocator
 # shuttle accumulators between AGPRs and VGPRs every k-block, while the VGPR

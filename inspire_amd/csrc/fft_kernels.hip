@@ -15,6 +15,7 @@
 // A workgroup of N/16 threads owns one N-point transform (N = 2d); the kernels
 // are instantiated per log2 N and dispatched at launch.
 #include "fft_regs.h"
+#include "preprocess_internal.h"
 
 namespace pirg {
 namespace {
@@ -238,6 +239,117 @@ __global__ __launch_bounds__(Geo<L>::NT) void DigitsSpectraKernel(
 }
 
 
+// Forward spectra of DB row polynomials (ComputeHintMatrix,
+// hint_computation.cc:120-152): job = (row * polys + k) * nu + a.
+template <int L, typename Elem>
+__global__ __launch_bounds__(Geo<L>::NT) void DbSpectraKernel(
+    const Elem* __restrict__ db, long long stride, long long valid, int polys, ChunkRule rule,
+    const double2* __restrict__ st, double2* __restrict__ dst) {
+  extern __shared__ double2 lds[];
+  constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
+  const int t = threadIdx.x;
+  const long long job = blockIdx.x;
+  const long long pidx = job / rule.num_chunks;
+  const int c = static_cast<int>(job % rule.num_chunks);
+  const long long x = pidx / polys;
+  const long long col0 = (pidx % polys) * d;
+  const Elem* row = db + x * stride;
+  double2 v[16];
+#pragma unroll
+  for (int k = 0; k < E / 2; ++k) {
+    const long long col = col0 + t + NT * k;
+    const u64 val = col < valid ? static_cast<u64>(row[col]) : 0ULL;
+    v[InSlot<L>(k)] = make_double2(ChunkValue(rule, val, c), 0.0);
+  }
+#pragma unroll
+  for (int k = E / 2; k < E; ++k) v[InSlot<L>(k)] = make_double2(0.0, 0.0);
+  Transform<L>(v, lds, st);
+  double2* out = dst + job * N + t;
+#pragma unroll
+  for (int e = 0; e < E; ++e) out[NT * e] = v[OutSlot<L>(e)];
+}
+
+// One step of PreprocessPack's reduction loop (lwes_to_rlwe.cc:313-346):
+// SignedGadgetInv digits of the chain states (polynomial.cc:376-427) and the
+// automorphed packing polynomials (polynomial.cc:429-456), each chunked and
+// transformed. Digit jobs come first, then automorphism jobs.
+template <int L>
+__global__ __launch_bounds__(Geo<L>::NT) void ChainSpectraKernel(
+    ChainSpectraArgs A, const double2* __restrict__ st) {
+  extern __shared__ double2 lds[];
+  constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
+  const int t = threadIdx.x;
+  const int nu = A.ur.num_chunks, nv = A.vr.num_chunks;
+  const int dig_jobs = A.n_dig * A.k * nu;
+  const int job = blockIdx.x;
+  double2 v[16];
+  double2* out;
+  if (job < dig_jobs) {
+    const int ch = job / (A.k * nu);
+    const int l = (job / nu) % A.k;
+    const int a = job % nu;
+    const int chain = A.dig_first + ch * A.dig_step;
+    const u64* state = A.state + static_cast<long long>(chain) * d;
+    const int ld = A.log_digit, ts = A.to_shift;
+    const u64 mask = (1ULL << ld) - 1;
+    const u64 halfm = 1ULL << (ld - 1);
+    uint32_t* d32 = A.digits32 == nullptr
+                        ? nullptr
+                        : A.digits32 + ((static_cast<long long>(chain >> 1) * A.k + l) * d +
+                                        ((chain & 1) ? A.slot_gh : A.slot_g)) * d;
+    u64* d64 = A.digits64 == nullptr
+                   ? nullptr
+                   : A.digits64 + (static_cast<long long>(ch) * A.k + l) * d;
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k) {
+      const int pos = t + NT * k;
+      const u64 x = state[pos] & A.qmask;
+      // (x + 2^(ts-1)) >> ts without the 65-bit intermediate.
+      u64 cur = ts > 0 ? (x >> ts) + ((x >> (ts - 1)) & 1ULL) : x;
+      u64 dig = 0;
+      for (int jj = 0; jj <= l; ++jj) {
+        const u64 lower = cur & mask;
+        if (lower >= halfm) {
+          dig = static_cast<u64>(static_cast<i64>(lower) - static_cast<i64>(1ULL << ld));
+          cur = (cur >> ld) + 1;
+        } else {
+          dig = lower;
+          cur = cur >> ld;
+        }
+      }
+      v[InSlot<L>(k)] = make_double2(ChunkValue(A.ur, dig, a), 0.0);
+      if (a == 0) {
+        if (d32 != nullptr) d32[pos] = static_cast<uint32_t>(dig);
+        if (d64 != nullptr) d64[pos] = dig;
+      }
+    }
+    out = A.uspec + static_cast<long long>(job) * N;
+  } else {
+    const int jw = job - dig_jobs;
+    const int ty = jw / (A.k * nv);
+    const int m = (jw / nv) % A.k;
+    const int b = jw % nv;
+    const u64* w = A.w + static_cast<long long>(m) * d;
+    const int pinv = A.pinv[ty];
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k) {
+      const int pos = t + NT * k;
+      // Automorph(w, power)[pos] = w[i] for i * power = pos (mod 2d), or
+      // -w[i - d] when that i is >= d.
+      const int i1 = static_cast<int>((static_cast<long long>(pos) * pinv) & (2 * d - 1));
+      const u64 val = i1 < d ? w[i1] : (0ULL - w[i1 - d]);
+      v[InSlot<L>(k)] = make_double2(ChunkValue(A.vr, val, b), 0.0);
+    }
+    out = A.vspec + static_cast<long long>(jw) * N;
+  }
+#pragma unroll
+  for (int k = E / 2; k < E; ++k) v[InSlot<L>(k)] = make_double2(0.0, 0.0);
+  Transform<L>(v, lds, st);
+  out += t;
+#pragma unroll
+  for (int e = 0; e < E; ++e) out[NT * e] = v[OutSlot<L>(e)];
+}
+
 // EvalPoly's Horner update (pir_server.cc:141-146): result = product + c_i,
 // not reduced; after the last step the modulus switch (pir_server.cc:288-297).
 __global__ void ExtCombineKernel(ExtCombineArgs A) {
@@ -319,6 +431,15 @@ struct Configure {
     e = hipFuncSetAttribute(reinterpret_cast<const void*>(IpKernel<L>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(DbSpectraKernel<L, uint16_t>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(DbSpectraKernel<L, uint8_t>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(ChainSpectraKernel<L>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void*>(DigitsSpectraKernel<L>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   }
@@ -349,6 +470,39 @@ struct IpL {
     hipLaunchKernelGGL(IpKernel<L>, dim3(static_cast<unsigned>(blocks)), dim3(Geo<L>::NT),
                        Geo<L>::kLds * sizeof(double2), s, jobs, static_cast<int>(n_jobs),
                        static_cast<int>(group_jobs), nterms, nu, nv, pairs, st, margin);
+    return hipGetLastError();
+  }
+};
+
+template <int L>
+struct DbSpectraL {
+  static hipError_t Run(const void* db, int elem_bytes, long long stride, long long valid,
+                        int rows, int polys, const ChunkRule& rule, const double2* st,
+                        double2* dst, hipStream_t s) {
+    const long long jobs = static_cast<long long>(rows) * polys * rule.num_chunks;
+    if (jobs > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (elem_bytes == 2) {
+      hipLaunchKernelGGL((DbSpectraKernel<L, uint16_t>), dim3(static_cast<unsigned>(jobs)),
+                         dim3(Geo<L>::NT), Geo<L>::kLds * sizeof(double2), s,
+                         static_cast<const uint16_t*>(db), stride, valid, polys, rule, st, dst);
+    } else if (elem_bytes == 1) {
+      hipLaunchKernelGGL((DbSpectraKernel<L, uint8_t>), dim3(static_cast<unsigned>(jobs)),
+                         dim3(Geo<L>::NT), Geo<L>::kLds * sizeof(double2), s,
+                         static_cast<const uint8_t*>(db), stride, valid, polys, rule, st, dst);
+    } else {
+      return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+};
+
+template <int L>
+struct ChainSpectraL {
+  static hipError_t Run(const ChainSpectraArgs& A, const double2* st, hipStream_t s) {
+    const int jobs = A.n_dig * A.k * A.ur.num_chunks + A.n_wtypes * A.k * A.vr.num_chunks;
+    if (jobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ChainSpectraKernel<L>, dim3(jobs), dim3(Geo<L>::NT),
+                       Geo<L>::kLds * sizeof(double2), s, A, st);
     return hipGetLastError();
   }
 };
@@ -397,6 +551,21 @@ hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
   IpJob jobs{uptr, vptr, optr};
   return DispatchLogN<IpL>(F.log_N, jobs, n_out, outs_per_group, nterms, nu, nv, pairs,
                            tw_bwd, margin, s);
+}
+
+hipError_t LaunchDbSpectra(const void* db, int elem_bytes, long long stride, long long valid_cols,
+                           int rows, int polys, const ChunkRule& rule, const FftDims& F,
+                           const double2* tw_fwd, double2* dst, hipStream_t s) {
+  if (rows <= 0 || polys <= 0) return hipSuccess;
+  if (!ValidDims(F)) return hipErrorInvalidValue;
+  return DispatchLogN<DbSpectraL>(F.log_N, db, elem_bytes, stride, valid_cols, rows, polys, rule,
+                                  tw_fwd, dst, s);
+}
+
+hipError_t LaunchChainSpectra(const ChainSpectraArgs& A, const FftDims& F,
+                              const double2* tw_fwd, hipStream_t s) {
+  if (!ValidDims(F) || A.log_digit < 1 || A.log_digit > 63) return hipErrorInvalidValue;
+  return DispatchLogN<ChainSpectraL>(F.log_N, A, tw_fwd, s);
 }
 
 hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,

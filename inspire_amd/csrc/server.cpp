@@ -294,7 +294,8 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
                     int db_cols, const std::vector<const void*>& pack_parts,
                     const std::vector<const uint64_t*>& a_parts,
                     const std::vector<const uint64_t*>& th_parts,
-                    const uint64_t* second_dim_a, const PackedPack* packed = nullptr) {
+                    const uint64_t* second_dim_a, const PackedPack* packed = nullptr,
+                    bool parts_on_device = false) {
   const int d = S->d;
   S->K = db_cols;
   PIRG_HIP(hipGetDevice(&S->device));
@@ -315,8 +316,8 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   const int rows_per_part = S->M / static_cast<int>(db_parts.size());
   for (size_t part = 0; part < db_parts.size(); ++part) {
     const int base = static_cast<int>(part) * rows_per_part;
-    er = UploadTiles(db_parts[part], static_cast<size_t>(wpr) * 8, rows_per_part, false,
-                     S->main.stream, [&](const void* src, int r0, int nr) {
+    er = UploadTiles(db_parts[part], static_cast<size_t>(wpr) * 8, rows_per_part,
+                     parts_on_device, S->main.stream, [&](const void* src, int r0, int nr) {
                        return LaunchDbToTiles(static_cast<const u64*>(src), S->P.db_bytes,
                                               S->M, db_cols, base + r0, nr, S->dbL,
                                               S->db_tiles.as<int8_t>(), S->main.stream);
@@ -341,7 +342,8 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
       packed != nullptr ? 0 : S->M / static_cast<int>(pack_parts.size());
   for (size_t part = 0; packed == nullptr && part < pack_parts.size(); ++part) {
     const int base = static_cast<int>(part) * pk_rows_per_part;
-    er = UploadTiles(pack_parts[part], pk_rowbytes, pk_rows_per_part, false, S->main.stream,
+    er = UploadTiles(pack_parts[part], pk_rowbytes, pk_rows_per_part, parts_on_device,
+                     S->main.stream,
                      [&](const void* src, int r0, int nr) {
                        return LaunchPackToTiles(src, S->P.mat_bytes, S->M, pk_cols,
                                                 base + r0, nr, S->pkL,
@@ -877,6 +879,50 @@ Error CreateFromImage(const pirg_params* params, const uint64_t* db_cdata, int d
     th[oi] = t_vec_h + static_cast<size_t>(oi) * S->k * d;
   }
   er = BuildResident(S.get(), db, db_cols, {}, at, th, second_dim_a, &pack);
+  if (!er.ok()) return er;
+  *out = std::move(S);
+  return Ok();
+}
+
+Error CreateFromDevice(const pirg_params* params, const uint64_t* db_cdata, int db_cols,
+                       const void* pack, const uint64_t* a_tilde, const uint64_t* t_vec_h,
+                       const uint64_t* second_dim_a, std::unique_ptr<pirg_server>* out) {
+  std::unique_ptr<pirg_server> S(new pirg_server());
+  Error er = DeriveServer(*params, S.get());
+  if (!er.ok()) return er;
+  if (const char* env = std::getenv("PIRG_WIDE_MIN")) {
+    const int v = std::atoi(env);
+    if (v > 0) S->wide_min = v;
+  }
+  if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
+  const int ell = S->ell, t = S->t, d = S->d, lt = ell * t;
+  std::vector<int> rows(ell, t * d), cols(ell, db_cols), olen(ell, t);
+  const size_t wpr = S->P.db_bytes == 2 ? db_cols / 4 : db_cols / 8;
+  std::vector<const uint64_t*> db(ell);
+  for (int s = 0; s < ell; ++s) db[s] = db_cdata + static_cast<size_t>(s) * t * d * wpr;
+  pirg_preprocessed_view v{};
+  v.num_db_matrices = ell;
+  v.db_cdata = db.data();
+  v.db_rows = rows.data();
+  v.db_cols = cols.data();
+  v.num_outputs = ell;
+  v.outputs_len = olen.data();
+  v.second_dim_a_len = t > 1 ? 2 * S->e : 0;
+  er = ValidateView(S.get(), &v);
+  if (!er.ok()) return er;
+  er = CheckDevice();
+  if (!er.ok()) return er;
+  const size_t pk_rowbytes = static_cast<size_t>(S->k) * d * (S->P.mat_bytes == 4 ? 4 : 2);
+  std::vector<const void*> pk(ell);
+  for (int s = 0; s < ell; ++s) {
+    pk[s] = static_cast<const char*>(pack) + static_cast<size_t>(s) * t * d * pk_rowbytes;
+  }
+  std::vector<const uint64_t*> at(lt), th(lt);
+  for (int oi = 0; oi < lt; ++oi) {
+    at[oi] = a_tilde + static_cast<size_t>(oi) * d;
+    th[oi] = t_vec_h + static_cast<size_t>(oi) * S->k * d;
+  }
+  er = BuildResident(S.get(), db, db_cols, pk, at, th, second_dim_a, nullptr, true);
   if (!er.ok()) return er;
   *out = std::move(S);
   return Ok();

@@ -212,6 +212,12 @@ Error CreateFromImage(const pirg_params* params, const uint64_t* db_cdata, int d
                       const PackedPack& pack, const uint64_t* a_tilde, const uint64_t* t_vec_h,
                       const uint64_t* second_dim_a, std::unique_ptr<pirg_server>* out);
 
+// Server from material already in device memory (the combined matrices, as
+// CreateFromImage takes them, with the pack matrix as int32 / int16 rows).
+Error CreateFromDevice(const pirg_params* params, const uint64_t* db_cdata, int db_cols,
+                       const void* pack, const uint64_t* a_tilde, const uint64_t* t_vec_h,
+                       const uint64_t* second_dim_a, std::unique_ptr<pirg_server>* out);
+
 // Uploads rows [0, rows) of a host matrix in chunks and converts them to
 // tiles. `rowbytes` is the size of one host row.
 template <typename Convert>

@@ -572,6 +572,32 @@ def interpolate(row, d: int, t: int, p: int) -> np.ndarray:
     return out
 
 
+def compute_hint_matrix(db, db_bits: int, a) -> np.ndarray:
+    """ComputeHintMatrix (crypto/hint_computation.cc:78-160): db is rows x cols
+    (values below 2^db_bits), a is n x d; returns rows x d uint64."""
+    db = np.ascontiguousarray(db, dtype=np.uint16)
+    a = _u64(a)
+    rows, cols = db.shape
+    n, d = a.shape
+    out = np.empty((rows, d), dtype=np.uint64)
+    _call(lib().orc_compute_hint_matrix, db.ctypes.data_as(_u16p), rows, cols, db_bits,
+          _p(a), n, d, _p(out))
+    return out
+
+
+def sample_a_components(P: Params, public_seed: int):
+    """The a-components preprocess() draws from public_seed:
+    (first_dim_a, second_dim_a or None, pack_a)."""
+    s = sizes(P)
+    first = np.empty((s.S, s.d), dtype=np.uint64)
+    second = np.empty((2 * s.e, s.d), dtype=np.uint64) if s.t > 1 else None
+    pack_a = np.empty((2 * s.k, s.d), dtype=np.uint64)
+    cp = P.c()
+    _call(lib().orc_sample_a_components, ctypes.byref(cp), ctypes.c_uint64(public_seed),
+          _p(first), _p(second) if second is not None else None, _p(pack_a))
+    return first, second, pack_a
+
+
 def fill_db_u16(seed: int, n: int, p: int) -> np.ndarray:
     out = np.empty(n, dtype=np.uint16)
     lib().orc_fill_db_u16(ctypes.c_uint64(seed), out.ctypes.data_as(_u16p),

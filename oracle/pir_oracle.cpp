@@ -1819,6 +1819,39 @@ int orc_interpolate(const uint64_t* row, int d, int t, uint64_t p,
   ORC_GUARD_END(err, errlen)
 }
 
+int orc_compute_hint_matrix(const uint16_t* db, int rows, int cols, int db_bits,
+                            const uint64_t* a, int n, int d, uint64_t* out,
+                            char* err, size_t errlen) {
+  ORC_GUARD_BEGIN
+  if (rows <= 0) InvalidArg("Database matrix cannot be empty.");
+  if (n <= 0) InvalidArg("a_components cannot be empty.");
+  std::vector<uint16_t> m(db, db + static_cast<size_t>(rows) * cols);
+  std::vector<Poly> as = SplitPolys(a, n, d);
+  std::vector<u64> h = ComputeHintMatrix(m, rows, cols, as, db_bits);
+  std::memcpy(out, h.data(), sizeof(u64) * h.size());
+  ORC_GUARD_END(err, errlen)
+}
+
+int orc_sample_a_components(const orc_params* P, uint64_t public_seed,
+                            uint64_t* first_dim_a, uint64_t* second_dim_a,
+                            uint64_t* pack_a, char* err, size_t errlen) {
+  ORC_GUARD_BEGIN
+  Derived D = Derive(P);
+  SplitMix64 prng(public_seed);
+  auto put = [&](const std::vector<Poly>& v, uint64_t* dst) {
+    for (size_t j = 0; j < v.size(); ++j) {
+      std::memcpy(dst + j * D.d, v[j].data(), sizeof(u64) * D.d);
+    }
+  };
+  put(SampleAComponents(D.d, D.qmask, D.S, prng), first_dim_a);
+  if (D.t > 1) {
+    std::vector<Poly> sa = SampleAComponents(D.d, D.qmask, 2 * D.e, prng);
+    if (second_dim_a != nullptr) put(sa, second_dim_a);
+  }
+  put(SampleAComponents(D.d, D.qmask, 2 * D.k, prng), pack_a);
+  ORC_GUARD_END(err, errlen)
+}
+
 int orc_preprocess_matrix_pack(const orc_params* P, const uint64_t* a_is,
                                const uint64_t* w, void* pack,
                                uint64_t* a_tilde, uint64_t* t_vec_h,

@@ -189,6 +189,17 @@ int orc_preprocess(const orc_params* P, const uint16_t* db_raw,
                    uint64_t public_seed, uint64_t* db_cdata, void* pack,
                    uint64_t* a_tilde, uint64_t* t_vec_h,
                    uint64_t* second_dim_a, char* err, size_t errlen);
+/* ComputeHintMatrix<uint64_t, S, uint64_t> (hint_computation.cc:78-160) of a
+ * rows x cols DB (values given as uint16, db_bits = 8 * sizeof(S)) against n
+ * a-components (n x d); out rows x d. */
+int orc_compute_hint_matrix(const uint16_t* db, int rows, int cols, int db_bits,
+                            const uint64_t* a, int n, int d, uint64_t* out,
+                            char* err, size_t errlen);
+/* The a-components orc_preprocess draws from public_seed, in its order:
+ * first_dim_a (S x d), second_dim_a (2e x d, only when t > 1), pack_a (2k x d). */
+int orc_sample_a_components(const orc_params* P, uint64_t public_seed,
+                            uint64_t* first_dim_a, uint64_t* second_dim_a,
+                            uint64_t* pack_a, char* err, size_t errlen);
 /* PreprocessMatrixPack on explicit LWE a-vectors (d x d) and w polys
  * (2k x d). pack is d x k*d int32 (condensed words) or int16. */
 int orc_preprocess_matrix_pack(const orc_params* P, const uint64_t* a_is,
