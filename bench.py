@@ -20,7 +20,9 @@ Also reported, each on its own synthetic material:
     5000 requests/s through the continuous-batching loop; p50 / p99 latency
     and sustained requests/s beside the one-at-a-time GPU path;
   * packing: 512 MatrixPack jobs at d = 2048, q = 2^56 with LWE counts uniform
-    in 16..2048, one graph launch.
+    in 16..2048, one graph launch;
+  * preprocessing: PreprocessingServer::Preprocess of a seeded raw DB of the
+    headline shape (n = 2^20, d = 4096, t = 8) on the GPU, stage times.
 
 The CPU baseline is the oracle's ProcessResponse restatement (one thread,
 matrix loops vectorised) on the same material and request.
@@ -471,6 +473,70 @@ def run_packing(njobs: int, seed: int = 11, verify: int = 2):
             "bit_identical": exact}
 
 
+def run_preprocess(P: ia.PirParams, cpu: bool, seed: int = 5):
+    """PreprocessingServer::Preprocess of the whole benchmark DB on the GPU:
+    interpolation + transpose, hint matrix, PreprocessMatrixPack of the t
+    blocks. Timed with HIP events from the moment the raw DB is in HBM. The
+    CPU leg checks a sample against the oracle (interpolated row groups, hint
+    rows, one PreprocessMatrixPack block on the GPU's hint block) and times the
+    same oracle calls to extrapolate a one-thread CPU time."""
+    d, t, n, k = P.degree, P.interp_degree, P.num_entries, P.pack_num_digits
+    r = n // t
+    S = (r + d - 1) // d
+    rng = np.random.default_rng(seed)
+    raw = np.frombuffer(rng.bytes(n * d * 2), dtype=np.uint16).copy()
+    raw[raw >= P.plaintext_modulus] = 0
+    qmask = np.uint64((1 << P.log_q) - 1)
+    first = rng.integers(0, 1 << 63, size=(S, d), dtype=np.uint64) & qmask
+    pack_a = rng.integers(0, 1 << 63, size=(2 * k, d), dtype=np.uint64) & qmask
+    log(f"preprocess: raw DB {raw.nbytes / 2**30:.1f} GiB, {S} first-dim a-components")
+    t0 = time.perf_counter()
+    mat = ia.preprocess(P, raw, first, pack_a)
+    wall = time.perf_counter() - t0
+    st = mat["stage_ms"]
+    out = {"config": {"n": n, "d": d, "t": t, "ell": 1, "log_q": P.log_q,
+                      "pack_gadget": [P.pack_log_digit, k]},
+           "gpu_ms": st, "wall_s_incl_pcie": wall}
+    log(f"preprocess: GPU {st['total']:.0f} ms (interpolate {st['interpolate']:.0f}, hints "
+        f"{st['hints']:.0f}, pack {st['preprocess_pack']:.0f}); {wall:.1f} s with transfers")
+    if cpu:
+        import oracle as O
+        OP = O.Params(**{f: getattr(P, f) for f in O.Params.__dataclass_fields__})
+        T = mat["db"].view(np.uint16).reshape(t * d, -1)
+        ok = True
+        groups = [0, 1, r // 2, r - 1]
+        c0 = time.perf_counter()
+        for g in groups:
+            want = O.interpolate(raw[g * t * d:(g + 1) * t * d], d, t, P.plaintext_modulus)
+            ok &= bool(np.array_equal(T[:, g].astype(np.uint64), want))
+        interp_s = (time.perf_counter() - c0) / len(groups) * r
+        rows = [0, d + 7, t * d - 1]
+        c0 = time.perf_counter()
+        want_h = O.compute_hint_matrix(T[rows, :S * d], 16, first)
+        hint_s = (time.perf_counter() - c0) / len(rows) * t * d
+        b = t - 1
+        hint_block = ia.compute_hint_matrix(T[b * d:(b + 1) * d, :S * d], first)
+        ok &= bool(np.array_equal(ia.compute_hint_matrix(T[rows, :S * d], first), want_h))
+        c0 = time.perf_counter()
+        wp, wa, wt = O.preprocess_matrix_pack(OP, hint_block, pack_a)
+        pack_s = (time.perf_counter() - c0) * t
+        ok &= bool(np.array_equal(mat["pack"][b * d:(b + 1) * d], wp))
+        ok &= bool(np.array_equal(mat["a_tilde"][b], wa))
+        ok &= bool(np.array_equal(mat["t_vec_h"][b], wt))
+        total_s = interp_s + hint_s + pack_s
+        out["bit_identical"] = ok
+        out["cpu_baseline"] = {
+            "value": total_s, "unit": "s (extrapolated)", "cores": 1, "kind": "port",
+            "sample": f"oracle Interpolate of {len(groups)} row groups, ComputeHintMatrix of "
+                      f"{len(rows)} rows and PreprocessMatrixPack of 1 of {t} blocks, each "
+                      f"scaled to the full DB (interpolate {interp_s:.0f} s, hints "
+                      f"{hint_s:.0f} s, pack {pack_s:.0f} s)"}
+        out["gpu_over_cpu"] = total_s / (st["total"] * 1e-3)
+        log(f"preprocess: oracle sample identical {ok}; one-thread CPU estimate "
+            f"{total_s:.0f} s")
+    return out
+
+
 def load_traffic():
     """HBM bytes per launch of the dominant kernels, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when present."""
@@ -495,6 +561,7 @@ def main():
     ap.add_argument("--no-pack", action="store_true")
     ap.add_argument("--no-load", action="store_true")
     ap.add_argument("--no-vpir", action="store_true")
+    ap.add_argument("--no-preprocess", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=3)
     ap.add_argument("--serve-rates", default="100,500,1000,2000,5000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -605,6 +672,10 @@ def main():
         log(f"vPIR: single {vpir['single_qps']:.0f} q/s, batched {vpir['batched_qps']:.0f} q/s "
             f"(width {vpir['best_batch']}), exact {vpir.get('bit_identical')}")
 
+    preprocess = None
+    if not args.no_preprocess:
+        preprocess = run_preprocess(P, cpu=not args.no_cpu)
+
     serving = None
     if not args.no_serve:
         rates = [float(x) for x in args.serve_rates.split(",")]
@@ -647,6 +718,7 @@ def main():
         "load": load,
         "vpir": vpir,
         "codec": codec,
+        "preprocess": preprocess,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

