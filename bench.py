@@ -562,6 +562,8 @@ def main():
     ap.add_argument("--no-load", action="store_true")
     ap.add_argument("--no-vpir", action="store_true")
     ap.add_argument("--no-preprocess", action="store_true")
+    ap.add_argument("--only-preprocess", action="store_true",
+                    help="run only the preprocessing section (for rocprofv3)")
     ap.add_argument("--cpu-queries", type=int, default=3)
     ap.add_argument("--serve-rates", default="100,500,1000,2000,5000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -577,6 +579,9 @@ def main():
         sys.exit(2)
 
     P = ia.metrics_tool_params(n=args.num_entries, log_d=args.log_degree, t=args.interp_degree)
+    if args.only_preprocess:
+        print(json.dumps({"preprocess": run_preprocess(P, cpu=not args.no_cpu)}))
+        return
     d, t = P.degree, P.interp_degree
     r = P.num_entries // t
     M = t * d
