@@ -144,6 +144,22 @@ def test_preprocess_matches_oracle(name, ell, n):
     _check_material(_gpu_preprocess(P, db, 9), O.preprocess(P, db, public_seed=9))
 
 
+@pytest.mark.parametrize("t,ell,mat_bytes", [(4, 1, 4), (1, 2, 2), (8, 2, 4)])
+def test_preprocess_u8_db_matches_oracle(t, ell, mat_bytes):
+    # Db = uint8_t: 64-column SIMD padding of the condensed DB, 8-bit hint
+    # chunks, the plaintext modulus as a uint8_t (preprocessing_server.h:196).
+    P = O.Params(log_degree=6, log_q=56, log_q1=56, log_q2=56, variance=4,
+                 plaintext_modulus=251, pack_log_digit=8, pack_num_digits=7,
+                 eval_log_digit=8, eval_num_digits=7, num_entries=40 * t,
+                 interp_degree=t, entry_size_multiple=ell, db_bytes=1, mat_bytes=mat_bytes)
+    s = O.sizes(P)
+    rng = np.random.default_rng(t * 10 + ell)
+    db = rng.integers(0, 251, size=P.num_entries * s.d * s.ell, dtype=np.uint16)
+    first, _, pack_a = O.sample_a_components(P, 13)
+    got = ia.preprocess(to_gpu_params(P), db.astype(np.uint8), first, pack_a)
+    _check_material(got, O.preprocess(P, db, public_seed=13))
+
+
 @pytest.mark.parametrize("log_d,n,t,ell", [(8, 512, 2, 1), (8, 256, 4, 2), (6, 2048, 8, 1)])
 def test_preprocess_metrics_params_matches_oracle(log_d, n, t, ell):
     P = O.metrics_tool_params(n=n, log_d=log_d, t=t, ell=ell)
