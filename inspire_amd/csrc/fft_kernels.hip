@@ -170,10 +170,12 @@ __global__ void FinalizeCombineKernel(FinalizeArgs A) {
     const int q = static_cast<int>(g / A.M);
     const int row = static_cast<int>(g % A.M);  // (s*t + i)*d + m
     u64 bprime = A.corr_db[q];
+#pragma unroll 8
     for (int k = 0; k < A.ksplit_db; ++k) {
       bprime += A.db_partial[((long long)k * A.nq + q) * A.M + row];
     }
     u64 bpart = A.corr_pk[A.corr_rows > 0 ? row / A.corr_rows : q];
+#pragma unroll 8
     for (int k = 0; k < A.ksplit_pk; ++k) {
       bpart += A.pk_partial[((long long)k * A.nq + q) * A.M + row];
     }
@@ -212,12 +214,30 @@ __global__ __launch_bounds__(Geo<L>::NT) void DigitsSpectraKernel(
   const int a = job % nu;
   const int jj = (job / nu) % (2 * A.e);
   const long long qs = job / (nu * 2 * A.e);  // q*ell + s
-  const u64* src = (jj < A.e ? A.st_a : A.st_b) + qs * (N / 2) + t;
   const int digit = jj % A.e;
   const int to_shift = A.log_q - A.e * A.log_digit;
   u64 x[E / 2];
+  if (A.prev_step < 0) {
+    const u64* src = (jj < A.e ? A.st_a : A.st_b) + qs * (N / 2) + t;
 #pragma unroll
-  for (int k = 0; k < E / 2; ++k) x[k] = src[NT * k];
+    for (int k = 0; k < E / 2; ++k) x[k] = src[NT * k];
+  } else {
+    // The previous Horner step's result (pir_server.cc:141-146): the sum of
+    // its chunk-pair partials plus c_i, unreduced.
+    constexpr int d = N / 2;
+    const int which = jj < A.e ? 0 : 1;
+    const long long q = qs / A.ell;
+    const int s = static_cast<int>(qs % A.ell);
+    const long long oi = (long long)s * A.t + A.prev_step;
+    const u64* pp = A.ext_partial + ((qs * 2 + which) * A.npairs) * (long long)d + t;
+    const u64* cp = (which == 0 ? A.a_tilde + oi * d : A.ct_b + (q * A.ell * A.t + oi) * d) + t;
+#pragma unroll
+    for (int k = 0; k < E / 2; ++k) x[k] = cp[NT * k];
+    for (int pi = 0; pi < A.npairs; ++pi) {
+#pragma unroll
+      for (int k = 0; k < E / 2; ++k) x[k] += pp[(long long)pi * d + NT * k];
+    }
+  }
   double2 v[16];
 #pragma unroll
   for (int k = 0; k < E / 2; ++k) {

@@ -214,6 +214,14 @@ struct DigitsArgs {
   int nq, ell, d, e, log_q, log_digit;
   ChunkRule rule;    // u rule of the external product (digits)
   double2* dspec;    // [nq][ell][2e][nu][N]
+  // prev_step >= 0: the Horner state is not in st_a / st_b but is the
+  // previous step's result, formed here as ExtCombineKernel would: the sum of
+  // its pair partials plus c_prev_step (a_tilde / ct_b).
+  int prev_step;
+  const u64* ext_partial;  // [nq][ell][2][npairs][d]
+  int npairs, t;
+  const u64* a_tilde;      // [ell*t][d]
+  const u64* ct_b;         // [nq][ell*t][d]
 };
 
 struct ExtCombineArgs {

@@ -445,14 +445,16 @@ Error RunPreprocess(const pirg_params& P, const void* db_raw, const u64* first_a
   } guard{s};
   // ev[0] start, ev[1] interpolation done, ev[2] / ev[3] around each shard's
   // hint stage, ev[4] after its pack stage.
-  hipEvent_t ev[5];
-  for (auto& e : ev) PIRG_HIP(hipEventCreate(&e));
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   struct EventGuard {
     hipEvent_t* e;
     ~EventGuard() {
-      for (int i = 0; i < 5; ++i) (void)hipEventDestroy(e[i]);
+      for (int i = 0; i < 5; ++i) {
+        if (e[i] != nullptr) (void)hipEventDestroy(e[i]);
+      }
     }
   } eguard{ev};
+  for (auto& e : ev) PIRG_HIP(hipEventCreate(&e));
   FftCtx C;
   if (!(er = C.Init(d, s)).ok()) return er;
 

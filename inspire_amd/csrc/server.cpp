@@ -671,6 +671,15 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
       DA.log_digit = S->P.eval_log_digit;
       DA.rule = S->ext_u;
       DA.dspec = W.d_spec.as<double2>();
+      // After the first step the digits kernel forms the state itself from
+      // the previous step's partials; only the last step runs ExtCombine
+      // (which also switches the modulus).
+      DA.prev_step = i == S->t - 2 ? -1 : i + 1;
+      DA.ext_partial = W.ext_partial.as<u64>();
+      DA.npairs = S->ext_pairs.count;
+      DA.t = S->t;
+      DA.a_tilde = S->a_tilde.as<u64>();
+      DA.ct_b = W.ct_b.as<u64>();
       PIRG_HIP(LaunchDigitsSpectra(DA, S->F, S->tw_fwd.as<double2>(), st));
       PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(et),
                         reinterpret_cast<const double2* const*>(et + n_ext_cap),
@@ -694,7 +703,7 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
       EA.log_q = S->log_q;
       EA.log_q1 = S->P.log_q1;
       EA.log_q2 = S->P.log_q2;
-      PIRG_HIP(LaunchExtCombine(EA, st));
+      if (i == 0) PIRG_HIP(LaunchExtCombine(EA, st));
     }
   }
   PIRG_HIP(mark(slot.ev[3]));
