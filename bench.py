@@ -573,9 +573,8 @@ def main():
                          "256, a copy and a read calibration kernel")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the answer path runs on a single MI355X in this "
-              "build (multi-GPU sharding is described in DESIGN.md, not implemented).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the answer path runs on a single MI355X "
+              "(sharding across GPUs is not implemented).", file=sys.stderr)
         sys.exit(2)
 
     P = ia.metrics_tool_params(n=args.num_entries, log_d=args.log_degree, t=args.interp_degree)

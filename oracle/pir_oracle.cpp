@@ -1523,7 +1523,7 @@ int orc_inner_product_exact_w(const uint64_t* u, const uint64_t* v, int n, int d
   ORC_GUARD_END(err, errlen)
 }
 
-// The twiddle table of FftPlan (crypto/fft.h:71-80) as this build computes it
+// The twiddle table of FftPlan (crypto/fft.h:71-80) as the oracle computes it
 // (interleaved re, im).
 void orc_fft_twiddles(int n, double* fwd, double* bwd) {
   FftPlan plan(static_cast<size_t>(n));
