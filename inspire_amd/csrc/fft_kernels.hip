@@ -28,11 +28,11 @@ using fftr::Transform;
 // Forward spectra (crypto/polynomial_fft.cc:119-152): one workgroup per
 // (polynomial, chunk). Input j = t + NT*k is coefficient j for j < d, zero
 // padding above.
-template <int L>
-__global__ __launch_bounds__(Geo<L>::NT) void SpectraKernel(
+template <int L, bool H>
+__global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 4 : 1))) void SpectraKernel(
     const u64* __restrict__ src, int per_group, long long group_stride,
     ChunkRule rule, const double2* __restrict__ st, double2* __restrict__ dst) {
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) double lds[];
   constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
   const int t = threadIdx.x;
   const int job = blockIdx.x;
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void SpectraKernel(
   for (int k = 0; k < E / 2; ++k) v[InSlot<L>(k)] = make_double2(ChunkValue(rule, x[k], c), 0.0);
 #pragma unroll
   for (int k = E / 2; k < E; ++k) v[InSlot<L>(k)] = make_double2(0.0, 0.0);
-  Transform<L>(v, lds, st);
+  Transform<L, H>(v, lds, st);
   double2* out = dst + (long long)job * N + t;
 #pragma unroll
   for (int e = 0; e < E; ++e) out[NT * e] = v[OutSlot<L>(e)];
@@ -87,11 +87,11 @@ __device__ __forceinline__ void MarginMax(double local, unsigned long long* marg
 // read the same u spectra) take consecutive slots on one XCD (workgroup w runs
 // on XCD w % 8), so that group's spectra are fetched into one L2 rather than
 // into all eight; the grid is padded to whole rounds of 8 groups.
-template <int L>
-__global__ __launch_bounds__(Geo<L>::NT) void IpKernel(
+template <int L, bool H>
+__global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 4 : 1))) void IpKernel(
     IpJob jobs, int n_jobs, int group_jobs, int nterms, int nu, int nv, PairList pairs,
     const double2* __restrict__ st, unsigned long long* margin) {
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) double lds[];
   constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
   const int t = threadIdx.x;
   const int w = blockIdx.x;
@@ -108,22 +108,29 @@ __global__ __launch_bounds__(Geo<L>::NT) void IpKernel(
   for (int i = 0; i < nterms; ++i) {
     const double2* Ui = U + ((long long)i * nu + a) * N;
     const double2* Vi = V + ((long long)i * nv + b) * N;
-    double2 uu[E], vv[E];
+    // H: positions in runs of 4, since two workgroups share a CU and each
+    // keeps its loads in flight in a quarter of the registers; otherwise all
+    // E positions' loads at once.
+    constexpr int kRun = H ? 4 : E;
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      uu[k] = Ui[NT * k];
-      vv[k] = Vi[NT * k];
-    }
+    for (int k0 = 0; k0 < E; k0 += kRun) {
+      double2 uu[kRun], vv[kRun];
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const double pr = uu[k].x * vv[k].x - uu[k].y * vv[k].y;
-      const double pim = uu[k].x * vv[k].y + uu[k].y * vv[k].x;
-      double2& acc = v[InSlot<L>(k)];
-      acc.x = acc.x + pr;
-      acc.y = acc.y + pim;
+      for (int k = 0; k < kRun && k0 + k < E; ++k) {
+        uu[k] = Ui[NT * (k0 + k)];
+        vv[k] = Vi[NT * (k0 + k)];
+      }
+#pragma unroll
+      for (int k = 0; k < kRun && k0 + k < E; ++k) {
+        const double pr = uu[k].x * vv[k].x - uu[k].y * vv[k].y;
+        const double pim = uu[k].x * vv[k].y + uu[k].y * vv[k].x;
+        double2& acc = v[InSlot<L>(k0 + k)];
+        acc.x = acc.x + pr;
+        acc.y = acc.y + pim;
+      }
     }
   }
-  Transform<L>(v, lds, st);
+  Transform<L, H>(v, lds, st);
   // x / 2d of polynomial_fft.cc:184: 2d is a power of two, so the product by
   // its exact reciprocal is the same correctly rounded value.
   const double inv_n = 1.0 / static_cast<double>(N);
@@ -203,10 +210,10 @@ __global__ void FinalizeCombineKernel(FinalizeArgs A) {
 
 // ExternalProduct's decomposition (crypto/external_product.cc:60-72,
 // polynomial.cc:340-374) fused with the forward FFT of each digit chunk.
-template <int L>
-__global__ __launch_bounds__(Geo<L>::NT) void DigitsSpectraKernel(
+template <int L, bool H>
+__global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 4 : 1))) void DigitsSpectraKernel(
     DigitsArgs A, const double2* __restrict__ st) {
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) double lds[];
   constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT;
   const int t = threadIdx.x;
   const int nu = A.rule.num_chunks;
@@ -252,7 +259,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void DigitsSpectraKernel(
   }
 #pragma unroll
   for (int k = E / 2; k < E; ++k) v[InSlot<L>(k)] = make_double2(0.0, 0.0);
-  Transform<L>(v, lds, st);
+  Transform<L, H>(v, lds, st);
   double2* out = A.dspec + (long long)job * N + t;
 #pragma unroll
   for (int e2 = 0; e2 < E; ++e2) out[NT * e2] = v[OutSlot<L>(e2)];
@@ -261,11 +268,11 @@ __global__ __launch_bounds__(Geo<L>::NT) void DigitsSpectraKernel(
 
 // Forward spectra of DB row polynomials (ComputeHintMatrix,
 // hint_computation.cc:120-152): job = (row * polys + k) * nu + a.
-template <int L, typename Elem>
-__global__ __launch_bounds__(Geo<L>::NT) void DbSpectraKernel(
+template <int L, typename Elem, bool H>
+__global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 4 : 1))) void DbSpectraKernel(
     const Elem* __restrict__ db, long long stride, long long valid, int polys, ChunkRule rule,
     const double2* __restrict__ st, double2* __restrict__ dst) {
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) double lds[];
   constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
   const int t = threadIdx.x;
   const long long job = blockIdx.x;
@@ -283,7 +290,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void DbSpectraKernel(
   }
 #pragma unroll
   for (int k = E / 2; k < E; ++k) v[InSlot<L>(k)] = make_double2(0.0, 0.0);
-  Transform<L>(v, lds, st);
+  Transform<L, H>(v, lds, st);
   double2* out = dst + job * N + t;
 #pragma unroll
   for (int e = 0; e < E; ++e) out[NT * e] = v[OutSlot<L>(e)];
@@ -296,7 +303,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void DbSpectraKernel(
 template <int L>
 __global__ __launch_bounds__(Geo<L>::NT) void ChainSpectraKernel(
     ChainSpectraArgs A, const double2* __restrict__ st) {
-  extern __shared__ double2 lds[];
+  extern __shared__ __align__(16) double lds[];
   constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
   const int t = threadIdx.x;
   const int nu = A.ur.num_chunks, nv = A.vr.num_chunks;
@@ -364,7 +371,7 @@ __global__ __launch_bounds__(Geo<L>::NT) void ChainSpectraKernel(
   }
 #pragma unroll
   for (int k = E / 2; k < E; ++k) v[InSlot<L>(k)] = make_double2(0.0, 0.0);
-  Transform<L>(v, lds, st);
+  Transform<L, false>(v, lds, st);
   out += t;
 #pragma unroll
   for (int e = 0; e < E; ++e) out[NT * e] = v[OutSlot<L>(e)];
@@ -440,28 +447,32 @@ hipError_t DispatchLogN(int L, Args&&... args) {
   }
 }
 
+// Launches of at least this many workgroups use the two-per-CU form of a
+// transform kernel (throughput); smaller ones the one-per-CU form, whose
+// transform has half the barriers and preloaded twiddles (latency).
+constexpr long long kHalfMinBlocks = 256;
+
 template <int L>
 struct Configure {
+  template <typename K>
+  static hipError_t Set(K* kernel, bool half) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               Geo<L>::LdsBytes(half));
+  }
   static hipError_t Run() {
-    const int bytes = static_cast<int>(Geo<L>::kLds * sizeof(double2));
     hipError_t e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(SpectraKernel<L>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(IpKernel<L>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(DbSpectraKernel<L, uint16_t>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(DbSpectraKernel<L, uint8_t>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(ChainSpectraKernel<L>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(DigitsSpectraKernel<L>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if ((e = Set(SpectraKernel<L, false>, false)) != hipSuccess) return e;
+    if ((e = Set(SpectraKernel<L, true>, true)) != hipSuccess) return e;
+    if ((e = Set(IpKernel<L, false>, false)) != hipSuccess) return e;
+    if ((e = Set(IpKernel<L, true>, true)) != hipSuccess) return e;
+    if ((e = Set(DbSpectraKernel<L, uint16_t, false>, false)) != hipSuccess) return e;
+    if ((e = Set(DbSpectraKernel<L, uint16_t, true>, true)) != hipSuccess) return e;
+    if ((e = Set(DbSpectraKernel<L, uint8_t, false>, false)) != hipSuccess) return e;
+    if ((e = Set(DbSpectraKernel<L, uint8_t, true>, true)) != hipSuccess) return e;
+    if ((e = Set(ChainSpectraKernel<L>, false)) != hipSuccess) return e;
+    if ((e = Set(DigitsSpectraKernel<L, false>, false)) != hipSuccess) return e;
+    return Set(DigitsSpectraKernel<L, true>, true);
   }
 };
 
@@ -470,9 +481,16 @@ struct SpectraL {
   static hipError_t Run(const u64* src, int npolys, int per_group, long long group_stride,
                         const ChunkRule& rule, const double2* st, double2* dst,
                         hipStream_t s) {
-    hipLaunchKernelGGL(SpectraKernel<L>, dim3(npolys * rule.num_chunks), dim3(Geo<L>::NT),
-                       Geo<L>::kLds * sizeof(double2), s, src, per_group, group_stride, rule,
-                       st, dst);
+    const long long blocks = static_cast<long long>(npolys) * rule.num_chunks;
+    if (blocks >= kHalfMinBlocks) {
+      hipLaunchKernelGGL((SpectraKernel<L, true>), dim3(blocks), dim3(Geo<L>::NT),
+                         Geo<L>::LdsBytes(true), s, src, per_group, group_stride, rule, st,
+                         dst);
+    } else {
+      hipLaunchKernelGGL((SpectraKernel<L, false>), dim3(blocks), dim3(Geo<L>::NT),
+                         Geo<L>::LdsBytes(false), s, src, per_group, group_stride, rule, st,
+                         dst);
+    }
     return hipGetLastError();
   }
 };
@@ -487,28 +505,42 @@ struct IpL {
     const long long groups = (n_out + outs_per_group - 1) / outs_per_group;
     const long long blocks = (groups + 7) / 8 * 8 * group_jobs;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(IpKernel<L>, dim3(static_cast<unsigned>(blocks)), dim3(Geo<L>::NT),
-                       Geo<L>::kLds * sizeof(double2), s, jobs, static_cast<int>(n_jobs),
-                       static_cast<int>(group_jobs), nterms, nu, nv, pairs, st, margin);
+    if (n_jobs >= kHalfMinBlocks) {
+      hipLaunchKernelGGL((IpKernel<L, true>), dim3(static_cast<unsigned>(blocks)),
+                         dim3(Geo<L>::NT), Geo<L>::LdsBytes(true), s, jobs,
+                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), nterms, nu, nv,
+                         pairs, st, margin);
+    } else {
+      hipLaunchKernelGGL((IpKernel<L, false>), dim3(static_cast<unsigned>(blocks)),
+                         dim3(Geo<L>::NT), Geo<L>::LdsBytes(false), s, jobs,
+                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), nterms, nu, nv,
+                         pairs, st, margin);
+    }
     return hipGetLastError();
   }
 };
 
 template <int L>
 struct DbSpectraL {
+  template <typename Elem, bool H>
+  static void Go(const void* db, long long stride, long long valid, long long jobs, int polys,
+                 const ChunkRule& rule, const double2* st, double2* dst, hipStream_t s) {
+    hipLaunchKernelGGL((DbSpectraKernel<L, Elem, H>), dim3(static_cast<unsigned>(jobs)),
+                       dim3(Geo<L>::NT), Geo<L>::LdsBytes(H), s, static_cast<const Elem*>(db),
+                       stride, valid, polys, rule, st, dst);
+  }
   static hipError_t Run(const void* db, int elem_bytes, long long stride, long long valid,
                         int rows, int polys, const ChunkRule& rule, const double2* st,
                         double2* dst, hipStream_t s) {
     const long long jobs = static_cast<long long>(rows) * polys * rule.num_chunks;
     if (jobs > 0x7fffffffLL) return hipErrorInvalidValue;
+    const bool half = jobs >= kHalfMinBlocks;
     if (elem_bytes == 2) {
-      hipLaunchKernelGGL((DbSpectraKernel<L, uint16_t>), dim3(static_cast<unsigned>(jobs)),
-                         dim3(Geo<L>::NT), Geo<L>::kLds * sizeof(double2), s,
-                         static_cast<const uint16_t*>(db), stride, valid, polys, rule, st, dst);
+      if (half) Go<uint16_t, true>(db, stride, valid, jobs, polys, rule, st, dst, s);
+      else Go<uint16_t, false>(db, stride, valid, jobs, polys, rule, st, dst, s);
     } else if (elem_bytes == 1) {
-      hipLaunchKernelGGL((DbSpectraKernel<L, uint8_t>), dim3(static_cast<unsigned>(jobs)),
-                         dim3(Geo<L>::NT), Geo<L>::kLds * sizeof(double2), s,
-                         static_cast<const uint8_t*>(db), stride, valid, polys, rule, st, dst);
+      if (half) Go<uint8_t, true>(db, stride, valid, jobs, polys, rule, st, dst, s);
+      else Go<uint8_t, false>(db, stride, valid, jobs, polys, rule, st, dst, s);
     } else {
       return hipErrorInvalidValue;
     }
@@ -522,7 +554,7 @@ struct ChainSpectraL {
     const int jobs = A.n_dig * A.k * A.ur.num_chunks + A.n_wtypes * A.k * A.vr.num_chunks;
     if (jobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(ChainSpectraKernel<L>, dim3(jobs), dim3(Geo<L>::NT),
-                       Geo<L>::kLds * sizeof(double2), s, A, st);
+                       Geo<L>::LdsBytes(false), s, A, st);
     return hipGetLastError();
   }
 };
@@ -531,8 +563,13 @@ template <int L>
 struct DigitsL {
   static hipError_t Run(const DigitsArgs& A, const double2* st, hipStream_t s) {
     const int jobs = A.nq * A.ell * 2 * A.e * A.rule.num_chunks;
-    hipLaunchKernelGGL(DigitsSpectraKernel<L>, dim3(jobs), dim3(Geo<L>::NT),
-                       Geo<L>::kLds * sizeof(double2), s, A, st);
+    if (jobs >= kHalfMinBlocks) {
+      hipLaunchKernelGGL((DigitsSpectraKernel<L, true>), dim3(jobs), dim3(Geo<L>::NT),
+                         Geo<L>::LdsBytes(true), s, A, st);
+    } else {
+      hipLaunchKernelGGL((DigitsSpectraKernel<L, false>), dim3(jobs), dim3(Geo<L>::NT),
+                         Geo<L>::LdsBytes(false), s, A, st);
+    }
     return hipGetLastError();
   }
 };

@@ -48,8 +48,12 @@ struct Geo {
   static constexpr int E = 1 << LE;
   static constexpr int NT = N / E;           // threads per transform
   static constexpr int NS = (L + 3) / 4;     // super-stages
-  // LDS entries (double2) of the exchanges: one pad slot per N/16 positions.
-  static constexpr int kLds = NS > 1 ? N + 16 : 1;
+  // LDS bytes of the exchanges, one pad slot per N/16 positions: a double2
+  // per position (one transform per CU), or a double per position with the
+  // real and imaginary parts exchanged in turn (two transforms per CU).
+  static constexpr int kLdsFull = NS > 1 ? (N + 16) * 16 : 16;
+  static constexpr int kLdsHalf = NS > 1 ? (N + 16) * 8 : 16;
+  static constexpr int LdsBytes(bool half) { return half ? kLdsHalf : kLdsFull; }
 };
 
 template <int L, int K>
@@ -113,8 +117,10 @@ __device__ __forceinline__ void Butterfly(double2& x, double2& y, const double2 
   y = make_double2(ux - vr, uy - vi);
 }
 
-// Stages 4K+1 .. 4K+P on every group the thread holds.
-template <int L, int K>
+// Stages 4K+1 .. 4K+P on every group the thread holds. H (two transforms
+// per CU): a stage's twiddles are loaded just before its butterflies, to
+// keep few registers live; otherwise all of them up front.
+template <int L, int K, bool H>
 __device__ __forceinline__ void Compute(double2 (&v)[16], int t,
                                         const double2* __restrict__ st) {
   using S = Super<L, K>;
@@ -123,52 +129,91 @@ __device__ __forceinline__ void Compute(double2 (&v)[16], int t,
   for (int i = 0; i < S::G; ++i) {
     int glo = 0;
     if constexpr (K > 0) glo = (t + Geo<L>::NT * i) & ((1 << S0) - 1);
-    double2 w[(1 << P) - 1];
+    if constexpr (H) {
 #pragma unroll
-    for (int r = 0; r < P; ++r) {
+      for (int r = 0; r < P; ++r) {
+        double2 w[8];
 #pragma unroll
-      for (int q = 0; q < (1 << r); ++q) {
-        w[(1 << r) - 1 + q] = st[(1 << (S0 + r)) - 1 + glo + (q << S0)];
+        for (int q = 0; q < (1 << r); ++q) {
+          w[q] = st[(1 << (S0 + r)) - 1 + glo + (q << S0)];
+        }
+#pragma unroll
+        for (int m = 0; m < (1 << P); ++m) {
+          if (m & (1 << r)) continue;
+          Butterfly(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)],
+                    w[m & ((1 << r) - 1)]);
+        }
       }
-    }
+    } else {
+      double2 w[(1 << P) - 1];
 #pragma unroll
-    for (int r = 0; r < P; ++r) {
+      for (int r = 0; r < P; ++r) {
 #pragma unroll
-      for (int m = 0; m < (1 << P); ++m) {
-        if (m & (1 << r)) continue;
-        Butterfly(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)],
-                  w[(1 << r) - 1 + (m & ((1 << r) - 1))]);
+        for (int q = 0; q < (1 << r); ++q) {
+          w[(1 << r) - 1 + q] = st[(1 << (S0 + r)) - 1 + glo + (q << S0)];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < P; ++r) {
+#pragma unroll
+        for (int m = 0; m < (1 << P); ++m) {
+          if (m & (1 << r)) continue;
+          Butterfly(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)],
+                    w[(1 << r) - 1 + (m & ((1 << r) - 1))]);
+        }
       }
     }
   }
 }
 
-template <int L, int K>
-__device__ __forceinline__ void Run(double2 (&v)[16], double2* lds,
-                                    const double2* __restrict__ st, int t) {
+// From the register layout of super-stage K to that of K + 1 through LDS:
+// whole double2 values, or (H) the real parts first, then the imaginary
+// parts, in half the LDS.
+template <int L, int K, bool H>
+__device__ __forceinline__ void Exchange(double2 (&v)[16], void* lds_raw, int t) {
   constexpr int E = Geo<L>::E;
-  if constexpr (K > 0) {
+  if constexpr (K > 0) __syncthreads();  // the previous exchange's reads are done
+  if constexpr (H) {
+    double* lds = static_cast<double*>(lds_raw);
 #pragma unroll
-    for (int idx = 0; idx < E; ++idx) v[idx] = lds[LdsSlot<L>(Position<L, K>(t, idx))];
-  }
-  Compute<L, K>(v, t, st);
-  if constexpr (K + 1 < Geo<L>::NS) {
-    if constexpr (K > 0) __syncthreads();  // every thread has read its inputs
+    for (int idx = 0; idx < E; ++idx) lds[LdsSlot<L>(Position<L, K>(t, idx))] = v[idx].x;
+    __syncthreads();
+#pragma unroll
+    for (int idx = 0; idx < E; ++idx) v[idx].x = lds[LdsSlot<L>(Position<L, K + 1>(t, idx))];
+    __syncthreads();
+#pragma unroll
+    for (int idx = 0; idx < E; ++idx) lds[LdsSlot<L>(Position<L, K>(t, idx))] = v[idx].y;
+    __syncthreads();
+#pragma unroll
+    for (int idx = 0; idx < E; ++idx) v[idx].y = lds[LdsSlot<L>(Position<L, K + 1>(t, idx))];
+  } else {
+    double2* lds = static_cast<double2*>(lds_raw);
 #pragma unroll
     for (int idx = 0; idx < E; ++idx) lds[LdsSlot<L>(Position<L, K>(t, idx))] = v[idx];
     __syncthreads();
-    Run<L, K + 1>(v, lds, st, t);
+#pragma unroll
+    for (int idx = 0; idx < E; ++idx) v[idx] = lds[LdsSlot<L>(Position<L, K + 1>(t, idx))];
+  }
+}
+
+template <int L, int K, bool H>
+__device__ __forceinline__ void Run(double2 (&v)[16], void* lds,
+                                    const double2* __restrict__ st, int t) {
+  Compute<L, K, H>(v, t, st);
+  if constexpr (K + 1 < Geo<L>::NS) {
+    Exchange<L, K, H>(v, lds, t);
+    Run<L, K + 1, H>(v, lds, st, t);
   }
 }
 
 // The whole transform. On entry v[InSlot(k)] holds input t + NT * k and the
-// LDS buffer (Geo<L>::kLds entries) is free; on return v[OutSlot(e)] holds
-// output position t + NT * e. Every thread of the workgroup (blockDim.x ==
-// NT) must call it.
-template <int L>
-__device__ __forceinline__ void Transform(double2 (&v)[16], double2* lds,
+// LDS buffer (Geo<L>::LdsBytes(H) bytes) is free; on return v[OutSlot(e)]
+// holds output position t + NT * e. Every thread of the workgroup
+// (blockDim.x == NT) must call it.
+template <int L, bool H>
+__device__ __forceinline__ void Transform(double2 (&v)[16], void* lds,
                                           const double2* __restrict__ st) {
-  Run<L, 0>(v, lds, st, static_cast<int>(threadIdx.x));
+  Run<L, 0, H>(v, lds, st, static_cast<int>(threadIdx.x));
 }
 
 }  // namespace fftr
