@@ -136,7 +136,26 @@ Error InterpolateRows(const void* raw, int elem_bytes, long long groups, int d, 
     return Invalid("a is not invertible modulo m (not coprime).");
   }
   const int log_t = Log2i(t);
-  // Stages alternate between `scratch` and `out` so the last lands in `out`.
+  if (t <= 16) {
+    InterpStage A{};
+    A.src = raw;
+    A.dst = out;
+    A.elem_bytes = elem_bytes;
+    A.groups = groups;
+    A.d = d;
+    A.log_d = Log2i(d);
+    A.t = t;
+    A.log_t = log_t;
+    A.first = 1;
+    A.ell = ell;
+    A.shard = shard;
+    A.p = p;
+    A.t_inv = t_inv;
+    PIRG_HIP(LaunchInterpFused(A, s));
+    return Ok();
+  }
+  // Larger t: one launch per stage, alternating between `scratch` and `out`
+  // so the last lands in `out`.
   void* bufs[2] = {scratch, out};
   int cur = (log_t % 2 == 1) ? 1 : 0;
   const void* src = raw;

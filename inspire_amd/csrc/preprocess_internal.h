@@ -25,6 +25,9 @@ struct InterpStage {
   uint32_t p, t_inv;
 };
 hipError_t LaunchInterpStage(const InterpStage& A, hipStream_t s);
+// All stages at once for t in {2, 4, 8, 16} (first and t_inv as for the
+// first and last stage above; len is unused).
+hipError_t LaunchInterpFused(const InterpStage& A, hipStream_t s);
 
 // dst[x][g] = src[g * src_stride + x] for x < cols, g < rows (zero for
 // rows <= g < dst_cols): InterpolateAndTranspose's transpose

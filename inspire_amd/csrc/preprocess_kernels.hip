@@ -67,14 +67,96 @@ __global__ __launch_bounds__(256) void InterpStageKernel(InterpStage A) {
       v = src[(g * t + hi) * d + sc];
     }
     if (neg) v = v == 0 ? 0 : p - v;
-    uint32_t nu = (u + v) % p;
-    uint32_t nv = (u + p - v) % p;
+    uint32_t nu = u + v;
+    if (nu >= p) nu -= p;
+    uint32_t nv = u + p - v;
+    if (nv >= p) nv -= p;
     if (A.t_inv != 0) {
       nu = static_cast<uint32_t>(static_cast<uint64_t>(nu) * A.t_inv % p);
       nv = static_cast<uint32_t>(static_cast<uint64_t>(nv) * A.t_inv % p);
     }
     dst[(g * t + lo) * d + r] = static_cast<E>(nu);
     dst[(g * t + hi) * d + r] = static_cast<E>(nv);
+  }
+}
+
+// All log2 t stages in registers for t <= 16. The stage-len twiddles are
+// X^power with power a multiple of R0 = 2d/t, and X^power only moves a
+// coefficient within its class {c0 + m R0}. Thread e owns class c0 of row
+// group g: t polynomials x t/2 members, so every butterfly and rotation is
+// a register operation fixed at compile time.
+template <typename E, int T>
+__global__ __launch_bounds__(256) void InterpFusedKernel(InterpStage A) {
+  constexpr int HALF = T / 2;
+  constexpr int LOG_T = T == 2 ? 1 : T == 4 ? 2 : T == 8 ? 3 : 4;
+  const int d = A.d;
+  const int r0 = 2 * d / T;
+  const long long total = A.groups * r0;
+  const E* src = static_cast<const E*>(A.src);
+  E* dst = static_cast<E*>(A.dst);
+  const uint32_t p = A.p;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total;
+       e += static_cast<long long>(gridDim.x) * 256) {
+    const long long g = e / r0;
+    const int c0 = static_cast<int>(e - g * r0);
+    uint32_t v[T][HALF];
+#pragma unroll
+    for (int x = 0; x < T; ++x) {
+      // Position x holds record bit_rev(x) of the group (:165-176).
+      const long long rec = g * T + Brev(x, LOG_T);
+      const E* row = src + (rec * A.ell + A.shard) * d + c0;
+#pragma unroll
+      for (int m = 0; m < HALF; ++m) v[x][m] = static_cast<uint32_t>(row[m * r0]) % p;
+    }
+#pragma unroll
+    for (int len = 2; len <= T; len <<= 1) {
+      const int half = len / 2;
+      // power = j * (2d - 2d/len) mod 2d, in units of r0 (2d = T r0).
+      const int stepu = (T - T / len) % T;
+#pragma unroll
+      for (int i = 0; i < T; i += len) {
+#pragma unroll
+        for (int j = 0; j < half; ++j) {
+          int pu = (j * stepu) % T;
+          bool neg_all = false;
+          if (pu >= HALF) {
+            neg_all = true;
+            pu -= HALF;
+          }
+          uint32_t rot[HALF];
+#pragma unroll
+          for (int m = 0; m < HALF; ++m) {
+            uint32_t val;
+            bool neg = neg_all;
+            if (m < pu) {
+              val = v[i + j + half][m - pu + HALF];
+              neg = !neg;
+            } else {
+              val = v[i + j + half][m - pu];
+            }
+            rot[m] = (neg && val != 0) ? p - val : val;
+          }
+#pragma unroll
+          for (int m = 0; m < HALF; ++m) {
+            const uint32_t u = v[i + j][m];
+            uint32_t a = u + rot[m];
+            if (a >= p) a -= p;
+            uint32_t b = u + p - rot[m];
+            if (b >= p) b -= p;
+            v[i + j][m] = a;
+            v[i + j + half][m] = b;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < T; ++x) {
+      E* row = dst + (g * T + x) * d + c0;
+#pragma unroll
+      for (int m = 0; m < HALF; ++m) {
+        row[m * r0] = static_cast<E>(static_cast<uint64_t>(v[x][m]) * A.t_inv % p);
+      }
+    }
   }
 }
 
@@ -267,6 +349,31 @@ hipError_t LaunchInterpStage(const InterpStage& A, hipStream_t s) {
   } else {
     return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t LaunchInterpFused(const InterpStage& A, hipStream_t s) {
+  if (A.d < 1 || (1 << A.log_d) != A.d || A.p == 0 || 2 * A.d < A.t) return hipErrorInvalidValue;
+  const long long total = A.groups * (2LL * A.d / A.t);
+  const dim3 grid(GridBlocks(total)), block(256);
+#define PIRG_INTERP_CASE(TT)                                                           \
+  case TT:                                                                             \
+    if (A.elem_bytes == 2) {                                                           \
+      hipLaunchKernelGGL((InterpFusedKernel<uint16_t, TT>), grid, block, 0, s, A);     \
+    } else if (A.elem_bytes == 1) {                                                    \
+      hipLaunchKernelGGL((InterpFusedKernel<uint8_t, TT>), grid, block, 0, s, A);      \
+    } else {                                                                           \
+      return hipErrorInvalidValue;                                                     \
+    }                                                                                  \
+    break;
+  switch (A.t) {
+    PIRG_INTERP_CASE(2)
+    PIRG_INTERP_CASE(4)
+    PIRG_INTERP_CASE(8)
+    PIRG_INTERP_CASE(16)
+    default: return hipErrorInvalidValue;
+  }
+#undef PIRG_INTERP_CASE
   return hipGetLastError();
 }
 
