@@ -447,7 +447,6 @@ Error RunPreprocess(const pirg_params& P, const void* db_raw, const u64* first_a
   pirg_server S;
   Error er = CheckParams(P, &S);
   if (!er.ok()) return er;
-  if (!(er = CheckDevice()).ok()) return er;
   const int d = S.d, t = S.t, ell = S.ell, k = S.k, r = S.r;
   const int eb = P.db_bytes;
   const int nfirst = (r + d - 1) / d;  // NumFirstDimSamples: PaddedCols / d
@@ -455,6 +454,7 @@ Error RunPreprocess(const pirg_params& P, const void* db_raw, const u64* first_a
   const int K = (r + simd - 1) / simd * simd;
   const uint32_t p = DbModulus(P);
   if (t > 1 && (p % 2 == 0)) return Invalid("plaintext_modulus must be odd.");
+  if (!(er = CheckDevice()).ok()) return er;
   M->K = K;
   hipStream_t s = nullptr;
   PIRG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -599,6 +599,9 @@ int pirg_interpolate(const void* rows, int db_bytes, long long nrows, int d, int
           errlen);
     }
     if (nrows < 0) return Report(Invalid("nrows must be non-negative."), err, errlen);
+    if (nrows > 0 && (rows == nullptr || out == nullptr)) {
+      return Report(Invalid("rows and out must not be null."), err, errlen);
+    }
     const size_t bytes = static_cast<size_t>(nrows) * t * d * db_bytes;
     if (t == 1 || nrows == 0) {
       if (bytes > 0) std::memcpy(out, rows, bytes);
@@ -640,6 +643,9 @@ int pirg_compute_hint_matrix(const void* db, int db_bytes, int rows, int cols,
                     err, errlen);
     }
     if (db_bytes != 1 && db_bytes != 2) return Report(Invalid("db_bytes must be 1 or 2."), err, errlen);
+    if (db == nullptr || a_components == nullptr || out == nullptr) {
+      return Report(Invalid("db, a_components and out must not be null."), err, errlen);
+    }
     if (!Pow2(d) || d < 2 || Log2i(d) > kMaxLogDegree) {
       return Report(Invalid("d must be a power of two in [2, 4096] on the GPU path."), err,
                     errlen);
@@ -670,6 +676,10 @@ int pirg_preprocess_matrix_pack(const pirg_params* params, int nblocks, const ui
                                 const uint64_t* w, void* pack, uint64_t* a_tilde,
                                 uint64_t* t_vec_h, char* err, size_t errlen) {
   try {
+    if (params == nullptr || a_is == nullptr || w == nullptr || pack == nullptr ||
+        a_tilde == nullptr || t_vec_h == nullptr) {
+      return Report(Invalid("null argument."), err, errlen);
+    }
     pirg_server S;
     Error er = CheckParams(*params, &S);
     if (!er.ok()) return Report(er, err, errlen);
@@ -713,6 +723,10 @@ int pirg_preprocess(const pirg_params* params, const void* db_raw, const uint64_
                     uint64_t* a_tilde, uint64_t* t_vec_h, double* stage_ms, char* err,
                     size_t errlen) {
   try {
+    if (params == nullptr || db_raw == nullptr || first_dim_a == nullptr || pack_a == nullptr) {
+      return Report(Invalid("params, db_raw, first_dim_a and pack_a must not be null."), err,
+                    errlen);
+    }
     Material M;
     StageTimes T;
     Error er = RunPreprocess(*params, db_raw, first_dim_a, pack_a, &M, &T);
@@ -734,6 +748,15 @@ int pirg_server_create_preprocessed(const pirg_params* params, const void* db_ra
                                     const uint64_t* pack_a, double* stage_ms, pirg_server** out,
                                     char* err, size_t errlen) {
   try {
+    if (params == nullptr || db_raw == nullptr || first_dim_a == nullptr || pack_a == nullptr ||
+        out == nullptr) {
+      return Report(Invalid("params, db_raw, first_dim_a, pack_a and out must not be null."),
+                    err, errlen);
+    }
+    if (params->interp_degree > 1 && second_dim_a == nullptr) {
+      return Report(Invalid("second_dim_a must not be null when interpolation_degree > 1."), err,
+                    errlen);
+    }
     Material M;
     StageTimes T;
     Error er = RunPreprocess(*params, db_raw, first_dim_a, pack_a, &M, &T);

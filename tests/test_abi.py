@@ -205,3 +205,63 @@ def test_proto_loader_checks_before_the_device(tmp_path):
     with pytest.raises(ia.PirError) as ei:
         ia.PirServer.create_from_directory(gp, str(tmp_path / "missing"))
     assert ei.value.code == 5
+
+
+# ---- Offline preprocessing: argument checks run before any device call ----
+
+def test_preprocessing_stage_checks_before_the_device():
+    rows = np.zeros(64, dtype=np.uint16)
+    for args, msg in [((rows, 12, 2, 257), "ring_degree must be a power of two."),
+                      ((rows, 16, 3, 257), "interpolation_degree must be a power of two."),
+                      ((rows, 2, 8, 257), "less than or equal to 2 * ring_degree"),
+                      ((rows, 16, 4, 256), "plaintext_modulus must be odd.")]:
+        with pytest.raises(ia.PirError) as ei:
+            ia.interpolate(*args)
+        assert ei.value.code == ia.INVALID_ARGUMENT and msg in ei.value.message
+    # t == 1 is the identity (db_row_interpolation.cc:141-143) and needs no device.
+    np.testing.assert_array_equal(ia.interpolate(np.arange(8, dtype=np.uint16), 8, 1, 257),
+                                  np.arange(8))
+    a = np.zeros((1, 4), dtype=np.uint64)
+    for db, comps, msg in [(np.zeros((0, 4), np.uint16), a, "Database matrix cannot be empty."),
+                           (np.zeros((2, 4), np.uint16), np.zeros((0, 4), np.uint64),
+                            "a_components cannot be empty."),
+                           (np.zeros((2, 3), np.uint16), a, "Database columns must match d")]:
+        with pytest.raises(ia.PirError) as ei:
+            ia.compute_hint_matrix(db, comps)
+        assert ei.value.code == ia.INVALID_ARGUMENT and msg in ei.value.message
+
+
+def test_preprocess_checks_before_the_device():
+    P = O.metrics_tool_params(n=512, log_d=8, t=2)
+    s = O.sizes(P)
+    db = np.zeros(P.num_entries * s.d, dtype=np.uint16)
+    first, second, pack_a = O.sample_a_components(P, 1)
+    bad = ia.PirParams(**{**{f: getattr(P, f) for f in ia.PirParams.__dataclass_fields__},
+                          "plaintext_modulus": 65536})
+    with pytest.raises(ia.PirError) as ei:
+        ia.preprocess(bad, db, first, pack_a)
+    assert ei.value.message == "plaintext_modulus must be odd."
+    bad = ia.PirParams(**{**{f: getattr(P, f) for f in ia.PirParams.__dataclass_fields__},
+                          "num_entries": 513})
+    with pytest.raises(ia.PirError) as ei:
+        ia.preprocess(bad, np.zeros(513 * s.d, np.uint16), first, pack_a)
+    assert "must divide num_entries" in ei.value.message
+    # Null arguments through the C ABI.
+    err = ctypes.create_string_buffer(256)
+    cp = to_gpu_params(P).c()
+    rc = ia.lib().pirg_preprocess(ctypes.byref(cp), None, None, None, None, None, None, None,
+                                  None, None, err, ctypes.c_size_t(len(err)))
+    assert rc == ia.INVALID_ARGUMENT and b"must not be null" in err.value
+
+
+def test_preprocess_without_device_fails_loudly():
+    """Valid arguments either preprocess (GPU present) or report UNAVAILABLE;
+    there is no CPU fallback."""
+    P = O.metrics_tool_params(n=512, log_d=8, t=2)
+    s = O.sizes(P)
+    db = np.zeros(P.num_entries * s.d, dtype=np.uint16)
+    first, second, pack_a = O.sample_a_components(P, 1)
+    try:
+        ia.preprocess(to_gpu_params(P), db, first, pack_a)
+    except ia.PirError as e:
+        assert e.code == ia.UNAVAILABLE
