@@ -87,10 +87,13 @@ __device__ __forceinline__ void MarginMax(double local, unsigned long long* marg
 // read the same u spectra) take consecutive slots on one XCD (workgroup w runs
 // on XCD w % 8), so that group's spectra are fetched into one L2 rather than
 // into all eight; the grid is padded to whole rounds of 8 groups.
-template <int L, bool H>
+//
+// FA: the accumulated spectra come ready from MacKernel (acc, one N-point
+// array per job) and the workgroup only inverts and rounds.
+template <int L, bool H, bool FA>
 __global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 4 : 1))) void IpKernel(
     IpJob jobs, int n_jobs, int group_jobs, int nterms, int nu, int nv, PairList pairs,
-    const double2* __restrict__ st, unsigned long long* margin) {
+    const double2* __restrict__ st, unsigned long long* margin, const double2* __restrict__ acc) {
   extern __shared__ __align__(16) double lds[];
   constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
   const int t = threadIdx.x;
@@ -103,6 +106,11 @@ __global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 
   const double2* U = jobs.uptr[o] + t;
   const double2* V = jobs.vptr[o] + t;
   double2 v[16];
+  if constexpr (FA) {
+    const double2* A = acc + (long long)job * N + t;
+#pragma unroll
+    for (int k = 0; k < E; ++k) v[InSlot<L>(k)] = A[NT * k];
+  } else {
 #pragma unroll
   for (int k = 0; k < E; ++k) v[k] = make_double2(0.0, 0.0);
   for (int i = 0; i < nterms; ++i) {
@@ -124,11 +132,12 @@ __global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 
       for (int k = 0; k < kRun && k0 + k < E; ++k) {
         const double pr = uu[k].x * vv[k].x - uu[k].y * vv[k].y;
         const double pim = uu[k].x * vv[k].y + uu[k].y * vv[k].x;
-        double2& acc = v[InSlot<L>(k0 + k)];
-        acc.x = acc.x + pr;
-        acc.y = acc.y + pim;
+        double2& ac = v[InSlot<L>(k0 + k)];
+        ac.x = ac.x + pr;
+        ac.y = ac.y + pim;
       }
     }
+  }
   }
   Transform<L, H>(v, lds, st);
   // x / 2d of polynomial_fft.cc:184: 2d is a power of two, so the product by
@@ -148,6 +157,36 @@ __global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 
     out[NT * e] = v0 - v1;
   }
   if (margin != nullptr) MarginMax<NT>(local_max, margin);
+}
+
+// The accumulation of IpKernel spread over the GPU (one thread per job and
+// position, same operations in the same term order), for launches with few
+// jobs, where one workgroup per job would be bound by one CU's bandwidth.
+__global__ __launch_bounds__(256) void MacKernel(IpJob jobs, int n_jobs, int nterms, int nu,
+                                                 int nv, PairList pairs, int log_n,
+                                                 double2* __restrict__ acc) {
+  const int N = 1 << log_n;
+  const long long total = static_cast<long long>(n_jobs) << log_n;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total;
+       e += static_cast<long long>(gridDim.x) * 256) {
+    const int job = static_cast<int>(e >> log_n);
+    const int pos = static_cast<int>(e & (N - 1));
+    const int o = job / pairs.count;
+    const int pi = job % pairs.count;
+    const int a = pairs.a[pi], b = pairs.b[pi];
+    const double2* U = jobs.uptr[o] + pos;
+    const double2* V = jobs.vptr[o] + pos;
+    double2 ac = make_double2(0.0, 0.0);
+    for (int i = 0; i < nterms; ++i) {
+      const double2 u = U[((long long)i * nu + a) * N];
+      const double2 v = V[((long long)i * nv + b) * N];
+      const double pr = u.x * v.x - u.y * v.y;
+      const double pim = u.x * v.y + u.y * v.x;
+      ac.x = ac.x + pr;
+      ac.y = ac.y + pim;
+    }
+    acc[e] = ac;
+  }
 }
 
 __device__ __forceinline__ u64 RescaleCoeff(u64 c, int log_cur, int log_new) {
@@ -464,8 +503,9 @@ struct Configure {
     hipError_t e;
     if ((e = Set(SpectraKernel<L, false>, false)) != hipSuccess) return e;
     if ((e = Set(SpectraKernel<L, true>, true)) != hipSuccess) return e;
-    if ((e = Set(IpKernel<L, false>, false)) != hipSuccess) return e;
-    if ((e = Set(IpKernel<L, true>, true)) != hipSuccess) return e;
+    if ((e = Set(IpKernel<L, false, false>, false)) != hipSuccess) return e;
+    if ((e = Set(IpKernel<L, true, false>, true)) != hipSuccess) return e;
+    if ((e = Set(IpKernel<L, false, true>, false)) != hipSuccess) return e;
     if ((e = Set(DbSpectraKernel<L, uint16_t, false>, false)) != hipSuccess) return e;
     if ((e = Set(DbSpectraKernel<L, uint16_t, true>, true)) != hipSuccess) return e;
     if ((e = Set(DbSpectraKernel<L, uint8_t, false>, false)) != hipSuccess) return e;
@@ -499,22 +539,30 @@ template <int L>
 struct IpL {
   static hipError_t Run(const IpJob& jobs, int n_out, int outs_per_group, int nterms, int nu,
                         int nv, const PairList& pairs, const double2* st,
-                        unsigned long long* margin, hipStream_t s) {
+                        unsigned long long* margin, double2* acc, hipStream_t s) {
     const long long n_jobs = static_cast<long long>(n_out) * pairs.count;
     const long long group_jobs = static_cast<long long>(outs_per_group) * pairs.count;
     const long long groups = (n_out + outs_per_group - 1) / outs_per_group;
     const long long blocks = (groups + 7) / 8 * 8 * group_jobs;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     if (n_jobs >= kHalfMinBlocks) {
-      hipLaunchKernelGGL((IpKernel<L, true>), dim3(static_cast<unsigned>(blocks)),
+      hipLaunchKernelGGL((IpKernel<L, true, false>), dim3(static_cast<unsigned>(blocks)),
                          dim3(Geo<L>::NT), Geo<L>::LdsBytes(true), s, jobs,
                          static_cast<int>(n_jobs), static_cast<int>(group_jobs), nterms, nu, nv,
-                         pairs, st, margin);
-    } else {
-      hipLaunchKernelGGL((IpKernel<L, false>), dim3(static_cast<unsigned>(blocks)),
+                         pairs, st, margin, nullptr);
+    } else if (acc != nullptr && n_jobs <= kIpSplitMaxJobs) {
+      const long long total = n_jobs << L;
+      hipLaunchKernelGGL(MacKernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256),
+                         0, s, jobs, static_cast<int>(n_jobs), nterms, nu, nv, pairs, L, acc);
+      hipLaunchKernelGGL((IpKernel<L, false, true>), dim3(static_cast<unsigned>(blocks)),
                          dim3(Geo<L>::NT), Geo<L>::LdsBytes(false), s, jobs,
                          static_cast<int>(n_jobs), static_cast<int>(group_jobs), nterms, nu, nv,
-                         pairs, st, margin);
+                         pairs, st, margin, acc);
+    } else {
+      hipLaunchKernelGGL((IpKernel<L, false, false>), dim3(static_cast<unsigned>(blocks)),
+                         dim3(Geo<L>::NT), Geo<L>::LdsBytes(false), s, jobs,
+                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), nterms, nu, nv,
+                         pairs, st, margin, nullptr);
     }
     return hipGetLastError();
   }
@@ -602,12 +650,12 @@ hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
                     u64* const* optr, int n_out, int outs_per_group, int nterms, int nu,
                     int nv, const PairList& pairs, const FftDims& F,
                     const double2* tw_bwd, unsigned long long* margin,
-                    hipStream_t s) {
+                    hipStream_t s, double2* acc) {
   if (n_out <= 0 || pairs.count <= 0) return hipSuccess;
   if (!ValidDims(F) || outs_per_group <= 0) return hipErrorInvalidValue;
   IpJob jobs{uptr, vptr, optr};
   return DispatchLogN<IpL>(F.log_N, jobs, n_out, outs_per_group, nterms, nu, nv, pairs,
-                           tw_bwd, margin, s);
+                           tw_bwd, margin, acc, s);
 }
 
 hipError_t LaunchDbSpectra(const void* db, int elem_bytes, long long stride, long long valid_cols,

@@ -236,12 +236,16 @@ struct ExtCombineArgs {
 };
 
 // outs_per_group: consecutive outputs that share their u spectra (one
-// request's outputs); their workgroups are placed on one XCD.
+// request's outputs); their workgroups are placed on one XCD. acc (optional,
+// kIpSplitMaxJobs * N complex values): launches of at most that many
+// (output, pair) jobs first accumulate the spectra across the whole GPU into
+// acc, then invert one job per workgroup.
+constexpr int kIpSplitMaxJobs = 64;
 hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
                     u64* const* optr, int n_out, int outs_per_group, int nterms, int nu,
                     int nv, const PairList& pairs, const FftDims& F,
                     const double2* tw_bwd, unsigned long long* margin,
-                    hipStream_t s);
+                    hipStream_t s, double2* acc = nullptr);
 hipError_t LaunchFinalizeCombine(const FinalizeArgs& A, hipStream_t s);
 hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,
                                const double2* tw_fwd, hipStream_t s);

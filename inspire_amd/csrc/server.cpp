@@ -461,6 +461,7 @@ std::vector<size_t> WorkspaceSizes(const pirg_server* S, int cap) {
       nq * S->ell * 2 * d * 8,
       nq * lt * 3 * sizeof(void*),
       nq * S->ell * 2 * 3 * sizeof(void*),
+      static_cast<size_t>(kIpSplitMaxJobs) * N * sizeof(double2),
   };
 }
 
@@ -488,7 +489,8 @@ Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap) {
   DevBuf* bufs[] = {&W.fd,       &W.sd,         &W.pk,          &W.bt_db,      &W.bt_pk,
                     &W.corr_db,  &W.corr_pk,    &W.db_partial,  &W.pk_partial, &W.yh_spec,
                     &W.sdb_spec, &W.d_spec,     &W.fin_partial, &W.ext_partial, &W.ct_b,
-                    &W.st_a,     &W.st_b,       &W.out,         &W.fin_tab,    &W.ext_tab};
+                    &W.st_a,     &W.st_b,       &W.out,         &W.fin_tab,    &W.ext_tab,
+                    &W.mac_acc};
   W.cap = 0;
   for (size_t i = 0; i < sizes.size(); ++i) {
     Error er = bufs[i]->alloc(sizes[i]);
@@ -626,7 +628,8 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
                       reinterpret_cast<const double2* const*>(tab + n_fin_cap),
                       reinterpret_cast<u64* const*>(tab + 2 * n_fin_cap),
                       nq * lt, lt, S->k, S->fin_u.num_chunks, S->fin_v.num_chunks,
-                      S->fin_pairs, S->F, S->tw_bwd.as<double2>(), margin, st));
+                      S->fin_pairs, S->F, S->tw_bwd.as<double2>(), margin, st,
+                      W.mac_acc.as<double2>()));
   }
   FinalizeArgs FA{};
   FA.db_partial = W.db_partial.as<u64>();
@@ -686,7 +689,7 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
                         reinterpret_cast<u64* const*>(et + 2 * n_ext_cap),
                         nq * S->ell * 2, S->ell * 2, 2 * S->e, S->ext_u.num_chunks,
                         S->ext_v.num_chunks, S->ext_pairs, S->F, S->tw_bwd.as<double2>(),
-                        margin, st));
+                        margin, st, W.mac_acc.as<double2>()));
       ExtCombineArgs EA{};
       EA.ext_partial = W.ext_partial.as<u64>();
       EA.npairs = S->ext_pairs.count;

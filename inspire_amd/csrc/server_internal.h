@@ -82,6 +82,7 @@ struct Workspace {
   DevBuf fin_partial, ext_partial;
   DevBuf ct_b, st_a, st_b, out;
   DevBuf fin_tab, ext_tab;       // IpKernel pointer tables
+  DevBuf mac_acc;                // accumulated spectra of small IpKernel launches
   HostBuf h_fd, h_sd, h_pk, h_out;
 };
 
