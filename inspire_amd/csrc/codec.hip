@@ -187,12 +187,15 @@ int pirg_poly_to_proto(const uint64_t* coeffs, int npolys, int len, int log_modu
   if (er.ok()) er = words.alloc(static_cast<size_t>(npolys) * pitch * 8);
   if (!er.ok()) return Report(er, err, errlen);
   auto run = [&]() -> Error {
-    PIRG_HIP(hipMemcpy(vals.p, coeffs, static_cast<size_t>(npolys) * len * 8,
-                       hipMemcpyHostToDevice));
+    CallStream cs;
+    PIRG_HIP(cs.create());
+    PIRG_HIP(CopySync(vals.p, coeffs, static_cast<size_t>(npolys) * len * 8,
+                      hipMemcpyHostToDevice, cs.s));
     PIRG_HIP(LaunchBitPack(vals.as<u64>(), npolys, len, log_modulus, words.as<u64>(), pitch,
-                           nullptr));
-    PIRG_HIP(hipMemcpy2D(out, poly_bytes, words.p, pitch * 8, poly_bytes, npolys,
-                         hipMemcpyDeviceToHost));
+                           cs.s));
+    PIRG_HIP(hipMemcpy2DAsync(out, poly_bytes, words.p, pitch * 8, poly_bytes, npolys,
+                              hipMemcpyDeviceToHost, cs.s));
+    PIRG_HIP(hipStreamSynchronize(cs.s));
     return Ok();
   };
   return Report(run(), err, errlen);
@@ -212,13 +215,15 @@ int pirg_poly_from_proto(const uint8_t* bytes, size_t poly_bytes, int npolys, in
   if (er.ok()) er = vals.alloc(static_cast<size_t>(npolys) * len * 8);
   if (!er.ok()) return Report(er, err, errlen);
   auto run = [&]() -> Error {
-    PIRG_HIP(hipMemset(words.p, 0, static_cast<size_t>(npolys) * pitch * 8));
-    PIRG_HIP(hipMemcpy2D(words.p, pitch * 8, bytes, poly_bytes, poly_bytes, npolys,
-                         hipMemcpyHostToDevice));
+    CallStream cs;
+    PIRG_HIP(cs.create());
+    PIRG_HIP(hipMemsetAsync(words.p, 0, static_cast<size_t>(npolys) * pitch * 8, cs.s));
+    PIRG_HIP(hipMemcpy2DAsync(words.p, pitch * 8, bytes, poly_bytes, poly_bytes, npolys,
+                              hipMemcpyHostToDevice, cs.s));
     PIRG_HIP(LaunchBitUnpack(words.as<u64>(), pitch, 0, npolys, len, log_modulus, false, 8,
-                             vals.p, nullptr));
-    PIRG_HIP(hipMemcpy(out, vals.p, static_cast<size_t>(npolys) * len * 8,
-                       hipMemcpyDeviceToHost));
+                             vals.p, cs.s));
+    PIRG_HIP(CopySync(out, vals.p, static_cast<size_t>(npolys) * len * 8,
+                      hipMemcpyDeviceToHost, cs.s));
     return Ok();
   };
   return Report(run(), err, errlen);

@@ -185,10 +185,12 @@ Error LoadFromBuffer(const pirg_params* params, const uint8_t* buffer, size_t si
   if (!er.ok()) return er;
   er = polys.alloc(all.size() * static_cast<size_t>(d) * 8);
   if (!er.ok()) return er;
-  PIRG_HIP(hipMemcpy(words.p, staged.data(), staged.size() * 8, hipMemcpyHostToDevice));
+  CallStream cs;
+  PIRG_HIP(cs.create());
+  PIRG_HIP(CopySync(words.p, staged.data(), staged.size() * 8, hipMemcpyHostToDevice, cs.s));
   PIRG_HIP(LaunchBitUnpack(words.as<u64>(), pitch, 0, static_cast<int>(all.size()), d, logm,
-                           false, 8, polys.p, nullptr));
-  PIRG_HIP(hipDeviceSynchronize());
+                           false, 8, polys.p, cs.s));
+  PIRG_HIP(hipStreamSynchronize(cs.s));
   const u64* dev = polys.as<u64>();
   const size_t lt = static_cast<size_t>(ell) * t;
   const u64* a_dev = dev;

@@ -171,19 +171,21 @@ Error Create(const pirg_params* params, int njobs, const void* const* pack,
   er = th.alloc(static_cast<size_t>(J) * k * d * 8);
   if (!er.ok()) return er;
   for (int j = 0; j < J; ++j) {
-    PIRG_HIP(hipMemcpy(K->a_tilde.as<u64>() + static_cast<size_t>(j) * d, a_tilde[j],
-                       static_cast<size_t>(d) * 8, hipMemcpyHostToDevice));
-    PIRG_HIP(hipMemcpy(th.as<u64>() + static_cast<size_t>(j) * k * d, t_vec_h[j],
-                       static_cast<size_t>(k) * d * 8, hipMemcpyHostToDevice));
+    PIRG_HIP(CopySync(K->a_tilde.as<u64>() + static_cast<size_t>(j) * d, a_tilde[j],
+                      static_cast<size_t>(d) * 8, hipMemcpyHostToDevice, K->stream));
+    PIRG_HIP(CopySync(th.as<u64>() + static_cast<size_t>(j) * k * d, t_vec_h[j],
+                      static_cast<size_t>(k) * d * 8, hipMemcpyHostToDevice, K->stream));
   }
   std::vector<double2> fwd, bwd;
   Twiddles(K->N, &fwd, &bwd);
   if ((er = K->tw_fwd.alloc(fwd.size() * sizeof(double2))).code) return er;
   if ((er = K->tw_bwd.alloc(bwd.size() * sizeof(double2))).code) return er;
-  PIRG_HIP(hipMemcpy(K->tw_fwd.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice));
-  PIRG_HIP(hipMemcpy(K->tw_bwd.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice));
+  PIRG_HIP(CopySync(K->tw_fwd.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice,
+                    K->stream));
+  PIRG_HIP(CopySync(K->tw_bwd.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice,
+                    K->stream));
   if ((er = K->margin.alloc(8)).code) return er;
-  PIRG_HIP(hipMemset(K->margin.p, 0, 8));
+  PIRG_HIP(FillSync(K->margin.p, 0, 8, K->stream));
   const int nu = K->fin_u.num_chunks, nv = K->fin_v.num_chunks;
   if ((er = K->t_spec.alloc(static_cast<size_t>(J) * k * nu * K->N * sizeof(double2))).code) return er;
   PIRG_HIP(LaunchSpectra(th.as<u64>(), J * k, 1, d, K->fin_u, K->F, K->tw_fwd.as<double2>(),
@@ -211,7 +213,7 @@ Error Create(const pirg_params* params, int njobs, const void* const* pack,
   for (auto& it : items) {
     if ((er = it.b->alloc(it.bytes)).code) return er;
   }
-  PIRG_HIP(hipMemset(K->zero.p, 0, 8));
+  PIRG_HIP(FillSync(K->zero.p, 0, 8, K->stream));
   if ((er = K->h_b.alloc(Md * 8)).code) return er;
   if ((er = K->h_keys.alloc(static_cast<size_t>(J) * 2 * k * d * 8)).code) return er;
   if ((er = K->h_out.alloc(Md * 2 * 8)).code) return er;
@@ -221,7 +223,8 @@ Error Create(const pirg_params* params, int njobs, const void* const* pack,
     tab[J + o] = K->yh_spec.as<double2>() + static_cast<size_t>(o) * k * nv * K->N;
     tab[2 * J + o] = K->fin_partial.as<u64>() + static_cast<size_t>(o) * K->fin_pairs.count * d;
   }
-  PIRG_HIP(hipMemcpy(K->fin_tab.p, tab.data(), tab.size() * sizeof(void*), hipMemcpyHostToDevice));
+  PIRG_HIP(CopySync(K->fin_tab.p, tab.data(), tab.size() * sizeof(void*), hipMemcpyHostToDevice,
+                    K->stream));
   PIRG_HIP(hipStreamSynchronize(K->stream));
   *out = std::move(K);
   return Ok();

@@ -81,16 +81,23 @@ bool ModInverse(long long a, long long m, uint32_t* out) {
   return true;
 }
 
-// FFT context for one ring degree: stream, twiddles, dims.
+// FFT context for one ring degree: stream, twiddles, dims. Given no stream,
+// it makes its own (destroyed, after a synchronize, before the twiddles are
+// freed).
 struct FftCtx {
   int d = 0, log_d = 0;
   FftDims F{};
   hipStream_t s = nullptr;
   DevBuf tw_fwd, tw_bwd;
+  CallStream own;
   Error Init(int d_, hipStream_t stream) {
     d = d_;
     log_d = Log2i(d);
     F = FftDims{d, log_d, 2 * d, log_d + 1};
+    if (stream == nullptr) {
+      PIRG_HIP(own.create());
+      stream = own.s;
+    }
     s = stream;
     PIRG_HIP(ConfigureFftKernels(F.N));
     std::vector<double2> fwd, bwd;
@@ -98,21 +105,21 @@ struct FftCtx {
     Error er = tw_fwd.alloc(fwd.size() * sizeof(double2));
     if (!er.ok()) return er;
     if (!(er = tw_bwd.alloc(bwd.size() * sizeof(double2))).ok()) return er;
-    PIRG_HIP(hipMemcpy(tw_fwd.p, fwd.data(), fwd.size() * sizeof(double2),
-                       hipMemcpyHostToDevice));
-    PIRG_HIP(hipMemcpy(tw_bwd.p, bwd.data(), bwd.size() * sizeof(double2),
-                       hipMemcpyHostToDevice));
+    PIRG_HIP(CopySync(tw_fwd.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice,
+                      s));
+    PIRG_HIP(CopySync(tw_bwd.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice,
+                      s));
     return Ok();
   }
 };
 
 // A device array of pointers, built on the host.
 template <typename T>
-Error UploadTable(const std::vector<T*>& v, DevBuf* buf) {
+Error UploadTable(const std::vector<T*>& v, DevBuf* buf, hipStream_t s) {
   Error er = buf->alloc(std::max<size_t>(v.size(), 1) * sizeof(T*));
   if (!er.ok()) return er;
   if (!v.empty()) {
-    PIRG_HIP(hipMemcpy(buf->p, v.data(), v.size() * sizeof(T*), hipMemcpyHostToDevice));
+    PIRG_HIP(CopySync(buf->p, v.data(), v.size() * sizeof(T*), hipMemcpyHostToDevice, s));
   }
   return Ok();
 }
@@ -219,7 +226,7 @@ Error HintMatrix(FftCtx& C, const void* db, int elem_bytes, long long stride, lo
   }
   DevBuf vdev, vspec;
   if (!(er = vdev.alloc(vinv.size() * 8)).ok()) return er;
-  PIRG_HIP(hipMemcpy(vdev.p, vinv.data(), vinv.size() * 8, hipMemcpyHostToDevice));
+  PIRG_HIP(CopySync(vdev.p, vinv.data(), vinv.size() * 8, hipMemcpyHostToDevice, C.s));
   if (!(er = vspec.alloc(static_cast<size_t>(n) * nv * N * sizeof(double2))).ok()) return er;
   PIRG_HIP(LaunchSpectra(vdev.as<u64>(), n, 1, d, vr, C.F, C.tw_fwd.as<double2>(),
                          vspec.as<double2>(), C.s));
@@ -257,9 +264,9 @@ Error HintMatrix(FftCtx& C, const void* db, int elem_bytes, long long stride, lo
     }
   }
   DevBuf tfu, tfv, tfo, ttu, ttv, tto;
-  if (!(er = UploadTable(fu, &tfu)).ok() || !(er = UploadTable(fv, &tfv)).ok() ||
-      !(er = UploadTable(fo, &tfo)).ok() || !(er = UploadTable(tu, &ttu)).ok() ||
-      !(er = UploadTable(tv, &ttv)).ok() || !(er = UploadTable(to, &tto)).ok()) {
+  if (!(er = UploadTable(fu, &tfu, C.s)).ok() || !(er = UploadTable(fv, &tfv, C.s)).ok() ||
+      !(er = UploadTable(fo, &tfo, C.s)).ok() || !(er = UploadTable(tu, &ttu, C.s)).ok() ||
+      !(er = UploadTable(tv, &ttv, C.s)).ok() || !(er = UploadTable(to, &tto, C.s)).ok()) {
     return er;
   }
   for (int x0 = 0; x0 < rows; x0 += R) {
@@ -342,9 +349,9 @@ Error MatrixPackPre(FftCtx& C, const pirg_params& P, const u64* a_is, int nb, co
     fo.push_back(part.as<u64>() + static_cast<size_t>(b) * np * d);
   }
   DevBuf tsu, tsv, tso, tfu, tfv, tfo;
-  if (!(er = UploadTable(su, &tsu)).ok() || !(er = UploadTable(sv, &tsv)).ok() ||
-      !(er = UploadTable(so, &tso)).ok() || !(er = UploadTable(fu, &tfu)).ok() ||
-      !(er = UploadTable(fv, &tfv)).ok() || !(er = UploadTable(fo, &tfo)).ok()) {
+  if (!(er = UploadTable(su, &tsu, C.s)).ok() || !(er = UploadTable(sv, &tsv, C.s)).ok() ||
+      !(er = UploadTable(so, &tso, C.s)).ok() || !(er = UploadTable(fu, &tfu, C.s)).ok() ||
+      !(er = UploadTable(fv, &tfv, C.s)).ok() || !(er = UploadTable(fo, &tfo, C.s)).ok()) {
     return er;
   }
   const int two_d = 2 * d;
@@ -517,7 +524,7 @@ Error RunPreprocess(const pirg_params& P, const void* db_raw, const u64* first_a
   DevBuf hint, wdev;
   if (!(er = hint.alloc(shard_rows * d * 8)).ok()) return er;
   if (!(er = wdev.alloc(static_cast<size_t>(2) * k * d * 8)).ok()) return er;
-  PIRG_HIP(hipMemcpy(wdev.p, pack_a, static_cast<size_t>(2) * k * d * 8, hipMemcpyHostToDevice));
+  PIRG_HIP(CopySync(wdev.p, pack_a, static_cast<size_t>(2) * k * d * 8, hipMemcpyHostToDevice, s));
   float interp_ms = 0, hint_ms = 0, pack_ms = 0, total_ms = 0;
   PIRG_HIP(hipEventElapsedTime(&interp_ms, ev[0], ev[1]));
   for (int sh = 0; sh < ell; ++sh) {
@@ -553,19 +560,21 @@ Error RunPreprocess(const pirg_params& P, const void* db_raw, const u64* first_a
 Error DownloadMaterial(const pirg_params& P, const Material& M, int ell, int t, int d, int k,
                        uint64_t* db_cdata, void* pack, uint64_t* a_tilde, uint64_t* t_vec_h) {
   const size_t rows = static_cast<size_t>(ell) * t * d;
+  CallStream cs;
+  PIRG_HIP(cs.create());
   if (db_cdata != nullptr) {
-    PIRG_HIP(hipMemcpy(db_cdata, M.db.p, rows * M.K * P.db_bytes, hipMemcpyDeviceToHost));
+    PIRG_HIP(CopySync(db_cdata, M.db.p, rows * M.K * P.db_bytes, hipMemcpyDeviceToHost, cs.s));
   }
   if (pack != nullptr) {
-    PIRG_HIP(hipMemcpy(pack, M.pack.p, rows * k * d * P.mat_bytes, hipMemcpyDeviceToHost));
+    PIRG_HIP(CopySync(pack, M.pack.p, rows * k * d * P.mat_bytes, hipMemcpyDeviceToHost, cs.s));
   }
   if (a_tilde != nullptr) {
-    PIRG_HIP(hipMemcpy(a_tilde, M.a_tilde.p, static_cast<size_t>(ell) * t * d * 8,
-                       hipMemcpyDeviceToHost));
+    PIRG_HIP(CopySync(a_tilde, M.a_tilde.p, static_cast<size_t>(ell) * t * d * 8,
+                      hipMemcpyDeviceToHost, cs.s));
   }
   if (t_vec_h != nullptr) {
-    PIRG_HIP(hipMemcpy(t_vec_h, M.t_vec_h.p, static_cast<size_t>(ell) * t * k * d * 8,
-                       hipMemcpyDeviceToHost));
+    PIRG_HIP(CopySync(t_vec_h, M.t_vec_h.p, static_cast<size_t>(ell) * t * k * d * 8,
+                      hipMemcpyDeviceToHost, cs.s));
   }
   return Ok();
 }
@@ -619,11 +628,14 @@ int pirg_interpolate(const void* rows, int db_bytes, long long nrows, int d, int
         !(er = b.alloc(bytes)).ok()) {
       return Report(er, err, errlen);
     }
-    hipError_t e = hipMemcpy(src.p, rows, bytes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return Report(HipErr(e, "hipMemcpy"), err, errlen);
-    er = InterpolateRows(src.p, db_bytes, nrows, d, t, 1, 0, p, a.p, b.p, nullptr);
+    CallStream cs;
+    hipError_t e = cs.create();
+    if (e != hipSuccess) return Report(HipErr(e, "hipStreamCreateWithFlags"), err, errlen);
+    e = CopySync(src.p, rows, bytes, hipMemcpyHostToDevice, cs.s);
+    if (e != hipSuccess) return Report(HipErr(e, "hipMemcpyAsync"), err, errlen);
+    er = InterpolateRows(src.p, db_bytes, nrows, d, t, 1, 0, p, a.p, b.p, cs.s);
     if (!er.ok()) return Report(er, err, errlen);
-    e = hipMemcpy(out, b.p, bytes, hipMemcpyDeviceToHost);
+    e = CopySync(out, b.p, bytes, hipMemcpyDeviceToHost, cs.s);
     if (e != hipSuccess) return Report(HipErr(e, "hipMemcpy"), err, errlen);
     return 0;
   } catch (const std::exception& ex) {
@@ -652,19 +664,18 @@ int pirg_compute_hint_matrix(const void* db, int db_bytes, int rows, int cols,
     }
     Error er = CheckDevice();
     if (!er.ok()) return Report(er, err, errlen);
-    hipStream_t s = nullptr;
     FftCtx C;
-    if (!(er = C.Init(d, s)).ok()) return Report(er, err, errlen);
+    if (!(er = C.Init(d, nullptr)).ok()) return Report(er, err, errlen);
     DevBuf dbd, o;
     const size_t dbb = static_cast<size_t>(rows) * cols * db_bytes;
     if (!(er = dbd.alloc(dbb)).ok() || !(er = o.alloc(static_cast<size_t>(rows) * d * 8)).ok()) {
       return Report(er, err, errlen);
     }
-    hipError_t e = hipMemcpy(dbd.p, db, dbb, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return Report(HipErr(e, "hipMemcpy"), err, errlen);
+    hipError_t e = CopySync(dbd.p, db, dbb, hipMemcpyHostToDevice, C.s);
+    if (e != hipSuccess) return Report(HipErr(e, "hipMemcpyAsync"), err, errlen);
     er = HintMatrix(C, dbd.p, db_bytes, cols, cols, rows, a_components, n, o.as<u64>());
     if (!er.ok()) return Report(er, err, errlen);
-    e = hipMemcpy(out, o.p, static_cast<size_t>(rows) * d * 8, hipMemcpyDeviceToHost);
+    e = CopySync(out, o.p, static_cast<size_t>(rows) * d * 8, hipMemcpyDeviceToHost, C.s);
     if (e != hipSuccess) return Report(HipErr(e, "hipMemcpy"), err, errlen);
     return 0;
   } catch (const std::exception& ex) {
@@ -697,21 +708,24 @@ int pirg_preprocess_matrix_pack(const pirg_params* params, int nblocks, const ui
         !(er = thd.alloc(static_cast<size_t>(nblocks) * k * d * 8)).ok()) {
       return Report(er, err, errlen);
     }
-    hipError_t e = hipMemcpy(ad.p, a_is, ab, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(wd.p, w, static_cast<size_t>(2) * k * d * 8, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return Report(HipErr(e, "hipMemcpy"), err, errlen);
+    hipError_t e = CopySync(ad.p, a_is, ab, hipMemcpyHostToDevice, C.s);
+    if (e == hipSuccess) {
+      e = CopySync(wd.p, w, static_cast<size_t>(2) * k * d * 8, hipMemcpyHostToDevice, C.s);
+    }
+    if (e != hipSuccess) return Report(HipErr(e, "hipMemcpyAsync"), err, errlen);
     er = MatrixPackPre(C, *params, ad.as<u64>(), nblocks, wd.as<u64>(), pd.p, atd.as<u64>(),
                        thd.as<u64>());
     if (!er.ok()) return Report(er, err, errlen);
-    e = hipMemcpy(pack, pd.p, pb, hipMemcpyDeviceToHost);
+    e = CopySync(pack, pd.p, pb, hipMemcpyDeviceToHost, C.s);
     if (e == hipSuccess) {
-      e = hipMemcpy(a_tilde, atd.p, static_cast<size_t>(nblocks) * d * 8, hipMemcpyDeviceToHost);
+      e = CopySync(a_tilde, atd.p, static_cast<size_t>(nblocks) * d * 8, hipMemcpyDeviceToHost,
+                   C.s);
     }
     if (e == hipSuccess) {
-      e = hipMemcpy(t_vec_h, thd.p, static_cast<size_t>(nblocks) * k * d * 8,
-                    hipMemcpyDeviceToHost);
+      e = CopySync(t_vec_h, thd.p, static_cast<size_t>(nblocks) * k * d * 8,
+                   hipMemcpyDeviceToHost, C.s);
     }
-    if (e != hipSuccess) return Report(HipErr(e, "hipMemcpy"), err, errlen);
+    if (e != hipSuccess) return Report(HipErr(e, "hipMemcpyAsync"), err, errlen);
     return 0;
   } catch (const std::exception& ex) {
     return Report(Internal(ex.what()), err, errlen);

@@ -326,10 +326,12 @@ Error Build(const pirg_params* params, const Material& mat, std::unique_ptr<pirg
     if (!er.ok()) return er;
     er = dev.alloc(polys.size() * static_cast<size_t>(d) * 8);
     if (!er.ok()) return er;
-    PIRG_HIP(hipMemcpy(words.p, staged.data(), staged.size() * 8, hipMemcpyHostToDevice));
+    CallStream cs;
+    PIRG_HIP(cs.create());
+    PIRG_HIP(CopySync(words.p, staged.data(), staged.size() * 8, hipMemcpyHostToDevice, cs.s));
     PIRG_HIP(LaunchBitUnpack(words.as<u64>(), pitch, 0, static_cast<int>(polys.size()), d, logm,
-                             false, 8, dev.p, nullptr));
-    PIRG_HIP(hipDeviceSynchronize());
+                             false, 8, dev.p, cs.s));
+    PIRG_HIP(hipStreamSynchronize(cs.s));
   }
   const size_t no = outs.size();
   const u64* base = dev.as<u64>();

@@ -357,15 +357,15 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   er = S->a_tilde.alloc(static_cast<size_t>(lt) * d * 8);
   if (!er.ok()) return er;
   for (int oi = 0; oi < lt; ++oi) {
-    PIRG_HIP(hipMemcpy(S->a_tilde.as<u64>() + static_cast<size_t>(oi) * d, a_parts[oi],
-                       static_cast<size_t>(d) * 8, hipMemcpyDefault));
+    PIRG_HIP(CopySync(S->a_tilde.as<u64>() + static_cast<size_t>(oi) * d, a_parts[oi],
+                      static_cast<size_t>(d) * 8, hipMemcpyDefault, S->main.stream));
   }
   DevBuf th;
   er = th.alloc(static_cast<size_t>(lt) * S->k * d * 8);
   if (!er.ok()) return er;
   for (int oi = 0; oi < lt; ++oi) {
-    PIRG_HIP(hipMemcpy(th.as<u64>() + static_cast<size_t>(oi) * S->k * d, th_parts[oi],
-                       static_cast<size_t>(S->k) * d * 8, hipMemcpyDefault));
+    PIRG_HIP(CopySync(th.as<u64>() + static_cast<size_t>(oi) * S->k * d, th_parts[oi],
+                      static_cast<size_t>(S->k) * d * 8, hipMemcpyDefault, S->main.stream));
   }
 
   // Twiddles.
@@ -375,11 +375,13 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   if (!er.ok()) return er;
   er = S->tw_bwd.alloc(bwd.size() * sizeof(double2));
   if (!er.ok()) return er;
-  PIRG_HIP(hipMemcpy(S->tw_fwd.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice));
-  PIRG_HIP(hipMemcpy(S->tw_bwd.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice));
+  PIRG_HIP(CopySync(S->tw_fwd.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice,
+                    S->main.stream));
+  PIRG_HIP(CopySync(S->tw_bwd.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice,
+                    S->main.stream));
   er = S->margin.alloc(8);
   if (!er.ok()) return er;
-  PIRG_HIP(hipMemset(S->margin.p, 0, 8));
+  PIRG_HIP(FillSync(S->margin.p, 0, 8, S->main.stream));
 
   // Resident spectra: t_vec_h chunks (finalize u operand) and second_dim_a
   // chunks (external product v operand of the a-component).
@@ -393,8 +395,8 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
       DevBuf sda;
       er = sda.alloc(static_cast<size_t>(2 * S->e) * d * 8);
       if (!er.ok()) return er;
-      PIRG_HIP(hipMemcpy(sda.p, second_dim_a, static_cast<size_t>(2 * S->e) * d * 8,
-                         hipMemcpyDefault));
+      PIRG_HIP(CopySync(sda.p, second_dim_a, static_cast<size_t>(2 * S->e) * d * 8,
+                        hipMemcpyDefault, S->main.stream));
       er = S->sda_spec.alloc(static_cast<size_t>(2 * S->e) * S->ext_v.num_chunks * S->N *
                              sizeof(double2));
       if (!er.ok()) return er;
@@ -514,7 +516,8 @@ Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap) {
     tab[n_fin + o] = W.yh_spec.as<double2>() + q * S->k * nv_f * N;
     tab[2 * n_fin + o] = W.fin_partial.as<u64>() + o * S->fin_pairs.count * d;
   }
-  PIRG_HIP(hipMemcpy(W.fin_tab.p, tab.data(), tab.size() * sizeof(void*), hipMemcpyHostToDevice));
+  PIRG_HIP(CopySync(W.fin_tab.p, tab.data(), tab.size() * sizeof(void*), hipMemcpyHostToDevice,
+                    slot.stream));
   if (S->t > 1) {
     const size_t n_ext = nq * S->ell * 2;
     std::vector<const void*> et(n_ext * 3);
@@ -525,7 +528,8 @@ Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap) {
                                  : W.sdb_spec.as<double2>() + q * 2 * S->e * nv_e * N;
       et[2 * n_ext + o] = W.ext_partial.as<u64>() + o * S->ext_pairs.count * d;
     }
-    PIRG_HIP(hipMemcpy(W.ext_tab.p, et.data(), et.size() * sizeof(void*), hipMemcpyHostToDevice));
+    PIRG_HIP(CopySync(W.ext_tab.p, et.data(), et.size() * sizeof(void*), hipMemcpyHostToDevice,
+                      slot.stream));
   }
   W.cap = cap;
   return Ok();
@@ -955,7 +959,9 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
   Error er = CheckDevice();
   if (!er.ok()) return er;
   PIRG_HIP(ConfigureScanKernels());
-  hipStream_t s = nullptr;
+  CallStream cs;
+  PIRG_HIP(cs.create());
+  hipStream_t s = cs.s;
   const int planes = is_u16 ? 2 : 4;
   const u64 c_off = is_u16 ? 32896ULL : 128ULL * (1 + 256 + 65536);
   LimbLayout L = MakeLayout(rows, cols, planes, c_off);
@@ -970,8 +976,8 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
   if ((er = bt.alloc(plan.btile_bytes)).code) return er;
   if ((er = corr.alloc((1 + VectorSumScratchWords(1)) * 8)).code) return er;
   if ((er = partial.alloc(static_cast<size_t>(ks) * rows * 8)).code) return er;
-  PIRG_HIP(hipMemcpy(src.p, cdata, static_cast<size_t>(rows) * wpr * 8, hipMemcpyHostToDevice));
-  PIRG_HIP(hipMemcpy(v.p, vec, static_cast<size_t>(cols) * 8, hipMemcpyHostToDevice));
+  PIRG_HIP(CopySync(src.p, cdata, static_cast<size_t>(rows) * wpr * 8, hipMemcpyHostToDevice, s));
+  PIRG_HIP(CopySync(v.p, vec, static_cast<size_t>(cols) * 8, hipMemcpyHostToDevice, s));
   if (is_u16) {
     PIRG_HIP(LaunchDbToTiles(src.as<u64>(), 2, rows, cols, 0, rows, L, tiles.as<int8_t>(), s));
   } else {
@@ -981,11 +987,10 @@ Error MatrixMultiply(const uint64_t* cdata, int rows, int cols, const uint64_t* 
   PIRG_HIP(LaunchVectorSums(v.as<u64>(), 1, cols, cols, c_off, corr.as<u64>() + 1,
                             corr.as<u64>(), s));
   PIRG_HIP(LaunchScan(tiles.as<int8_t>(), L, plan, bt.as<int8_t>(), partial.as<u64>(), s));
-  PIRG_HIP(hipDeviceSynchronize());
   std::vector<u64> part(static_cast<size_t>(ks) * rows);
   u64 cc = 0;
-  PIRG_HIP(hipMemcpy(part.data(), partial.p, part.size() * 8, hipMemcpyDeviceToHost));
-  PIRG_HIP(hipMemcpy(&cc, corr.p, 8, hipMemcpyDeviceToHost));
+  PIRG_HIP(CopySync(part.data(), partial.p, part.size() * 8, hipMemcpyDeviceToHost, s));
+  PIRG_HIP(CopySync(&cc, corr.p, 8, hipMemcpyDeviceToHost, s));
   for (int i = 0; i < rows; ++i) {
     u64 acc = cc;
     for (int k = 0; k < ks; ++k) acc += part[static_cast<size_t>(k) * rows + i];
@@ -1255,8 +1260,8 @@ int pirg_server_scan_times(pirg_server* S, double* db_scan_ms, double* pack_scan
 double pirg_server_fft_margin(pirg_server* S, int reset) {
   unsigned long long bits = 0;
   (void)hipStreamSynchronize(S->main.stream);
-  (void)hipMemcpy(&bits, S->margin.p, 8, hipMemcpyDeviceToHost);
-  if (reset) (void)hipMemset(S->margin.p, 0, 8);
+  (void)CopySync(&bits, S->margin.p, 8, hipMemcpyDeviceToHost, S->main.stream);
+  if (reset) (void)FillSync(S->margin.p, 0, 8, S->main.stream);
   double v;
   std::memcpy(&v, &bits, 8);
   return v;
@@ -1309,13 +1314,15 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
     if ((er = twf.alloc(fwd.size() * sizeof(double2))).code) return er;
     if ((er = twb.alloc(bwd.size() * sizeof(double2))).code) return er;
     if ((er = tab.alloc(3 * sizeof(void*))).code) return er;
-    PIRG_HIP(hipMemcpy(du.p, u, static_cast<size_t>(n) * d * 8, hipMemcpyHostToDevice));
-    PIRG_HIP(hipMemcpy(dv.p, v, static_cast<size_t>(n) * d * 8, hipMemcpyHostToDevice));
-    PIRG_HIP(hipMemcpy(twf.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice));
-    PIRG_HIP(hipMemcpy(twb.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice));
+    CallStream cs;
+    PIRG_HIP(cs.create());
+    hipStream_t s = cs.s;
+    PIRG_HIP(CopySync(du.p, u, static_cast<size_t>(n) * d * 8, hipMemcpyHostToDevice, s));
+    PIRG_HIP(CopySync(dv.p, v, static_cast<size_t>(n) * d * 8, hipMemcpyHostToDevice, s));
+    PIRG_HIP(CopySync(twf.p, fwd.data(), fwd.size() * sizeof(double2), hipMemcpyHostToDevice, s));
+    PIRG_HIP(CopySync(twb.p, bwd.data(), bwd.size() * sizeof(double2), hipMemcpyHostToDevice, s));
     const void* ptrs[3] = {us.p, vs.p, part.p};
-    PIRG_HIP(hipMemcpy(tab.p, ptrs, sizeof(ptrs), hipMemcpyHostToDevice));
-    hipStream_t s = nullptr;
+    PIRG_HIP(CopySync(tab.p, ptrs, sizeof(ptrs), hipMemcpyHostToDevice, s));
     PIRG_HIP(LaunchSpectra(du.as<u64>(), n, 1, d, ur, F, twf.as<double2>(), us.as<double2>(), s));
     PIRG_HIP(LaunchSpectra(dv.as<u64>(), n, 1, d, vr, F, twf.as<double2>(), vs.as<double2>(), s));
     void* const* t = tab.as<void* const>();
@@ -1324,8 +1331,7 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
                       reinterpret_cast<u64* const*>(t + 2), 1, 1, n,
                       ur.num_chunks, vr.num_chunks, pairs, F, twb.as<double2>(), nullptr, s));
     PIRG_HIP(LaunchSumPairs(part.as<u64>(), pairs.count, d, res.as<u64>(), s));
-    PIRG_HIP(hipDeviceSynchronize());
-    PIRG_HIP(hipMemcpy(out, res.p, static_cast<size_t>(d) * 8, hipMemcpyDeviceToHost));
+    PIRG_HIP(CopySync(out, res.p, static_cast<size_t>(d) * 8, hipMemcpyDeviceToHost, s));
     return Ok();
   };
   Error er = run();
@@ -1339,14 +1345,16 @@ int pirg_copy_probe(size_t bytes, int repeats, double* ms) {
     DevBuf a, b;
     if ((er = a.alloc(bytes)).code) return er;
     if ((er = b.alloc(bytes)).code) return er;
-    PIRG_HIP(hipMemset(a.p, 1, bytes));
+    CallStream cs;
+    PIRG_HIP(cs.create());
+    PIRG_HIP(FillSync(a.p, 1, bytes, cs.s));
     hipEvent_t e0, e1;
     PIRG_HIP(hipEventCreate(&e0));
     PIRG_HIP(hipEventCreate(&e1));
-    PIRG_HIP(LaunchCopyProbe(a.p, b.p, bytes, nullptr));
-    PIRG_HIP(hipEventRecord(e0, nullptr));
-    for (int i = 0; i < repeats; ++i) PIRG_HIP(LaunchCopyProbe(a.p, b.p, bytes, nullptr));
-    PIRG_HIP(hipEventRecord(e1, nullptr));
+    PIRG_HIP(LaunchCopyProbe(a.p, b.p, bytes, cs.s));
+    PIRG_HIP(hipEventRecord(e0, cs.s));
+    for (int i = 0; i < repeats; ++i) PIRG_HIP(LaunchCopyProbe(a.p, b.p, bytes, cs.s));
+    PIRG_HIP(hipEventRecord(e1, cs.s));
     PIRG_HIP(hipEventSynchronize(e1));
     float t = 0;
     PIRG_HIP(hipEventElapsedTime(&t, e0, e1));
@@ -1366,15 +1374,17 @@ int pirg_read_probe(size_t bytes, int repeats, double* ms) {
     DevBuf a, o;
     if ((er = a.alloc(bytes)).code) return er;
     if ((er = o.alloc(static_cast<size_t>(ReadProbeOutWords()) * 4)).code) return er;
-    PIRG_HIP(hipMemset(a.p, 1, bytes));
+    CallStream cs;
+    PIRG_HIP(cs.create());
+    PIRG_HIP(FillSync(a.p, 1, bytes, cs.s));
     hipEvent_t e0, e1;
     PIRG_HIP(hipEventCreate(&e0));
     PIRG_HIP(hipEventCreate(&e1));
     const int passes = std::max(repeats, 1);
-    PIRG_HIP(LaunchReadProbe(a.p, bytes, 1, o.as<int>(), nullptr));
-    PIRG_HIP(hipEventRecord(e0, nullptr));
-    PIRG_HIP(LaunchReadProbe(a.p, bytes, passes, o.as<int>(), nullptr));
-    PIRG_HIP(hipEventRecord(e1, nullptr));
+    PIRG_HIP(LaunchReadProbe(a.p, bytes, 1, o.as<int>(), cs.s));
+    PIRG_HIP(hipEventRecord(e0, cs.s));
+    PIRG_HIP(LaunchReadProbe(a.p, bytes, passes, o.as<int>(), cs.s));
+    PIRG_HIP(hipEventRecord(e1, cs.s));
     PIRG_HIP(hipEventSynchronize(e1));
     float t = 0;
     PIRG_HIP(hipEventElapsedTime(&t, e0, e1));
@@ -1394,13 +1404,15 @@ int pirg_mfma_probe(int blocks, int iters, double* ms, double* tops) {
     if (blocks <= 0 || iters <= 0) return Invalid("blocks and iters must be positive.");
     DevBuf o;
     if ((er = o.alloc(static_cast<size_t>(blocks) * 256 * 4)).code) return er;
+    CallStream cs;
+    PIRG_HIP(cs.create());
     hipEvent_t e0, e1;
     PIRG_HIP(hipEventCreate(&e0));
     PIRG_HIP(hipEventCreate(&e1));
-    PIRG_HIP(LaunchMfmaProbe(blocks, iters, o.as<int>(), nullptr));
-    PIRG_HIP(hipEventRecord(e0, nullptr));
-    PIRG_HIP(LaunchMfmaProbe(blocks, iters, o.as<int>(), nullptr));
-    PIRG_HIP(hipEventRecord(e1, nullptr));
+    PIRG_HIP(LaunchMfmaProbe(blocks, iters, o.as<int>(), cs.s));
+    PIRG_HIP(hipEventRecord(e0, cs.s));
+    PIRG_HIP(LaunchMfmaProbe(blocks, iters, o.as<int>(), cs.s));
+    PIRG_HIP(hipEventRecord(e1, cs.s));
     PIRG_HIP(hipEventSynchronize(e1));
     float t = 0;
     PIRG_HIP(hipEventElapsedTime(&t, e0, e1));

@@ -39,6 +39,35 @@ int Report(const Error& e, char* err, size_t errlen);
     if (_e != hipSuccess) return HipErr(_e, #expr); \
   } while (0)
 
+// Blocking copies and fills ordered on one of the library's non-blocking
+// streams. Nothing here uses the legacy default stream: HIP rejects a
+// legacy-stream operation while any thread captures a graph, and the serving
+// loop captures on its own thread while callers keep using the library.
+inline hipError_t CopySync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind,
+                           hipStream_t s) {
+  const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+  return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+inline hipError_t FillSync(void* dst, int value, size_t bytes, hipStream_t s) {
+  const hipError_t e = hipMemsetAsync(dst, value, bytes, s);
+  return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+
+// A non-blocking stream for one call of a standalone entry point.
+struct CallStream {
+  hipStream_t s = nullptr;
+  CallStream() = default;
+  CallStream(const CallStream&) = delete;
+  CallStream& operator=(const CallStream&) = delete;
+  ~CallStream() {
+    if (s != nullptr) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+  hipError_t create() { return hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+};
+
 // Device allocation released with its owner.
 struct DevBuf {
   void* p = nullptr;
