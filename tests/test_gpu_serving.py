@@ -162,3 +162,42 @@ def test_direct_calls_while_serving(material, requests):
     np.testing.assert_array_equal(direct, want)
     want, _, _ = O.process_response(P, material, requests[4])
     np.testing.assert_array_equal(served[4], want)
+
+
+def test_library_calls_while_serving(material, requests):
+    # While the serving loop captures and replays graphs on its own thread,
+    # the caller's thread grows workspaces, captures its own batch graphs and
+    # runs the standalone entry points. None of them may touch the legacy
+    # default stream (HIP rejects that during another thread's capture).
+    rng = np.random.default_rng(5)
+    u = rng.integers(0, 1 << 64, size=(3, 256), dtype=np.uint64)
+    v = rng.integers(0, 1 << 64, size=(3, 256), dtype=np.uint64)
+    want_ip, _ = O.inner_product_fft(u, v, 64, 64, 16)
+    polys = rng.integers(0, 1 << 40, size=(4, 256), dtype=np.uint64)
+    with server_from_material(P, material) as srv, \
+            server_from_material(P, material) as other:
+        eager = [other.process_response(gpu_request(r)) for r in requests[:6]]
+        other.use_graphs(True)
+        srv.serve_start(max_batch=8, slots=2, max_outstanding=64)
+        try:
+            tag = 0
+            for wave in range(5):
+                for _ in range(wave + 3):
+                    srv.submit(gpu_request(requests[tag % len(requests)]), tag=tag)
+                    tag += 1
+                # A new batch size: a larger workspace and a new graph.
+                n = wave + 2
+                got, st, _ = other.process_batch([gpu_request(r) for r in requests[:n]])
+                assert st == [0] * n
+                for a, b in zip(eager[:n], got):
+                    np.testing.assert_array_equal(b.ciphertexts, a.ciphertexts)
+                np.testing.assert_array_equal(ia.inner_product_fft(u, v, 64, 64, 16), want_ip)
+                enc = ia.poly_to_proto(polys, 40)
+                np.testing.assert_array_equal(ia.poly_from_proto(enc, 256, 40), polys)
+                assert other.fft_margin() >= 0.0
+            served = _drain(srv, tag)
+        finally:
+            srv.serve_stop()
+    for t in (0, tag - 1):
+        want, _, _ = O.process_response(P, material, requests[t % len(requests)])
+        np.testing.assert_array_equal(served[t], want)
