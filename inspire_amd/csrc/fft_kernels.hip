@@ -110,48 +110,6 @@ __global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 
     const double2* A = acc + (long long)job * N + t;
 #pragma unroll
     for (int k = 0; k < E; ++k) v[InSlot<L>(k)] = A[NT * k];
-  } else if constexpr (H && Geo<L>::NS > 1) {
-    // Two transforms share a CU, so a thread has registers for only half its
-    // positions' loads beside the accumulators. The positions are taken in
-    // two halves, each with all of a term's loads in flight at once: the
-    // upper half first, parked in the (still unused) exchange LDS, then the
-    // lower half in registers. Each position still sums its terms in order.
-    constexpr int EH = E / 2;
-    double2* park = reinterpret_cast<double2*>(lds) + t;
-#pragma unroll
-    for (int half = 1; half >= 0; --half) {
-      double2 ac[EH];
-#pragma unroll
-      for (int k = 0; k < EH; ++k) ac[k] = make_double2(0.0, 0.0);
-      for (int i = 0; i < nterms; ++i) {
-        const double2* Ui = U + ((long long)i * nu + a) * N + NT * EH * half;
-        const double2* Vi = V + ((long long)i * nv + b) * N + NT * EH * half;
-        double2 uu[EH], vv[EH];
-#pragma unroll
-        for (int k = 0; k < EH; ++k) {
-          uu[k] = Ui[NT * k];
-          vv[k] = Vi[NT * k];
-        }
-#pragma unroll
-        for (int k = 0; k < EH; ++k) {
-          const double pr = uu[k].x * vv[k].x - uu[k].y * vv[k].y;
-          const double pim = uu[k].x * vv[k].y + uu[k].y * vv[k].x;
-          ac[k].x = ac[k].x + pr;
-          ac[k].y = ac[k].y + pim;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < EH; ++k) {
-        if (half == 1) {
-          park[NT * k] = ac[k];
-        } else {
-          v[InSlot<L>(k)] = ac[k];
-        }
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < EH; ++k) v[InSlot<L>(EH + k)] = park[NT * k];
-    __syncthreads();  // every thread has its parked half back before the exchanges reuse the LDS
   } else {
 #pragma unroll
   for (int k = 0; k < E; ++k) v[k] = make_double2(0.0, 0.0);
