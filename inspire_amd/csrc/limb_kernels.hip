@@ -761,6 +761,167 @@ __global__ __launch_bounds__(512) void WideScanPipeKernel(
                                  (long long)(strip0 + h * RW) * 16, ks, partial);
 }
 
+// The tall digit-major scan (NP = 2, 7 digit tiles): each wave owns 6 strips (a workgroup 12
+// strips, 192 rows) by one 16-query group, so a workgroup moves 52 KiB of
+// operands per k-block for 624 MFMAs where WideScanPipeKernel moves 44 KiB for
+// 416, and each wave reads 19 fragments from LDS per 78 MFMAs instead of 15
+// per 52. The 168 accumulators leave no room for a second set of A
+// fragments, so A is streamed one MFMA block ahead (one fragment per block)
+// while B is read a whole step ahead. The A ring leads by 2 steps and the B
+// ring by 3; besides the barrier at the top of a step (A(i) and B(i + 1) have
+// landed everywhere), one before the last block guards the read of step
+// i + 1's first A fragment.
+template <int CT>
+struct TallShape {
+  static constexpr int NP = 2, RS = 12, RW = 6, AF = RS * NP / 8;
+  static constexpr int BH0 = (CT + 1) / 2, BH1 = CT - BH0;
+  static constexpr int kA = RS * NP * 64;  // v4i of A per stage
+  static constexpr int kB = 4 * CT * 64;   // v4i of B per stage
+  static constexpr int SA = 3, SB = 3;
+  static constexpr size_t kRing = static_cast<size_t>(SA * kA + SB * kB) * 16;
+  static constexpr size_t kRed = static_cast<size_t>(8) * RW * 16 * kQMax * 8;
+  static constexpr size_t kLdsBytes = kRing > kRed ? kRing : kRed;
+};
+
+template <int CT>
+__global__ __launch_bounds__(512) void WideScanTallKernel(
+    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
+    const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial) {
+  using W = TallShape<CT>;
+  constexpr int NP = W::NP, RS = W::RS, RW = W::RW, AF = W::AF, SA = W::SA, SB = W::SB;
+  extern __shared__ v4i lds[];
+  const int nx = (strips + RS - 1) / RS;
+  WideTile wt;
+  if (!DecodeWideTile(blockIdx.x, nx, nx * ksplit, ngp / 4, &wt)) return;
+  const int lane = threadIdx.x & 63;
+  // Wave-uniform, so that addresses derived from it stay in scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int h = wave >> 2;
+  const int g = wave & 3;
+  const int strip0 = wt.strip_block * RS;
+  const int ks = wt.ks;
+  const int grp = wt.zb * 4 + g;
+  const int kb0 = ks * kbps;
+  const int nkb = min(kblocks, kb0 + kbps) - kb0;
+  const v4i* A = reinterpret_cast<const v4i*>(tiles);
+  const v4i* Bg = reinterpret_cast<const v4i*>(btiles) + (long long)grp * CT * 64 + lane;
+  const v4i* Af[AF];
+#pragma unroll
+  for (int jf = 0; jf < AF; ++jf) {
+    const int f = wave * AF + jf;
+    Af[jf] = A + TileIndex(min(strip0 + f / NP, strips - 1), 0, f % NP, kblocks, NP) * 64 + lane;
+  }
+  const int c0 = h == 0 ? 0 : W::BH0;
+  v4i* const ring_a = lds;
+  v4i* const ring_b = lds + SA * W::kA;
+  v4i acc[RW][CT];
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[r][c] = v4i{0, 0, 0, 0};
+
+  // A fragment j of step s (steps past the k-range re-read the last k-block),
+  // and this wave's B share cc of step s (past the range: the zero block).
+  auto dma_a = [&](int s, int j) {
+    const long long kba = kb0 + min(s, nkb - 1);
+    DmaLoadStream16(Af[j] + kba * kStripGroup * NP * 64,
+                    ring_a + (s % SA) * W::kA + (wave * AF + j) * 64);
+  };
+  auto dma_b = [&](int s, int cc) {
+    if (h == 0 || cc < W::BH1) {
+      const long long kbb = s < nkb ? kb0 + s : kblocks;
+      const int c = c0 + cc;
+      DmaLoad16(Bg + (kbb * ngp * CT + c) * 64, ring_b + (s % SB) * W::kB + (g * CT + c) * 64);
+    }
+  };
+  auto read_a = [&](int s, int blk) -> v4i {
+    const int r = blk / NP, p = blk % NP;
+    return ring_a[(s % SA) * W::kA + ((h * RW + r) * NP + p) * 64 + lane];
+  };
+  auto read_b = [&](int s, int c) -> v4i {
+    return ring_b[(s % SB) * W::kB + (g * CT + c) * 64 + lane];
+  };
+  constexpr int kBlk = RW * NP;      // MFMA blocks per step, one per (strip, plane)
+  constexpr int kDma = AF + W::BH0;  // DMA slots per step (an h = 1 wave skips the last)
+  static_assert(kDma <= kBlk && CT <= kBlk, "one DMA slot and one B read per block at most");
+  // Own DMAs allowed in flight: na steps' A issues and nb steps' B issues.
+  auto wait_own = [&](auto na_c, auto nb_c) {
+    constexpr int na = decltype(na_c)::value, nb = decltype(nb_c)::value;
+    if (h == 0) {
+      WaitVmcnt<AF * na + W::BH0 * nb>();
+    } else {
+      WaitVmcnt<AF * na + W::BH1 * nb>();
+    }
+  };
+  // Step i: B fragments of step i in cb, its first A fragment in a_in. Issues
+  // A(i + 2) and B(i + 3) (A first), reads step i + 1's B into nb and leaves
+  // its first A fragment in a_in. Issue order: prologue A(0) B(0) A(1) B(1)
+  // B(2), then per step A(i + 2) B(i + 3). Before the last block, A(i + 1)
+  // must have landed: issued after it are B(i + 2) and all of step i's.
+  auto step = [&](int i, v4i (&cb)[CT], v4i (&nb)[CT], v4i& a_in) {
+    v4i ab[2];
+    ab[0] = a_in;
+#pragma unroll
+    for (int blk = 0; blk < kBlk; ++blk) {
+      const int r = blk / NP, p = blk % NP;
+      const v4i acur = ab[blk & 1];
+#pragma unroll
+      for (int c = p; c < CT; ++c)
+        acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur, cb[c - p], acc[r][c], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (blk < kDma) {
+        if (blk < AF) {
+          dma_a(i + 2, blk);
+        } else {
+          dma_b(i + 3, blk - AF);
+        }
+      }
+      if (blk < CT) nb[blk] = read_b(i + 1, blk);
+      if (blk + 1 < kBlk) {
+        ab[(blk + 1) & 1] = read_a(i, blk + 1);
+      } else {
+        wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{});
+        LdsBarrier();
+        a_in = read_a(i + 1, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  v4i cb0[CT], cb1[CT], a_in;
+#pragma unroll
+  for (int j = 0; j < AF; ++j) dma_a(0, j);
+#pragma unroll
+  for (int cc = 0; cc < W::BH0; ++cc) dma_b(0, cc);
+#pragma unroll
+  for (int j = 0; j < AF; ++j) dma_a(1, j);
+#pragma unroll
+  for (int cc = 0; cc < W::BH0; ++cc) dma_b(1, cc);
+#pragma unroll
+  for (int cc = 0; cc < W::BH0; ++cc) dma_b(2, cc);
+  // A(0), B(0) and B(1) have landed everywhere (only B(2) may be in flight).
+  wait_own(std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{});
+  LdsBarrier();
+#pragma unroll
+  for (int c = 0; c < CT; ++c) cb0[c] = read_b(0, c);
+  a_in = read_a(0, 0);
+  const int nkb2 = (nkb + 1) & ~1;
+  for (int i = 0; i < nkb2; i += 2) {
+    step(i, cb0, cb1, a_in);
+    // A(i + 1) and B(i + 2) have landed everywhere and step i is consumed:
+    // in flight at most step i's issues.
+    wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+    LdsBarrier();
+    step(i + 1, cb1, cb0, a_in);
+    wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+    LdsBarrier();
+  }
+  WaitVmcnt<0>();
+  LdsBarrier();
+  u64* red = reinterpret_cast<u64*>(lds) + wave * (RW * 16 * kQMax);
+  WaveEpilogueDigitMajor<RW, CT>(acc, red, lane, grp, nq, rows,
+                                 (long long)(strip0 + h * RW) * 16, ks, partial);
+}
+
 int GridBlocks(long long total, int threads) {
   long long b = (total + threads - 1) / threads;
   if (b > 65535LL * 16) b = 65535LL * 16;
@@ -809,6 +970,20 @@ hipError_t LaunchWidePipeT(const int8_t* tiles, const LimbLayout& L, const ScanP
   return hipGetLastError();
 }
 
+template <int CT>
+hipError_t LaunchWideTallT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
+                           const int8_t* btiles, u64* partial, hipStream_t s) {
+  using W = TallShape<CT>;
+  static_assert(W::kLdsBytes <= 160 * 1024, "WideScanTall LDS");
+  const long long ntiles = static_cast<long long>((L.strips + W::RS - 1) / W::RS) * P.ksplit;
+  const long long blocks = (ntiles + 7) / 8 * 8 * (P.ngp / 4);
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((WideScanTallKernel<CT>), dim3(static_cast<unsigned>(blocks)), dim3(512),
+                     W::kLdsBytes, s, tiles, L.strips, L.rows, L.kblocks, P.kb_per_split,
+                     P.ksplit, btiles, P.ngp, P.nq, partial);
+  return hipGetLastError();
+}
+
 template <int NP>
 hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                         const int8_t* btiles, u64* partial, hipStream_t s) {
@@ -831,6 +1006,9 @@ hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan
       }
     case 7:
       if (!P.digit_major) return LaunchWideT<NP, 7, false>(tiles, L, P, btiles, partial, s);
+      if constexpr (NP == 2) {
+        if (P.tall) return LaunchWideTallT<7>(tiles, L, P, btiles, partial, s);
+      }
       // Two register sets of fragments fit beside the accumulators for NP <= 2.
       if constexpr (NP <= 2) {
         return LaunchWidePipeT<NP, 7>(tiles, L, P, btiles, partial, s);
@@ -953,6 +1131,13 @@ hipError_t ConfigureWide() {
                              static_cast<int>(WideShape<NP, CT>::kLdsBytes));
 }
 
+template <int CT>
+hipError_t ConfigureWideTall() {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanTallKernel<CT>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(TallShape<CT>::kLdsBytes));
+}
+
 template <int NP, int CT>
 hipError_t ConfigureWidePipe() {
   return hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanPipeKernel<NP, CT>),
@@ -968,7 +1153,8 @@ hipError_t ConfigureScanKernels() {
                            ConfigureWide<4, 2>(), ConfigureWide<4, 4>(), ConfigureWide<4, 7>(),
                            ConfigureWide<4, 7, true>(), ConfigureWidePipe<1, 7>(),
                            ConfigureWidePipe<2, 7>(), ConfigureWide<4, 4, true>(),
-                           ConfigureWidePipe<1, 4>(), ConfigureWidePipe<2, 4>()};
+                           ConfigureWidePipe<1, 4>(), ConfigureWidePipe<2, 4>(),
+                           ConfigureWideTall<7>()};
   for (hipError_t e : es) {
     if (e != hipSuccess) return e;
   }
@@ -993,7 +1179,9 @@ void CompletePlan(const LimbLayout& L, int ks_scale, ScanPlan* P) {
   // K-split: exact int32 sums need at most 65536 / planes columns per split;
   // beyond that, enough workgroups to occupy every CU a few times over.
   const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
-  const int row_wgs = P->wide ? (L.strips + 7) / 8 : (L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR);
+  const int wide_rows = P->tall ? 12 : 8;  // strips per wide workgroup
+  const int row_wgs = P->wide ? (L.strips + wide_rows - 1) / wide_rows
+                              : (L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR);
   const int z = P->wide ? P->ngp / 4 : P->ngp;
   const int target = (P->wide ? 512 : 2048) * std::max(ks_scale, 1);
   int ks = (target + row_wgs * z - 1) / (row_wgs * z);
@@ -1013,7 +1201,7 @@ inline bool DigitMajorShifts(int ns) { return ns == 7 || ns == 4; }
 std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits) {
   std::vector<ScanPlan> out;
   const int ns = (out_bits + 7) / 8;
-  auto add = [&](int wide, int t, int dm) {
+  auto add = [&](int wide, int t, int dm, int tall) {
     for (int ks_scale : {1, 2}) {
       ScanPlan P{};
       P.wide = wide;
@@ -1021,11 +1209,12 @@ std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bi
       P.ns = ns;
       P.ctg = t;
       P.digit_major = dm;
+      P.tall = tall;
       CompletePlan(L, ks_scale, &P);
       bool dup = false;
       for (const ScanPlan& o : out) {
         dup = dup || (o.wide == P.wide && o.ctg == P.ctg && o.ksplit == P.ksplit &&
-                      o.digit_major == P.digit_major);
+                      o.digit_major == P.digit_major && o.tall == P.tall);
       }
       if (!dup) out.push_back(P);
     }
@@ -1034,16 +1223,19 @@ std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bi
     for (int t : (wide ? std::vector<int>{2, 4, 7} : std::vector<int>{1, 2, 4})) {
       // A narrow workgroup re-reads A for every group: keep those with few groups.
       if (!wide && (nq + QueriesPerGroup(t, ns) - 1) / QueriesPerGroup(t, ns) > 2) continue;
-      add(wide, t, 0);
+      add(wide, t, 0, 0);
     }
   }
   // Digit-major groups hold 16 queries: worth trying from two groups on.
-  if (DigitMajorShifts(ns) && nq > 16) add(1, ns, 1);
+  if (DigitMajorShifts(ns) && nq > 16) {
+    add(1, ns, 1, 0);
+    if (L.planes == 2 && ns == 7) add(1, ns, 1, 1);
+  }
   return out;
 }
 
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries,
-                      bool allow_digit_major) {
+                      bool allow_digit_major, bool allow_tall) {
   ScanPlan P{};
   P.nq = nq;
   P.ns = (out_bits + 7) / 8;
@@ -1066,6 +1258,7 @@ ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_qu
     P.wide = 1;
     P.ctg = ns;
     P.digit_major = 1;
+    P.tall = allow_tall && L.planes == 2 && ns == 7 ? 1 : 0;
   } else {
     P.wide = 1;
     P.ctg = kWideTiles[2];

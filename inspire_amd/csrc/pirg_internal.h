@@ -130,6 +130,9 @@ struct ScanPlan {
   // digit s of its 16 queries, so plane p's operand for output tile s is
   // plane 0's tile s - p, unchanged.
   int digit_major;
+  // Digit-major with two A planes and 7 digits only: 1 runs the tall kernel
+  // (6 strips per wave, 12 per workgroup) instead of the pipelined one (4 and 8).
+  int tall;
   int ngp;           // groups in the B layout (padded for WideScan)
   int ksplit;        // k-ranges (exact int32 sums and occupancy)
   int kb_per_split;
@@ -143,7 +146,7 @@ struct ScanPlan {
 // wide_min_queries: batches of at least this many queries use WideScan even
 // when they would fit NarrowScan.
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries,
-                      bool allow_digit_major = true);
+                      bool allow_digit_major = true, bool allow_tall = true);
 // The configurations an autotuner may time for nq queries (kind, tile width,
 // K-split).
 std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits);

@@ -417,7 +417,7 @@ ScanPlan PlanFor(const pirg_server* S, bool db, int nq) {
   const auto& tuned = db ? S->tuned_db : S->tuned_pk;
   auto it = tuned.find(nq);
   if (it != tuned.end()) return it->second;
-  return MakeScanPlan(db ? S->dbL : S->pkL, nq, S->log_q, S->wide_min, S->digit_major);
+  return MakeScanPlan(db ? S->dbL : S->pkL, nq, S->log_q, S->wide_min, S->digit_major, S->tall);
 }
 
 namespace {
@@ -824,20 +824,26 @@ Error ValidateView(const pirg_server* S, const pirg_preprocessed_view* v) {
   return Ok();
 }
 
+// Overrides of the default scan plans, for tests and tuning: PIRG_WIDE_MIN
+// sets the batch size from which the DB and pack products use the MFMA-bound
+// WideScan, PIRG_DIGIT_MAJOR=0 keeps WideScan on the query-major B layout and
+// PIRG_TALL=0 keeps digit-major scans on the 8-strip kernel (tests cover every
+// kernel at the same batch sizes with them).
+void ApplyPlanOverrides(pirg_server* S) {
+  if (const char* env = std::getenv("PIRG_WIDE_MIN")) {
+    const int v = std::atoi(env);
+    if (v > 0) S->wide_min = v;
+  }
+  if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
+  if (const char* env = std::getenv("PIRG_TALL")) S->tall = std::atoi(env) != 0;
+}
+
 Error CreateFromParts(const pirg_params* params, const pirg_preprocessed_view* v,
                       std::unique_ptr<pirg_server>* out) {
   std::unique_ptr<pirg_server> S(new pirg_server());
   Error er = DeriveServer(*params, S.get());
   if (!er.ok()) return er;
-  // PIRG_WIDE_MIN sets the batch size from which the DB and pack products
-  // use the MFMA-bound WideScan (tests force either kernel with it).
-  if (const char* env = std::getenv("PIRG_WIDE_MIN")) {
-    const int v = std::atoi(env);
-    if (v > 0) S->wide_min = v;
-  }
-  // PIRG_DIGIT_MAJOR=0 keeps WideScan on the query-major B layout (tests
-  // cover both layouts at the same batch sizes with it).
-  if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
+  ApplyPlanOverrides(S.get());
   er = ValidateView(S.get(), v);
   if (!er.ok()) return er;
   er = CheckDevice();
@@ -866,11 +872,7 @@ Error CreateFromImage(const pirg_params* params, const uint64_t* db_cdata, int d
   std::unique_ptr<pirg_server> S(new pirg_server());
   Error er = DeriveServer(*params, S.get());
   if (!er.ok()) return er;
-  if (const char* env = std::getenv("PIRG_WIDE_MIN")) {
-    const int v = std::atoi(env);
-    if (v > 0) S->wide_min = v;
-  }
-  if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
+  ApplyPlanOverrides(S.get());
   // The image holds the combined matrices: view them as ell shards.
   const int ell = S->ell, t = S->t, d = S->d, lt = ell * t;
   std::vector<int> rows(ell, t * d), cols(ell, db_cols), olen(ell, t);
@@ -906,11 +908,7 @@ Error CreateFromDevice(const pirg_params* params, const uint64_t* db_cdata, int 
   std::unique_ptr<pirg_server> S(new pirg_server());
   Error er = DeriveServer(*params, S.get());
   if (!er.ok()) return er;
-  if (const char* env = std::getenv("PIRG_WIDE_MIN")) {
-    const int v = std::atoi(env);
-    if (v > 0) S->wide_min = v;
-  }
-  if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
+  ApplyPlanOverrides(S.get());
   const int ell = S->ell, t = S->t, d = S->d, lt = ell * t;
   std::vector<int> rows(ell, t * d), cols(ell, db_cols), olen(ell, t);
   const size_t wpr = S->P.db_bytes == 2 ? db_cols / 4 : db_cols / 8;
@@ -1008,7 +1006,8 @@ namespace {
 std::string DescribePlan(const ScanPlan& P) {
   char buf[128];
   std::snprintf(buf, sizeof(buf), "%s tiles=%d ksplit=%d",
-                P.digit_major ? "wide-dm" : (P.wide ? "wide" : "narrow"), P.ctg, P.ksplit);
+                P.digit_major ? (P.tall ? "wide-dm-tall" : "wide-dm") : (P.wide ? "wide" : "narrow"),
+                P.ctg, P.ksplit);
   return buf;
 }
 

@@ -152,8 +152,10 @@ struct pirg_server {
   pirg::LimbLayout dbL{}, pkL{};
   // Batches of at least this many requests use the MFMA-bound WideScan.
   int wide_min = 10;
-  // Default plans may use the digit-major WideScan layout.
+  // Default plans may use the digit-major WideScan layout, and for two-plane
+  // matrices its tall kernel.
   bool digit_major = true;
+  bool tall = true;
   // Scan configurations chosen by pirg_server_autotune, per batch size.
   std::map<int, pirg::ScanPlan> tuned_db, tuned_pk;
   pirg::DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;

@@ -139,17 +139,21 @@ def test_batch_equals_sequential_with_invalid_request():
                 np.testing.assert_array_equal(single, want)
 
 
-@pytest.mark.parametrize("nq,wide_min,digit_major", [
-    (1, 10, 1), (2, 10, 1), (3, 10, 1), (9, 10, 1), (1, 1, 1), (3, 1, 1), (16, 10, 1),
-    (33, 10, 0), (33, 10, 1), (64, 10, 0), (64, 10, 1), (70, 10, 0), (70, 10, 1),
-    (129, 10, 1)])
-def test_wide_batches_match_oracle(nq, wide_min, digit_major, monkeypatch):
+@pytest.mark.parametrize("nq,wide_min,digit_major,tall", [
+    (1, 10, 1, 1), (2, 10, 1, 1), (3, 10, 1, 1), (9, 10, 1, 1), (1, 1, 1, 1), (3, 1, 1, 1),
+    (16, 10, 1, 1), (33, 10, 0, 1), (33, 10, 1, 1), (33, 10, 1, 0), (64, 10, 0, 1),
+    (64, 10, 1, 1), (64, 10, 1, 0), (70, 10, 0, 1), (70, 10, 1, 1), (70, 10, 1, 0),
+    (129, 10, 1, 1), (129, 10, 1, 0)])
+def test_wide_batches_match_oracle(nq, wide_min, digit_major, tall, monkeypatch):
     # Every scan configuration: NarrowScan with 1, 2 and 4 tiles per group,
     # WideScan with 2, 4 and 7 tiles per wave in the query-major B layout and
-    # with 7 digit tiles in the digit-major one, several workgroup column
-    # blocks (70 and 129 queries) and partly filled groups.
+    # with 7 digit tiles in the digit-major one (8 strips per workgroup, or
+    # the tall kernel's 12, which here leaves the last row block a third
+    # full: 1024 rows = 5 1/3 blocks), several workgroup column blocks (70 and
+    # 129 queries) and partly filled groups.
     monkeypatch.setenv("PIRG_WIDE_MIN", str(wide_min))
     monkeypatch.setenv("PIRG_DIGIT_MAJOR", str(digit_major))
+    monkeypatch.setenv("PIRG_TALL", str(tall))
     P = O.metrics_tool_params(n=4096, log_d=9, t=2)
     mat = O.synthetic_material(P, seed=nq)
     reqs = [O.synthetic_request(P, seed=1000 * nq + i) for i in range(nq)]
