@@ -99,16 +99,14 @@ def server_from(P, mat):
 
 def time_width(srv: ia.PirServer, reqs, B: int, steps: int, warmup: int):
     """Graph-replay throughput of batches of B staged requests, and the DB /
-    pack scan-kernel times of eager runs of the same batches."""
+    pack scan-kernel times inside those replays (HIP events recorded by event
+    nodes of the graph around each scan, on the stream the scans run on)."""
     srv.batch_stage(reqs[:B])
-    srv.use_graphs(False)
-    srv.scan_times_ms(reset=True)
-    for _ in range(2):
-        srv.batch_run(B)
-    db_ms, pk_ms, n = srv.scan_times_ms(reset=True)
     srv.use_graphs(True)
-    for _ in range(warmup):
+    srv.time_graph_scans(True)
+    for _ in range(max(warmup, 1)):
         srv.batch_run(B)
+    srv.scan_times_ms(reset=True)
     lat = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -116,10 +114,18 @@ def time_width(srv: ia.PirServer, reqs, B: int, steps: int, warmup: int):
         srv.batch_run(B)
         lat.append((time.perf_counter() - s0) * 1e3)
     total = time.perf_counter() - t0
+    db_ms, pk_ms, n = srv.scan_times_ms(reset=True)
+    srv.time_graph_scans(False)
     srv.use_graphs(False)
+    # The last replay's answers, sampled, against the single-request path
+    # (whose first answer the CPU baseline compares with the oracle).
+    outs = srv.batch_fetch(B)
+    same = all(np.array_equal(outs[i].ciphertexts, srv.process_response(reqs[i]).ciphertexts)
+               for i in sorted({0, B // 2, B - 1}))
     return {"batch": B, "qps": B * steps / total, "ms_per_batch": float(np.mean(lat)),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "db_scan_ms": db_ms / max(n, 1), "pack_scan_ms": pk_ms / max(n, 1)}
+            "db_scan_ms": db_ms / max(n, 1), "pack_scan_ms": pk_ms / max(n, 1),
+            "matches_single": bool(same)}
 
 
 def cpu_baseline(P, mat, req, max_queries=3, budget_s=30.0):
@@ -371,18 +377,17 @@ def run_vpir(widths, steps: int, warmup: int, cpu: bool, local: bool = False):
         srv.autotune([w for w in widths])
         for B in widths:
             srv.batch_stage(reqs[:B])
-            srv.use_graphs(False)
-            srv.scan_times_ms(reset=True)
-            for _ in range(2):
-                srv.batch_run(B)
-            db_ms, pk_ms, nl = srv.scan_times_ms(reset=True)
             srv.use_graphs(True)
-            for _ in range(warmup):
+            srv.time_graph_scans(True)
+            for _ in range(max(warmup, 1)):
                 srv.batch_run(B)
+            srv.scan_times_ms(reset=True)
             t0 = time.perf_counter()
             for _ in range(steps):
                 srv.batch_run(B)
             dt = (time.perf_counter() - t0) / steps
+            db_ms, pk_ms, nl = srv.scan_times_ms(reset=True)
+            srv.time_graph_scans(False)
             srv.use_graphs(False)
             pack_bytes = M * k * d * 4
             line = {"batch": B, "qps": B / dt, "ms_per_batch": dt * 1e3,
@@ -594,18 +599,21 @@ def main():
     log(f"server resident in HBM ({time.perf_counter() - t0:.1f}s)")
     reqs = synthetic_requests(P, max(widths + [64]), seed=7)
     if args.profile:
+        # The bench's timed region, shortened: graph replays at widths 1, 64
+        # and 256 with the scans timed by the graph's own event nodes.
+        scans = {}
         for B in (1, 64, 256):
             if B > len(reqs):
                 continue
-            srv.batch_stage(reqs[:B])
-            for _ in range(2):
-                srv.batch_run(B)
+            res = time_width(srv, reqs, B, steps=4, warmup=1)
+            scans[B] = {"db_scan_ms": res["db_scan_ms"], "pack_scan_ms": res["pack_scan_ms"],
+                        "ms_per_batch": res["ms_per_batch"],
+                        "matches_single": res["matches_single"]}
         run_codec(steps=2, cpu=False)
         copy_ms = ia.copy_probe(2 << 30, repeats=3)
         read_ms = ia.read_probe(4 << 30, repeats=3)
         print(json.dumps({"profile": True, "copy_bytes": 2 << 30, "copy_ms": copy_ms,
-                          "read_bytes": 4 << 30, "read_ms": read_ms,
-                          "scan_ms": srv.scan_times_ms()}))
+                          "read_bytes": 4 << 30, "read_ms": read_ms, "widths": scans}))
         srv.close()
         return
 
@@ -712,6 +720,7 @@ def main():
         "p99_ms_single": single["p99_ms"],
         "p50_ms_batch": best["p50_ms"],
         "p99_ms_batch": best["p99_ms"],
+        "batches_match_single": all(w["matches_single"] for w in widths_out),
         "roofline": roofline,
         "roofline_single_scan": roofline_single,
         "mfma_probe_tops": mfma_tops,

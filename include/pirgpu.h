@@ -163,6 +163,12 @@ int pirg_server_autotune(pirg_server* server, const int* widths, int nwidths,
  * do not add to the stage times below. No reference counterpart. */
 int pirg_server_use_graphs(pirg_server* server, int enable);
 
+/* Graphs captured from now on bracket the DB and pack scans with event-record
+ * nodes, so that graph replays (not only eager runs) add the scan-kernel
+ * times that pirg_server_scan_times reports. Changing the mode drops the
+ * graphs captured so far. Off by default. No reference counterpart. */
+int pirg_server_time_graph_scans(pirg_server* server, int enable);
+
 /* ---- Serving loop (no reference counterpart: the reference answers one
  * request per call). The DB and preprocessed material stay resident; a
  * worker thread keeps `slots` execution slots busy, each answering up to
@@ -396,8 +402,10 @@ int pirg_vpir_batch_fetch(pirg_vpir_server* server, int nreq, uint64_t* const* o
                           char* err, size_t errlen);
 int pirg_vpir_autotune(pirg_vpir_server* server, const int* widths, int nwidths,
                        char* report, size_t report_len, char* err, size_t errlen);
-/* As pirg_server_use_graphs / pirg_server_scan_times, for the underlying server. */
+/* As pirg_server_use_graphs / pirg_server_time_graph_scans /
+ * pirg_server_scan_times, for the underlying server. */
 int pirg_vpir_use_graphs(pirg_vpir_server* server, int enable);
+int pirg_vpir_time_graph_scans(pirg_vpir_server* server, int enable);
 int pirg_vpir_scan_times(pirg_vpir_server* server, double* db_scan_ms, double* pack_scan_ms,
                          long long* launches, int reset);
 

@@ -470,6 +470,11 @@ class PirServer:
         """process_batch / batch_run replay one captured hipGraph per batch size."""
         _check(lib().pirg_server_use_graphs(self._h, int(enable)), _err())
 
+    def time_graph_scans(self, enable: bool = True) -> None:
+        """Graphs captured from now on carry event nodes around the two scans,
+        so scan_times_ms also counts graph replays."""
+        _check(lib().pirg_server_time_graph_scans(self._h, int(enable)), _err())
+
     # Serving loop (continuous batching; thread-safe submit / poll).
     def serve_start(self, max_batch: int = 64, slots: int = 2,
                     max_outstanding: int = 4096) -> None:
@@ -692,6 +697,9 @@ class VpirServer:
 
     def use_graphs(self, enable: bool = True) -> None:
         lib().pirg_vpir_use_graphs(self._h, int(enable))
+
+    def time_graph_scans(self, enable: bool = True) -> None:
+        _check(lib().pirg_vpir_time_graph_scans(self._h, int(enable)), _err())
 
     def scan_times_ms(self, reset: bool = False):
         a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()

@@ -599,6 +599,13 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   auto mark = [&](hipEvent_t e) -> hipError_t {
     return timed ? hipEventRecord(e, st) : hipSuccess;
   };
+  // Scan brackets: eager runs record them directly; under capture with
+  // graph_scan_events they become event-record nodes of the graph.
+  const bool scan_nodes = !timed && slot.graph_scan_events;
+  auto kmark = [&](hipEvent_t e) -> hipError_t {
+    if (timed) return hipEventRecord(e, st);
+    return scan_nodes ? hipEventRecordWithFlags(e, st, hipEventRecordExternal) : hipSuccess;
+  };
   PIRG_HIP(mark(slot.ev[0]));
   // DB product (pir_server.cc:215-229).
   const ScanPlan pd = PlanFor(S, true, nq);
@@ -606,20 +613,20 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   PIRG_HIP(LaunchQueryDigits(W.fd.as<u64>(), S->K, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
   PIRG_HIP(LaunchVectorSums(W.fd.as<u64>(), nq, S->K, S->K, S->dbL.c_off,
                             W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
-  PIRG_HIP(mark(slot.kev[0]));
+  PIRG_HIP(kmark(slot.kev[0]));
   PIRG_HIP(LaunchScan(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
                       W.db_partial.as<u64>(), st));
-  PIRG_HIP(mark(slot.kev[1]));
+  PIRG_HIP(kmark(slot.kev[1]));
   PIRG_HIP(mark(slot.ev[1]));
   // Pack product (pir_server.cc:237-248): y = concat(y_vec_g).
   PIRG_HIP(LaunchQueryDigits(W.pk.as<u64>(), 2 * S->k * d, S->k * d, S->pkL, pp,
                               W.bt_pk.as<int8_t>(), st));
   PIRG_HIP(LaunchVectorSums(W.pk.as<u64>(), nq, 2 * S->k * d, S->k * d, S->pkL.c_off,
                             W.corr_pk.as<u64>() + nq, W.corr_pk.as<u64>(), st));
-  PIRG_HIP(mark(slot.kev[2]));
+  PIRG_HIP(kmark(slot.kev[2]));
   PIRG_HIP(LaunchScan(S->pk_tiles.as<int8_t>(), S->pkL, pp, W.bt_pk.as<int8_t>(),
                       W.pk_partial.as<u64>(), st));
-  PIRG_HIP(mark(slot.kev[3]));
+  PIRG_HIP(kmark(slot.kev[3]));
   // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279). A
   // local-finalize server (vPIR, vpir_server.cc:148-160) adds no inner product.
   if (!S->local_finalize) {
@@ -757,7 +764,21 @@ void CollectTiming(pirg_server* S, Slot& slot) {
   if (!slot.timing_pending) return;
   if (hipEventSynchronize(slot.ev[3]) != hipSuccess) return;
   slot.timing_pending = false;
-  if (slot.graph_run) return;  // a graph replay has no per-stage events
+  if (slot.graph_run) {
+    // A graph replay has no per-stage events; its scan brackets are nodes of
+    // the graph when it was captured with graph_scan_events.
+    if (!slot.graph_scan_events) return;
+    float ka = 0, kb = 0;
+    if (hipEventElapsedTime(&ka, slot.kev[0], slot.kev[1]) != hipSuccess ||
+        hipEventElapsedTime(&kb, slot.kev[2], slot.kev[3]) != hipSuccess) {
+      return;
+    }
+    std::lock_guard<std::mutex> lock(S->mu);
+    S->db_scan_ms += ka;
+    S->pk_scan_ms += kb;
+    S->scans += 1;
+    return;
+  }
   float a = 0, b = 0, c = 0;
   (void)hipEventElapsedTime(&a, slot.ev[0], slot.ev[1]);
   (void)hipEventElapsedTime(&b, slot.ev[1], slot.ev[2]);
@@ -1229,6 +1250,19 @@ int pirg_batch_fetch(pirg_server* S, int nreq, uint64_t* const* outs, char* err,
 int pirg_server_use_graphs(pirg_server* S, int enable) {
   if (S == nullptr) return PIRG_INVALID_ARGUMENT;
   S->use_graphs = enable != 0;
+  return 0;
+}
+
+int pirg_server_time_graph_scans(pirg_server* S, int enable) {
+  if (S == nullptr) return PIRG_INVALID_ARGUMENT;
+  const bool on = enable != 0;
+  if (on != S->main.graph_scan_events) {
+    // Graphs captured in the other mode are rebuilt on their next replay.
+    CollectTiming(S, S->main);
+    if (hipStreamSynchronize(S->main.stream) != hipSuccess) return PIRG_INTERNAL;
+    S->main.DropGraphs();
+    S->main.graph_scan_events = on;
+  }
   return 0;
 }
 

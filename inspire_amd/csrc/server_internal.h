@@ -126,6 +126,9 @@ struct Slot {
   hipEvent_t done = nullptr;                                 // serving: batch finished
   bool timing_pending = false;
   bool graph_run = false;  // the pending timing came from a graph launch
+  // Graphs captured on this slot bracket the two scans with event-record
+  // nodes (kev), so replays report scan-kernel times too.
+  bool graph_scan_events = false;
   std::map<int, hipGraphExec_t> graphs;
   Slot() = default;
   Slot(const Slot&) = delete;

@@ -220,6 +220,10 @@ int pirg_vpir_use_graphs(pirg_vpir_server* V, int enable) {
   return V == nullptr ? 3 : pirg_server_use_graphs(V->inner, enable);
 }
 
+int pirg_vpir_time_graph_scans(pirg_vpir_server* V, int enable) {
+  return V == nullptr ? 3 : pirg_server_time_graph_scans(V->inner, enable);
+}
+
 int pirg_vpir_scan_times(pirg_vpir_server* V, double* db_scan_ms, double* pack_scan_ms,
                          long long* launches, int reset) {
   return V == nullptr ? 3

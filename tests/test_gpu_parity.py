@@ -205,12 +205,22 @@ def test_process_errors_match_reference_messages():
         np.testing.assert_array_equal(srv.process_response(gpu_request(req)).ciphertexts, want)
 
 
-def test_full_size_response_matches_oracle():
-    """The headline shape (n = 2^20, d = 4096, t = 8): one request, GPU vs the
-    oracle, bit for bit, on range-faithful synthetic material."""
+@pytest.fixture(scope="module")
+def full_size():
+    """The headline shape (n = 2^20, d = 4096, t = 8) on range-faithful
+    synthetic material: (params, material, server)."""
     import bench
     P = ia.metrics_tool_params(n=1 << 20, log_d=12, t=8)
     mat = bench.synthetic_material(P, seed=2026)
+    srv = bench.server_from(P, mat)
+    yield P, mat, srv
+    srv.close()
+
+
+def test_full_size_response_matches_oracle(full_size):
+    """One request at the headline shape, GPU vs the oracle, bit for bit."""
+    import bench
+    P, mat, srv = full_size
     req = bench.synthetic_requests(P, 1, seed=7)[0]
     OP = O.Params(**{f: getattr(P, f) for f in O.Params.__dataclass_fields__})
     omat = O.Material(mat["db"], mat["pack"], mat["a_tilde"], mat["t_vec_h"],
@@ -218,10 +228,35 @@ def test_full_size_response_matches_oracle():
     want, _, _ = O.process_response(OP, omat, O.Request(req.first_dimension_query,
                                                         req.second_dimension_query,
                                                         req.packing_key))
-    with ia.PirServer.create_flat(P, mat["db"], mat["db_cols"], mat["pack"], mat["a_tilde"],
-                                  mat["t_vec_h"], mat["second_dim_a"]) as srv:
-        got = srv.process_response(req).ciphertexts
-    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(srv.process_response(req).ciphertexts, want)
+
+
+def test_full_size_batches_match_single_answers(full_size):
+    """Batches of 256 (the bench's width, after autotuning) and of a ragged 200
+    requests at the headline shape answer each request exactly as the
+    single-request path does (which the test above pins to the oracle); the
+    graph-replayed batch equals the eager one."""
+    import bench
+    P, _, srv = full_size
+    reqs = bench.synthetic_requests(P, 256, seed=77)
+    srv.autotune([256])
+    singles = {i: srv.process_response(reqs[i]).ciphertexts for i in (0, 1, 131, 199, 255)}
+    for n in (256, 200):
+        got, st, _ = srv.process_batch(reqs[:n])
+        assert st == [0] * n
+        for i, want in singles.items():
+            if i < n:
+                np.testing.assert_array_equal(got[i].ciphertexts, want, err_msg=f"n={n} i={i}")
+    srv.batch_stage(reqs)
+    srv.use_graphs(True)
+    try:
+        for _ in range(2):
+            srv.batch_run(256)
+            out = srv.batch_fetch(256)
+            for i, want in singles.items():
+                np.testing.assert_array_equal(out[i].ciphertexts, want, err_msg=f"graph i={i}")
+    finally:
+        srv.use_graphs(False)
 
 
 def test_autotuned_plans_match_oracle():

@@ -54,6 +54,37 @@ def test_graph_replays_match_eager(material, requests):
         np.testing.assert_array_equal(eager[i].ciphertexts, want)
 
 
+def test_graph_scan_timing_nodes(material, requests):
+    # Graphs captured with scan event nodes answer exactly as eager runs, and
+    # their replays (not only eager runs) add to the scan times; switching the
+    # mode back recaptures graphs without the nodes.
+    reqs = [gpu_request(r) for r in requests[:12]]
+    with server_from_material(P, material) as srv:
+        eager, _, _ = srv.process_batch(reqs)
+        srv.batch_stage(reqs)
+        srv.use_graphs(True)
+        srv.batch_run(12)
+        srv.scan_times_ms(reset=True)
+        srv.batch_run(12)
+        assert srv.scan_times_ms(reset=True) == (0.0, 0.0, 0)
+        srv.time_graph_scans(True)
+        for rep in range(3):
+            srv.batch_run(12)
+            for a, b in zip(eager, srv.batch_fetch(12)):
+                np.testing.assert_array_equal(b.ciphertexts, a.ciphertexts)
+        db_ms, pk_ms, n = srv.scan_times_ms(reset=True)
+        assert n == 3 and db_ms > 0 and pk_ms > 0
+        got, _, _ = srv.process_batch(reqs)
+        for a, b in zip(eager, got):
+            np.testing.assert_array_equal(b.ciphertexts, a.ciphertexts)
+        srv.time_graph_scans(False)
+        srv.scan_times_ms(reset=True)
+        srv.batch_run(12)
+        assert srv.scan_times_ms(reset=True)[2] == 0
+        for a, b in zip(eager, srv.batch_fetch(12)):
+            np.testing.assert_array_equal(b.ciphertexts, a.ciphertexts)
+
+
 def _drain(srv, n, timeout_ms=60000):
     got = {}
     while len(got) < n:
