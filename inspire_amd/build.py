@@ -67,7 +67,8 @@ def build(verbose: bool = False) -> str:
     objs = []
     for src in SOURCES:
         obj = os.path.join(BUILD_DIR, src.replace(".", "_") + ".o")
-        lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
+        # Host .cpp files launch kernels and use the HIP runtime: compile all as HIP.
+        lang = ["-x", "hip"]
         cmd = [cc, *COMMON_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c",
                os.path.join(CSRC, src), "-o", obj]
         if verbose:
