@@ -12,8 +12,8 @@ A step answers one batch of B requests (PirServer::ProcessResponse for each)
 as one hipGraph launch with the requests already in HBM. The scan
 configuration of every batch width is autotuned first. `value` is the best
 batched requests/s over the widths; the single-request path (B = 1) is timed
-beside it. Eager (non-graph) runs of the same batches time the DB-scan kernel
-with HIP events for the rooflines.
+beside it. Event-record nodes of the batch graph, around each scan, time the
+scan kernels inside the timed replays for the rooflines.
 
 Also reported, each on its own synthetic material:
   * serving: n = 262144, d = 2048, t = 4, ell = 2; Poisson arrivals at 100 to
@@ -25,7 +25,8 @@ Also reported, each on its own synthetic material:
     headline shape (n = 2^20, d = 4096, t = 8) on the GPU, stage times.
 
 The CPU baseline is the oracle's ProcessResponse restatement (one thread,
-matrix loops vectorised) on the same material and request.
+matrix loops vectorised) on the same material and request; a second, labelled
+variant runs its DB and pack products row-parallel on up to 16 host threads.
 """
 from __future__ import annotations
 
@@ -128,10 +129,12 @@ def time_width(srv: ia.PirServer, reqs, B: int, steps: int, warmup: int):
             "matches_single": bool(same)}
 
 
-def cpu_baseline(P, mat, req, max_queries=3, budget_s=30.0):
-    """The oracle's ProcessResponse (one thread) on full requests of the same
-    workload; stops after `max_queries` or the time budget."""
+def cpu_baseline(P, mat, req, max_queries=3, budget_s=30.0, threads=1):
+    """The oracle's ProcessResponse on full requests of the same workload,
+    its DB and pack products on `threads` threads (1: the reference's loop);
+    stops after `max_queries` or the time budget."""
     import oracle as O
+    O.set_matrix_threads(threads)
     OP = O.Params(**{f: getattr(P, f) for f in O.Params.__dataclass_fields__})
     omat = O.Material(mat["db"], mat["pack"], mat["a_tilde"], mat["t_vec_h"],
                       mat["second_dim_a"], mat["db_cols"])
@@ -144,6 +147,7 @@ def cpu_baseline(P, mat, req, max_queries=3, budget_s=30.0):
         times.append(time.perf_counter() - t0)
         if time.perf_counter() - start > budget_s:
             break
+    O.set_matrix_threads(1)
     return len(times) / sum(times), len(times), out, stages
 
 
@@ -657,7 +661,7 @@ def main():
             line["db_scan_hbm_frac"] = roofline_single["frac"]
         widths_out.append(line)
 
-    cpu = None
+    cpu = cpu_parallel = None
     if not args.no_cpu:
         qps, nq, cpu_out, stages = cpu_baseline(P, mat, reqs[0], max_queries=args.cpu_queries)
         gpu_out = srv.process_response(reqs[0]).ciphertexts
@@ -666,6 +670,17 @@ def main():
                          f"through the oracle ProcessResponse, one thread (last run's stage "
                          f"ms: db {stages[0]:.0f}, pack {stages[1]:.0f}, eval {stages[2]:.0f})",
                "bit_identical": bool(np.array_equal(cpu_out, gpu_out))}
+        # Row-parallel variant: the DB and pack products on the host's cores
+        # (16 at most, the CPU share of one GPU here), the FFT stages on one.
+        nthr = max(1, min(16, os.cpu_count() or 1))
+        pq, pn, pout, pst = cpu_baseline(P, mat, reqs[0], max_queries=args.cpu_queries,
+                                         threads=nthr)
+        cpu_parallel = {"value": pq, "unit": "queries/s", "cores": nthr, "kind": "port",
+                        "sample": f"{pn} full request(s) as above, DB and pack products "
+                                  f"row-parallel on {nthr} threads, FFT stages on one (last "
+                                  f"run's stage ms: db {pst[0]:.0f}, pack {pst[1]:.0f}, "
+                                  f"eval {pst[2]:.0f})",
+                        "bit_identical": bool(np.array_equal(pout, gpu_out))}
     want0 = srv.process_response(reqs[0]).ciphertexts
     srv.close()
     load = None
@@ -733,6 +748,7 @@ def main():
         "codec": codec,
         "preprocess": preprocess,
         "cpu_baseline": cpu,
+        "cpu_baseline_parallel": cpu_parallel,
     }
     print(json.dumps(line))
 

@@ -535,6 +535,13 @@ def client_decrypt(P: Params, secret_seed: int, response: np.ndarray) -> np.ndar
     return out
 
 
+def set_matrix_threads(n: int) -> None:
+    """Threads for ProcessResponse's DB and pack products (1 = the reference's
+    single-threaded loop); results are unchanged."""
+    lib().orc_set_matrix_threads.restype = None
+    lib().orc_set_matrix_threads(ctypes.c_int(int(n)))
+
+
 def process_response(P: Params, mat: Material, req: Request, first_dim_len=None):
     """PirServer::ProcessResponse (crypto/pir/server/pir_server.cc:152-306).
 

@@ -36,6 +36,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -695,6 +696,27 @@ void MulI16(const int16_t* data, int rows, int cols, const u64* vec, u64* out) {
     for (int j = 0; j < cols; ++j) acc += static_cast<u64>(row[j]) * vec[j];
     out[i] = acc;
   }
+}
+
+// Threads for the DB and pack products of ProcessResponse (1: the
+// reference's single-threaded loop). Rows are independent, so splitting them
+// changes no result.
+int g_matrix_threads = 1;
+
+// Runs f(row0, nrows) over [0, rows) in g_matrix_threads contiguous slices.
+template <class F>
+void ParallelRows(int rows, F f) {
+  const int n = std::max(1, std::min(g_matrix_threads, rows));
+  if (n == 1) {
+    f(0, rows);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const int per = (rows + n - 1) / n;
+  for (int r0 = 0; r0 < rows; r0 += per) {
+    pool.emplace_back(f, r0, std::min(per, rows - r0));
+  }
+  for (auto& th : pool) th.join();
 }
 
 // The pack matrix of the server in either representation.
@@ -1400,6 +1422,8 @@ int ImageLogModulus(const Derived& D) { return D.log_q >= 64 ? 64 : D.log_q; }
 
 extern "C" {
 
+void orc_set_matrix_threads(int n) { g_matrix_threads = n < 1 ? 1 : n; }
+
 int orc_get_sizes(const orc_params* P, orc_sizes* out, char* err,
                   size_t errlen) {
   ORC_GUARD_BEGIN
@@ -1751,11 +1775,13 @@ int orc_process_response(const orc_params* P, const uint64_t* db_cdata,
   std::vector<u64> fd(db_cols, 0);
   std::copy(first_dim, first_dim + fd_len, fd.begin());
   std::vector<u64> b_prime(D.M);
-  if (P->db_bytes == 1) {
-    CondensedMulU8Rows(db_cdata, db_cols, 0, D.M, fd.data(), b_prime.data());
-  } else {
-    CondensedMulU16Rows(db_cdata, db_cols, 0, D.M, fd.data(), b_prime.data());
-  }
+  ParallelRows(D.M, [&](int r0, int nr) {
+    if (P->db_bytes == 1) {
+      CondensedMulU8Rows(db_cdata, db_cols, r0, nr, fd.data(), b_prime.data() + r0);
+    } else {
+      CondensedMulU16Rows(db_cdata, db_cols, r0, nr, fd.data(), b_prime.data() + r0);
+    }
+  });
   double t1 = NowMs();
   // pir_server.cc:237-254: one pack product.
   std::vector<u64> y(static_cast<size_t>(k) * d);
@@ -1769,7 +1795,9 @@ int orc_process_response(const orc_params* P, const uint64_t* db_cdata,
     pv.i16 = static_cast<const int16_t*>(pack);
   }
   std::vector<u64> b_part(D.M);
-  pv.Multiply(0, D.M, k * d, y.data(), b_part.data());
+  ParallelRows(D.M, [&](int r0, int nr) {
+    pv.Multiply(r0, nr, k * d, y.data(), b_part.data() + r0);
+  });
   double pack_ms = NowMs() - t1, eval_ms = 0.0;
   // pir_server.cc:256-303.
   for (int s = 0; s < ell; ++s) {

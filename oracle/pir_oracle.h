@@ -156,6 +156,11 @@ int orc_external_product(const orc_params* P, const uint64_t* ct_a,
                          uint64_t* out_b, double* max_round_err, char* err,
                          size_t errlen);
 
+/* Threads for the DB and pack products of orc_process_response (default 1,
+ * the reference's single-threaded loop); rows split in contiguous slices,
+ * results unchanged. */
+void orc_set_matrix_threads(int n);
+
 /* ---- Server (crypto/pir/server/pir_server.cc) ----
  * Material layout (concatenated exactly as PirServer::Create does):
  *   db_cdata:   M rows x db_words_per_row condensed words (shard-major)

@@ -183,3 +183,19 @@ def test_interpolate_round_trip(d, t, p):
 def test_interpolate_errors():
     with pytest.raises(O.OracleError, match="plaintext_modulus must be odd"):
         O.interpolate(np.zeros(64, np.uint64), 32, 2, 256)
+
+
+def test_row_parallel_products_match_single_thread():
+    # The oracle's DB and pack products split over threads (the bench's
+    # parallel CPU baseline) answer exactly as the single-threaded loop.
+    P = O.metrics_tool_params(n=4096, log_d=8, t=4, ell=2)
+    mat = O.synthetic_material(P, seed=31)
+    req = O.synthetic_request(P, seed=32)
+    want, _, _ = O.process_response(P, mat, req)
+    try:
+        for n in (2, 3, 7):
+            O.set_matrix_threads(n)
+            got, _, _ = O.process_response(P, mat, req)
+            np.testing.assert_array_equal(got, want)
+    finally:
+        O.set_matrix_threads(1)
