@@ -270,6 +270,22 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
                            int u_is_signed, uint64_t* out, char* err,
                            size_t errlen);
 
+/* The same product on device memory, with no device allocation: u, v (n x d)
+ * and out (d) are device pointers, and the work is queued on `stream` (a
+ * hipStream_t; NULL for the default stream) without waiting for it. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream capture, serving loops that forbid
+ * hipMalloc). Its spectra and partial results live in one static 32 MiB
+ * device array per GPU, so n * (u chunks + v chunks) * 2d * 16 +
+ * pairs * d * 8 bytes may not exceed about 31.5 MiB (INVALID_ARGUMENT
+ * otherwise); calls that overlap on one GPU (other streams, other threads)
+ * race on that array. The first call on each device uploads the twiddle
+ * tables and may not run under stream capture (FAILED_PRECONDITION). Errors
+ * and messages otherwise as pirg_inner_product_fft. */
+int pirg_inner_product_fft_noalloc(const uint64_t* u, const uint64_t* v, int n, int d,
+                                   int u_bits, int v_bits, int chunk_bits, int u_is_signed,
+                                   uint64_t* out, void* stream, char* err, size_t errlen);
+
 /* Streams `bytes` through a plain device-to-device copy kernel `repeats`
  * times and reports the average kernel time. No reference counterpart: the
  * known-size kernel that calibrates HBM byte counters and bandwidth. */

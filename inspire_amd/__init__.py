@@ -812,6 +812,19 @@ def inner_product_fft(u, v, u_bits=64, v_bits=64, chunk_bits=20, u_signed=False)
     return out
 
 
+def inner_product_fft_noalloc(d_u: int, d_v: int, n: int, d: int, u_bits: int, v_bits: int,
+                              chunk_bits: int, u_signed: bool, d_out: int,
+                              stream: int = 0) -> None:
+    """pirg_inner_product_fft_noalloc on device addresses: queued on `stream`,
+    no device allocation (the working memory is the library's static array)."""
+    err = _err()
+    rc = lib().pirg_inner_product_fft_noalloc(ctypes.c_void_p(d_u), ctypes.c_void_p(d_v), n, d,
+                                              u_bits, v_bits, chunk_bits, int(u_signed),
+                                              ctypes.c_void_p(d_out), ctypes.c_void_p(stream),
+                                              err, ctypes.c_size_t(len(err)))
+    _check(rc, err)
+
+
 # ---- Offline preprocessing (crypto/pir/server/preprocessing_server.h) ----
 
 def interpolate(rows, d: int, t: int, plaintext_modulus: int, db_bytes: int = 2) -> np.ndarray:

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libpirgpu.so")
 
 SOURCES = ["limb_kernels.hip", "fft_kernels.hip", "codec.hip", "server.cpp", "serve.cpp",
            "pack.cpp", "loader.cpp", "proto_loader.cpp", "vpir.cpp", "preprocess_kernels.hip",
-           "preprocess.cpp"]
+           "preprocess.cpp", "noalloc.cpp"]
 # Per-source flags. The scan kernels keep up to 224 int32 accumulators per
 # lane; LLVM's default AGPR form for their MFMAs makes the register allocator
 # shuttle accumulators between AGPRs and VGPRs every k-block, while the VGPR
