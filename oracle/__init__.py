@@ -228,6 +228,36 @@ def inner_product_fft(u, v, u_bits=64, v_bits=64, chunk_bits=20, u_signed=False)
     return out, me.value
 
 
+def inner_product_fft_pairs(u, v, u_bits, v_bits, chunk_bits, u_signed=False, fused=False):
+    """The values InnerProductFft rounds (polynomial_fft.cc:176-184), per
+    accumulated chunk pair: returns (pairs, pre) with pairs a list of (a, b)
+    in the reference's order and pre[p, j] = backward_out[j].real() / 2d for
+    j < 2d. fused evaluates every complex product with FMA contraction."""
+    u = _u64_2d(u)
+    v = _u64_2d(v)
+    n, d = u.shape
+    max_chunks = (64 + chunk_bits - 1) // chunk_bits
+    nu = min((u_bits + chunk_bits - 1) // chunk_bits, max_chunks)
+    nv = min((v_bits + chunk_bits - 1) // chunk_bits, max_chunks)
+    pre = np.empty((nu * nv, 2 * d), dtype=np.float64)
+    pa = np.empty(nu * nv, dtype=np.int32)
+    pb = np.empty(nu * nv, dtype=np.int32)
+    npairs = ctypes.c_int(0)
+    ip = ctypes.POINTER(ctypes.c_int)
+    _call(lib().orc_inner_product_fft_pairs, _p(u), _p(v), n, d, u_bits, v_bits, chunk_bits,
+          int(u_signed), int(fused), pre.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+          pa.ctypes.data_as(ip), pb.ctypes.data_as(ip), ctypes.byref(npairs))
+    k = npairs.value
+    return [(int(pa[i]), int(pb[i])) for i in range(k)], pre[:k].copy()
+
+
+def set_fft_contraction(fused: bool) -> None:
+    """Process-wide: InnerProductFft and what uses it evaluate complex products
+    with FMA contraction (True) or as the reference's uncontracted build (False,
+    the default). For rounding analysis only."""
+    lib().orc_set_fft_contraction(int(bool(fused)))
+
+
 def inner_product_exact(u, v, u_bits=64, v_bits=64, chunk_bits=20, u_signed=False):
     """Exact 128-bit schoolbook counterpart of inner_product_fft (closed form)."""
     u = _u64_2d(u)

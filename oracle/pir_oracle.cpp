@@ -316,8 +316,27 @@ struct FftPlan {
   }
 };
 
-// fft.h:106-129.
-void ComputeFft(const cd* in, cd* out, const FftPlan& plan, const CVec& tw) {
+// std::complex<double> multiplication as a build with floating-point
+// contraction evaluates it where the target has FMA instructions (GCC's
+// default -ffp-contract=fast for C++, or clang's =on, with e.g.
+// -march=skylake-avx512): each part's first product is fused with the
+// subtraction / addition, (fma(xr, wr, -(xi*wi)), fma(xr, wi, xi*wr)). One
+// plausible contracted build of crypto/fft.h:117 and polynomial_fft.cc:162,
+// used only to measure which roundings depend on the build.
+inline cd MulFused(const cd& x, const cd& w) {
+  return cd(std::fma(x.real(), w.real(), -(x.imag() * w.imag())),
+            std::fma(x.real(), w.imag(), x.imag() * w.real()));
+}
+
+// Set by orc_set_fft_contraction: InnerProductFft (and everything built on
+// it: ExternalProduct, FinalizeMatrixPack, ProcessResponse) evaluates its
+// complex products as MulFused does. Off by default; on only to measure what
+// a contracted build of the reference would answer.
+bool g_fft_fused = false;
+
+// fft.h:106-129 (kFused: the products as MulFused evaluates them).
+template <bool kFused>
+void ComputeFftT(const cd* in, cd* out, const FftPlan& plan, const CVec& tw) {
   const size_t n = plan.size;
   for (size_t i = 0; i < n; ++i) out[plan.bit_rev[i]] = in[i];
   for (size_t len = 2; len <= n; len <<= 1) {
@@ -326,12 +345,17 @@ void ComputeFft(const cd* in, cd* out, const FftPlan& plan, const CVec& tw) {
     for (size_t i = 0; i < n; i += len) {
       for (size_t j = 0; j < half; ++j) {
         cd u = out[i + j];
-        cd v = out[i + j + half] * tw[j * step];
+        cd v = kFused ? MulFused(out[i + j + half], tw[j * step])
+                      : out[i + j + half] * tw[j * step];
         out[i + j] = u + v;
         out[i + j + half] = u - v;
       }
     }
   }
+}
+
+void ComputeFft(const cd* in, cd* out, const FftPlan& plan, const CVec& tw) {
+  ComputeFftT<false>(in, out, plan, tw);
 }
 
 // The chunk-extraction rule of InnerProductFft (polynomial_fft.cc:118-152):
@@ -371,6 +395,7 @@ struct ChunkRule {
 
 // Forward spectra of every chunk of one polynomial, zero-padded to 2d
 // (polynomial_fft.cc:119-152).
+template <bool kFused = false>
 std::vector<CVec> ChunkSpectra(const u64* coeffs, int d, const ChunkRule& rule,
                                const FftPlan& plan) {
   const int len2 = 2 * d;
@@ -382,7 +407,7 @@ std::vector<CVec> ChunkSpectra(const u64* coeffs, int d, const ChunkRule& rule,
       if (j < d) val = rule.Value(coeffs[j], c);
       in[j] = cd(val, 0.0);
     }
-    ComputeFft(in.data(), out[c].data(), plan, plan.tw_fwd);
+    ComputeFftT<kFused>(in.data(), out[c].data(), plan, plan.tw_fwd);
   }
   return out;
 }
@@ -426,6 +451,7 @@ struct IpAccum {
         acc(setup.ur.num_chunks * setup.vr.num_chunks,
             CVec(2 * d_, cd(0.0, 0.0))) {}
   // polynomial_fft.cc:154-164.
+  template <bool kFused = false>
   void Mac(const std::vector<CVec>& us, const std::vector<CVec>& vs) {
     const int len2 = 2 * d;
     for (int a = 0; a < s.ur.num_chunks; ++a) {
@@ -434,12 +460,13 @@ struct IpAccum {
         CVec& ac = acc[a * s.vr.num_chunks + b];
         const CVec& ua = us[a];
         const CVec& vb = vs[b];
-        for (int j = 0; j < len2; ++j) ac[j] += ua[j] * vb[j];
+        for (int j = 0; j < len2; ++j) ac[j] += kFused ? MulFused(ua[j], vb[j]) : ua[j] * vb[j];
       }
     }
   }
   // polynomial_fft.cc:167-196: inverse FFT, divide by 2d, round to int64,
   // shift by chunk_bits*(a+b) and fold X^d = -1.
+  template <bool kFused = false>
   Poly Finish(const FftPlan& plan, double* max_err) const {
     const int len2 = 2 * d;
     const int cb = s.ur.chunk_bits;
@@ -448,8 +475,8 @@ struct IpAccum {
     for (int a = 0; a < s.ur.num_chunks; ++a) {
       for (int b = 0; b < s.vr.num_chunks; ++b) {
         if (cb * (a + b) >= 64) continue;
-        ComputeFft(acc[a * s.vr.num_chunks + b].data(), out.data(), plan,
-                   plan.tw_bwd);
+        ComputeFftT<kFused>(acc[a * s.vr.num_chunks + b].data(), out.data(), plan,
+                            plan.tw_bwd);
         for (int j = 0; j < len2; ++j) {
           const double vd = out[j].real() / len2;
           const double rd = std::round(vd);
@@ -488,12 +515,63 @@ Poly InnerProductFft(const std::vector<Poly>& u, const std::vector<Poly>& v,
                                 chunk_bits, u_is_signed);
   FftPlan plan(2 * d);
   IpAccum acc(s, d);
+  if (g_fft_fused) {
+    for (size_t i = 0; i < u.size(); ++i) {
+      acc.Mac<true>(ChunkSpectra<true>(u[i].data(), d, s.ur, plan),
+                    ChunkSpectra<true>(v[i].data(), d, s.vr, plan));
+    }
+    return acc.Finish<true>(plan, max_err);
+  }
   for (size_t i = 0; i < u.size(); ++i) {
     auto us = ChunkSpectra(u[i].data(), d, s.ur, plan);
     auto vs = ChunkSpectra(v[i].data(), d, s.vr, plan);
     acc.Mac(us, vs);
   }
   return acc.Finish(plan, max_err);
+}
+
+// The values InnerProductFft rounds, per accumulated chunk pair: pre[p][j] =
+// backward_out[j].real() / len2 (polynomial_fft.cc:176-184) for the p-th pair
+// (a, b) with chunk_bits*(a+b) < 64, in the reference's a-major order, before
+// round() and before the shift and fold. kFused evaluates every complex
+// product (butterflies and the spectra products of :162) with MulFused.
+template <bool kFused>
+int InnerProductFftPairs(const std::vector<Poly>& u, const std::vector<Poly>& v, int u_bits,
+                         int v_bits, int chunk_bits, bool u_is_signed, double* pre,
+                         int* pair_a, int* pair_b) {
+  const int d = static_cast<int>(u[0].size());
+  IpSetup s = InnerProductSetup(static_cast<int>(u.size()), d, u_bits, v_bits, chunk_bits,
+                                u_is_signed);
+  const int len2 = 2 * d;
+  FftPlan plan(len2);
+  const int nu = s.ur.num_chunks, nv = s.vr.num_chunks;
+  std::vector<CVec> acc(nu * nv, CVec(len2, cd(0.0, 0.0)));
+  for (size_t i = 0; i < u.size(); ++i) {
+    auto us = ChunkSpectra<kFused>(u[i].data(), d, s.ur, plan);
+    auto vs = ChunkSpectra<kFused>(v[i].data(), d, s.vr, plan);
+    for (int a = 0; a < nu; ++a) {
+      for (int b = 0; b < nv; ++b) {
+        if (chunk_bits * (a + b) >= 64) continue;
+        CVec& ac = acc[a * nv + b];
+        for (int j = 0; j < len2; ++j) {
+          ac[j] += kFused ? MulFused(us[a][j], vs[b][j]) : us[a][j] * vs[b][j];
+        }
+      }
+    }
+  }
+  int np = 0;
+  CVec out(len2);
+  for (int a = 0; a < nu; ++a) {
+    for (int b = 0; b < nv; ++b) {
+      if (chunk_bits * (a + b) >= 64) continue;
+      ComputeFftT<kFused>(acc[a * nv + b].data(), out.data(), plan, plan.tw_bwd);
+      for (int j = 0; j < len2; ++j) pre[static_cast<size_t>(np) * len2 + j] = out[j].real() / len2;
+      pair_a[np] = a;
+      pair_b[np] = b;
+      ++np;
+    }
+  }
+  return np;
 }
 
 // Closed-form checker: sum_i (U_i * V_i) mod (X^d + 1, 2^64) where U_i, V_i are
@@ -1506,6 +1584,22 @@ int orc_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
                            v_bits, chunk_bits, u_is_signed != 0, &me);
   std::memcpy(out, r.data(), sizeof(u64) * d);
   if (max_round_err != nullptr) *max_round_err = me;
+  ORC_GUARD_END(err, errlen)
+}
+
+void orc_set_fft_contraction(int fused) { g_fft_fused = fused != 0; }
+
+int orc_inner_product_fft_pairs(const uint64_t* u, const uint64_t* v, int n, int d,
+                                int u_bits, int v_bits, int chunk_bits, int u_is_signed,
+                                int fused, double* pre, int* pair_a, int* pair_b,
+                                int* npairs, char* err, size_t errlen) {
+  ORC_GUARD_BEGIN
+  const auto U = SplitPolys(u, n, d);
+  const auto V = SplitPolys(v, n, d);
+  *npairs = fused ? InnerProductFftPairs<true>(U, V, u_bits, v_bits, chunk_bits,
+                                               u_is_signed != 0, pre, pair_a, pair_b)
+                  : InnerProductFftPairs<false>(U, V, u_bits, v_bits, chunk_bits,
+                                                u_is_signed != 0, pre, pair_a, pair_b);
   ORC_GUARD_END(err, errlen)
 }
 

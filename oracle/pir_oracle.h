@@ -82,6 +82,22 @@ int orc_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
                           int u_bits, int v_bits, int chunk_bits,
                           int u_is_signed, uint64_t* out, double* max_round_err,
                           char* err, size_t errlen);
+/* The values InnerProductFft rounds, per accumulated chunk pair (a, b) with
+ * chunk_bits*(a+b) < 64, a-major: pre[p*2d + j] = backward_out[j].real() / 2d
+ * (polynomial_fft.cc:176-184), before round(), shift and fold; pair_a/pair_b
+ * receive (a, b); *npairs their number (pre holds u_chunks*v_chunks*2d
+ * doubles). fused = 1 evaluates every complex product as a build with FMA
+ * contraction may (fma(xr, wr, -(xi*wi)), fma(xr, wi, xi*wr)). Rounding
+ * analysis only. */
+int orc_inner_product_fft_pairs(const uint64_t* u, const uint64_t* v, int n, int d,
+                                int u_bits, int v_bits, int chunk_bits, int u_is_signed,
+                                int fused, double* pre, int* pair_a, int* pair_b,
+                                int* npairs, char* err, size_t errlen);
+/* Process-wide: fused = 1 makes InnerProductFft and everything built on it
+ * (ExternalProduct, FinalizeMatrixPack, ProcessResponse) evaluate complex
+ * products with FMA contraction, as above; 0 (default) restores the
+ * reference's uncontracted arithmetic. Rounding analysis only. */
+void orc_set_fft_contraction(int fused);
 /* Exact negacyclic inner product mod 2^64 with the same operand
  * interpretation (chunk truncation / sign extension) as InnerProductFft, by
  * 128-bit schoolbook. A closed-form checker for orc_inner_product_fft. */
