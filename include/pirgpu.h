@@ -41,8 +41,16 @@ extern "C" {
  * (crypto/pir/pir_params.h:36-105, crypto/encryption.h:29-98), with the
  * compile-time types of the reference's template instantiation made runtime:
  * db_bytes 2 = uint16_t DB, 1 = uint8_t DB; mat_bytes 4 = int32_t pack
- * matrix (condensed), 2 = int16_t pack matrix (plain). CoeffType is uint64_t.
- * log_q is LogModulus() (64 means modulus 0, i.e. Z_{2^64}). */
+ * matrix (condensed; a plain uint32_t matrix has the same bytes and, with
+ * 32-bit coefficients, the same products), 2 = int16_t pack matrix (plain),
+ * 8 = uint64_t pack matrix (plain; entries must lie in [-2^31, 2^31) as
+ * two's complement, which BuildPreprocessMatrix's centering gives);
+ * coeff_bits 64 (or 0)
+ * = CoeffType uint64_t, 32 = uint32_t (pir_server.cc:806-818). Coefficients
+ * cross the ABI as uint64_t either way; with uint32_t they are the 32-bit
+ * values zero-extended, and the answer wraps where Polynomial<uint32_t>
+ * does. log_q is LogModulus() (64, or 32 for uint32_t, means modulus 0,
+ * i.e. Z_{2^width}). */
 typedef struct {
   int log_degree;
   int log_q;
@@ -57,6 +65,7 @@ typedef struct {
   int entry_size_multiple;
   int db_bytes;
   int mat_bytes;
+  int coeff_bits;
 } pirg_params;
 
 /* View of PirPreprocessedData<Db, uint64_t, Mat>

@@ -91,6 +91,7 @@ class _CParams(ctypes.Structure):
         ("entry_size_multiple", ctypes.c_int),
         ("db_bytes", ctypes.c_int),
         ("mat_bytes", ctypes.c_int),
+        ("coeff_bits", ctypes.c_int),
     ]
 
 
@@ -125,7 +126,8 @@ class _CRequest(ctypes.Structure):
 class PirParams:
     """PirParams<uint64_t> (crypto/pir/pir_params.h:36-105) with the
     reference's template types as fields (db_bytes 2 = uint16_t DB, 1 = uint8_t;
-    mat_bytes 4 = int32_t pack matrix, 2 = int16_t)."""
+    mat_bytes 4 = int32_t pack matrix, 2 = int16_t; coeff_bits 64 = uint64_t
+    coefficients, 32 = uint32_t)."""
 
     log_degree: int = 11
     log_q: int = 52
@@ -142,6 +144,7 @@ class PirParams:
     entry_size_multiple: int = 1
     db_bytes: int = 2
     mat_bytes: int = 4
+    coeff_bits: int = 64
 
     @property
     def degree(self) -> int:

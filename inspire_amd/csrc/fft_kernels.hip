@@ -233,11 +233,12 @@ __global__ void FinalizeCombineKernel(FinalizeArgs A) {
     A.ct_b[(long long)q * A.M + row] = bagg;
     const int s = oi / A.t, i = oi % A.t;
     if (i == A.t - 1) {
-      const u64 a = A.a_tilde[(long long)oi * A.d + m];
+      const u64 a = A.a_tilde[(long long)oi * A.d + m] & A.cmask;
       if (A.t == 1) {
+        // Rescale's result is cast to CoeffType (polynomial.cc:297-301).
         u64* o = A.out + (((long long)q * A.ell + s) * 2) * A.d;
-        o[m] = RescaleCoeff(a, A.log_q, A.log_q1);
-        o[A.d + m] = RescaleCoeff(bagg, A.log_q, A.log_q2);
+        o[m] = RescaleCoeff(a, A.log_q, A.log_q1) & A.cmask;
+        o[A.d + m] = RescaleCoeff(bagg, A.log_q, A.log_q2) & A.cmask;
       } else {
         A.st_a[((long long)q * A.ell + s) * A.d + m] = a;
         A.st_b[((long long)q * A.ell + s) * A.d + m] = bagg;
@@ -433,10 +434,12 @@ __global__ void ExtCombineKernel(ExtCombineArgs A) {
     const long long oi = (long long)s * A.t + A.step_i;
     const u64 c = (which == 0) ? A.a_tilde[oi * A.d + m]
                                : A.ct_b[(q * A.ell * A.t + oi) * A.d + m];
-    v += c;
+    // Horner's unreduced Add wraps at CoeffType's width (pir_server.cc:141-146);
+    // Rescale reads that value and its result is cast to CoeffType.
+    v = (v + c) & A.cmask;
     if (A.step_i == 0) {
       A.out[(qs * 2 + which) * A.d + m] =
-          RescaleCoeff(v, A.log_q, which == 0 ? A.log_q1 : A.log_q2);
+          RescaleCoeff(v, A.log_q, which == 0 ? A.log_q1 : A.log_q2) & A.cmask;
     } else {
       (which == 0 ? A.st_a : A.st_b)[qs * A.d + m] = v;
     }

@@ -55,6 +55,12 @@ Error CheckPackParams(const pirg_params& P, int njobs) {
     return Invalid("log_degree must be in [6, 12] for batched packing.");
   }
   if (P.log_q < 1 || P.log_q > 64) return Invalid("log_q must be in [1, 64].");
+  if (P.coeff_bits != 0 && P.coeff_bits != 32 && P.coeff_bits != 64) {
+    return Invalid("coeff_bits must be 32 or 64.");
+  }
+  if (P.coeff_bits == 32 && P.log_q > 32) {
+    return Invalid("log_q exceeds the coefficient width (32 bits).");
+  }
   if (P.pack_log_digit <= 0 || P.pack_num_digits <= 0) {
     return Invalid("pack gadget parameters must be positive.");
   }
@@ -110,6 +116,7 @@ Error Enqueue(pirg_packer* K, bool timed) {
   // No modulus switch: Rescale(x, log_q, log_q) is the identity on reduced x.
   FA.log_q1 = K->log_q;
   FA.log_q2 = K->log_q;
+  FA.cmask = ~0ULL;  // b_agg is reduced mod q; no Rescale
   PIRG_HIP(LaunchFinalizeCombine(FA, st));
   return Ok();
 }
@@ -137,7 +144,7 @@ Error Create(const pirg_params* params, int njobs, const void* const* pack,
   er = MakeRules(k, d, params->pack_log_digit, params->log_q, ChunkFor(k, d), true, &K->fin_u,
                  &K->fin_v);
   if (!er.ok()) return er;
-  K->fin_pairs = MakePairs(K->fin_u, K->fin_v);
+  K->fin_pairs = MakePairs(K->fin_u, K->fin_v, params->coeff_bits == 32 ? 32 : 64);
   PIRG_HIP(hipGetDevice(&K->device));
   PIRG_HIP(hipStreamCreateWithFlags(&K->stream, hipStreamNonBlocking));
   for (auto& e : K->ev) PIRG_HIP(hipEventCreate(&e));

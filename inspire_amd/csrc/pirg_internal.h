@@ -38,7 +38,7 @@ __host__ __device__ inline double ChunkValue(const ChunkRule& r, u64 x, int c) {
 }
 
 // The (a, b) chunk pairs InnerProductFft accumulates (it skips pairs with
-// chunk_bits*(a+b) >= 64, polynomial_fft.cc:154-173).
+// chunk_bits*(a+b) >= 8*sizeof(CoeffType), polynomial_fft.cc:154-173).
 struct PairList {
   int count;
   int a[64];
@@ -46,11 +46,11 @@ struct PairList {
   int shift[64];
 };
 
-inline PairList MakePairs(const ChunkRule& u, const ChunkRule& v) {
+inline PairList MakePairs(const ChunkRule& u, const ChunkRule& v, int width = 64) {
   PairList p{};
   for (int a = 0; a < u.num_chunks; ++a) {
     for (int b = 0; b < v.num_chunks; ++b) {
-      if (u.chunk_bits * (a + b) >= 64) continue;
+      if (u.chunk_bits * (a + b) >= width) continue;
       p.a[p.count] = a;
       p.b[p.count] = b;
       p.shift[p.count] = u.chunk_bits * (a + b);
@@ -209,6 +209,7 @@ struct FinalizeArgs {
   u64* st_b;               // [nq][ell][d]
   u64* out;                // [nq][ell][2][d] (written when t == 1)
   int nq, M, d, t, ell, log_q, log_q1, log_q2;
+  u64 cmask;               // 2^width - 1 of CoeffType
 };
 
 struct DigitsArgs {
@@ -236,6 +237,7 @@ struct ExtCombineArgs {
   u64* st_b;
   u64* out;                // [nq][ell][2][d]
   int nq, ell, t, d, step_i, log_q, log_q1, log_q2;
+  u64 cmask;               // 2^width - 1 of CoeffType
 };
 
 // outs_per_group: consecutive outputs that share their u spectra (one

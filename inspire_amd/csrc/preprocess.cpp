@@ -439,6 +439,9 @@ Error CheckParams(const pirg_params& P, pirg_server* S) {
   Error er = DeriveServer(P, S);
   if (!er.ok()) return er;
   if (S->log_d > kMaxLogDegree) return Invalid("ring degree too large for the GPU path.");
+  if (P.mat_bytes == 8) {
+    return Invalid("GPU preprocessing writes int32_t or int16_t pack matrices (mat_bytes 4 or 2).");
+  }
   return Ok();
 }
 
@@ -575,6 +578,12 @@ Error DownloadMaterial(const pirg_params& P, const Material& M, int ell, int t, 
   if (t_vec_h != nullptr) {
     PIRG_HIP(CopySync(t_vec_h, M.t_vec_h.p, static_cast<size_t>(ell) * t * k * d * 8,
                       hipMemcpyDeviceToHost, cs.s));
+    // CoeffType uint32_t: t_vec_h's signed digits as Polynomial<uint32_t>
+    // holds them (the other outputs are already below q <= 2^32).
+    if (P.coeff_bits == 32) {
+      const size_t n = static_cast<size_t>(ell) * t * k * d;
+      for (size_t x = 0; x < n; ++x) t_vec_h[x] &= 0xFFFFFFFFULL;
+    }
   }
   return Ok();
 }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -170,8 +171,20 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
       P.log_q2 < 1 || P.log_q2 > 64) {
     return Invalid("log moduli must be in [1, 64].");
   }
+  if (P.coeff_bits != 0 && P.coeff_bits != 32 && P.coeff_bits != 64) {
+    return Invalid("coeff_bits must be 32 or 64.");
+  }
+  const int width = P.coeff_bits == 32 ? 32 : 64;
+  if (P.log_q > width) {
+    return Invalid("log_q exceeds the coefficient width (" + std::to_string(width) + " bits).");
+  }
   if (P.db_bytes != 1 && P.db_bytes != 2) return Invalid("db_bytes must be 1 or 2.");
-  if (P.mat_bytes != 2 && P.mat_bytes != 4) return Invalid("mat_bytes must be 2 or 4.");
+  if (P.mat_bytes != 2 && P.mat_bytes != 4 && P.mat_bytes != 8) {
+    return Invalid("mat_bytes must be 2, 4 or 8.");
+  }
+  if (P.mat_bytes == 8 && width != 64) {
+    return Invalid("mat_bytes 8 (a uint64_t pack matrix) needs 64-bit coefficients.");
+  }
   if (P.pack_num_digits <= 0 || P.eval_num_digits <= 0) {
     return Invalid("log_digit and num_digits must be positive.");
   }
@@ -189,13 +202,15 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
   S->M = S->ell * S->t * d;
   S->log_q = P.log_q;
   S->qmask = P.log_q >= 64 ? ~0ULL : ((1ULL << P.log_q) - 1);
+  S->width = width;
+  S->cmask = width >= 64 ? ~0ULL : ((1ULL << width) - 1);
   // FinalizeMatrixPack's InnerProduct (lwes_to_rlwe.cc:472-474).
   Error er = MakeRules(S->k, d, P.pack_log_digit, P.log_q, ChunkFor(S->k, d), true,
                        &S->fin_u, &S->fin_v);
   if (!er.ok()) {
     S->deferred = er;
   } else {
-    S->fin_pairs = MakePairs(S->fin_u, S->fin_v);
+    S->fin_pairs = MakePairs(S->fin_u, S->fin_v, width);
   }
   if (S->t > 1 && S->deferred.ok()) {
     // ExternalProduct (external_product.cc:100-111).
@@ -204,7 +219,7 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
     if (!er.ok()) {
       S->deferred = er;
     } else {
-      S->ext_pairs = MakePairs(S->ext_u, S->ext_v);
+      S->ext_pairs = MakePairs(S->ext_u, S->ext_v, width);
     }
   }
   if (S->fin_pairs.count > 64 || S->ext_pairs.count > 64) {
@@ -664,6 +679,7 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   FA.log_q = S->log_q;
   FA.log_q1 = S->P.log_q1;
   FA.log_q2 = S->P.log_q2;
+  FA.cmask = S->cmask;
   PIRG_HIP(LaunchFinalizeCombine(FA, st));
   PIRG_HIP(mark(slot.ev[2]));
   // EvalPoly (pir_server.cc:126-150) and the modulus switch.
@@ -717,6 +733,7 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
       EA.log_q = S->log_q;
       EA.log_q1 = S->P.log_q1;
       EA.log_q2 = S->P.log_q2;
+      EA.cmask = S->cmask;
       if (i == 0) PIRG_HIP(LaunchExtCombine(EA, st));
     }
   }
@@ -859,6 +876,46 @@ void ApplyPlanOverrides(pirg_server* S) {
   if (const char* env = std::getenv("PIRG_TALL")) S->tall = std::atoi(env) != 0;
 }
 
+// Matrix<uint64_t> pack blocks (PirServer<Db, uint64_t, uint64_t>,
+// pir_server.cc:806-818) hold BuildPreprocessMatrix's values
+// (lwes_to_rlwe.cc:210-222): NTT outputs over Z_{2^64}, minus 2^log_q when at
+// least 2^(log_q-1), so an entry equals a small integer mod 2^log_q but not
+// necessarily mod 2^64. Only the product mod 2^log_q is read (FinalizeMatrixPack
+// reduces b + b_part), so each entry is reduced mod 2^log_q, centered, and
+// narrowed to the int32_t form (whose condensed words are the row-major int32
+// values); the preprocessing's bound |x| < 2^(pack_log_digit + log d - 1)
+// keeps the centered values inside [-2^31, 2^31).
+Error NarrowU64Pack(const pirg_server* S, const std::vector<const void*>& blocks,
+                    std::vector<std::vector<int32_t>>* store, std::vector<const void*>* out) {
+  const size_t n = static_cast<size_t>(S->d) * S->k * S->d;
+  store->assign(blocks.size(), {});
+  out->assign(blocks.size(), nullptr);
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    if (blocks[b] == nullptr) return Invalid("pack block " + std::to_string(b) + " is null.");
+    const uint64_t* src = static_cast<const uint64_t*>(blocks[b]);
+    std::vector<int32_t>& dst = (*store)[b];
+    dst.resize(n);
+    const int lq = S->log_q;
+    for (size_t x = 0; x < n; ++x) {
+      int64_t v = static_cast<int64_t>(src[x]);
+      if (lq < 64) {
+        const u64 r = src[x] & S->qmask;
+        v = r >= (1ULL << (lq - 1)) ? static_cast<int64_t>(r) - static_cast<int64_t>(1ULL << lq)
+                                    : static_cast<int64_t>(r);
+      }
+      if (v < INT32_MIN || v > INT32_MAX) {
+        return Invalid("pack matrix entry " + std::to_string(x) + " of block " +
+                       std::to_string(b) +
+                       " lies outside [-2^31, 2^31) mod q; the GPU path holds uint64_t "
+                       "pack matrices as int32.");
+      }
+      dst[x] = static_cast<int32_t>(v);
+    }
+    (*out)[b] = dst.data();
+  }
+  return Ok();
+}
+
 Error CreateFromParts(const pirg_params* params, const pirg_preprocessed_view* v,
                       std::unique_ptr<pirg_server>* out) {
   std::unique_ptr<pirg_server> S(new pirg_server());
@@ -867,11 +924,19 @@ Error CreateFromParts(const pirg_params* params, const pirg_preprocessed_view* v
   ApplyPlanOverrides(S.get());
   er = ValidateView(S.get(), v);
   if (!er.ok()) return er;
+  const int lt = S->ell * S->t;
+  std::vector<const void*> pk(v->pack, v->pack + lt);
+  std::vector<std::vector<int32_t>> narrowed;
+  if (S->P.mat_bytes == 8) {
+    std::vector<const void*> pk32;
+    er = NarrowU64Pack(S.get(), pk, &narrowed, &pk32);
+    if (!er.ok()) return er;
+    pk = pk32;
+    S->P.mat_bytes = 4;
+  }
   er = CheckDevice();
   if (!er.ok()) return er;
   std::vector<const uint64_t*> db(v->db_cdata, v->db_cdata + S->ell);
-  const int lt = S->ell * S->t;
-  std::vector<const void*> pk(v->pack, v->pack + lt);
   std::vector<const uint64_t*> at(v->a_tilde, v->a_tilde + lt);
   std::vector<const uint64_t*> th(v->t_vec_h, v->t_vec_h + lt);
   er = BuildResident(S.get(), db, v->db_cols[0], pk, at, th, v->second_dim_a);
@@ -1133,8 +1198,7 @@ int pirg_server_create_flat(const pirg_params* params, const uint64_t* db_cdata,
   const int ell = probe.ell, t = probe.t, d = probe.d, k = probe.k;
   const int lt = ell * t;
   const size_t wpr = params->db_bytes == 2 ? db_cols / 4 : db_cols / 8;
-  const size_t pk_rowbytes = params->mat_bytes == 4 ? static_cast<size_t>(k) * d * 4
-                                                    : static_cast<size_t>(k) * d * 2;
+  const size_t pk_rowbytes = static_cast<size_t>(k) * d * params->mat_bytes;
   std::vector<const uint64_t*> dbp(ell);
   std::vector<int> rows(ell, t * d), cols(ell, db_cols), olen(ell, t);
   for (int s = 0; s < ell; ++s) dbp[s] = db_cdata + static_cast<size_t>(s) * t * d * wpr;

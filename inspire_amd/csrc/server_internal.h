@@ -148,6 +148,9 @@ struct pirg_server {
   int d = 0, log_d = 0, N = 0, log_N = 0, t = 0, ell = 0, k = 0, e = 0;
   int r = 0, K = 0, M = 0, log_q = 0;
   pirg::u64 qmask = 0;
+  // CoeffType: width 64 (uint64_t) or 32 (uint32_t); cmask = 2^width - 1.
+  int width = 64;
+  pirg::u64 cmask = ~0ULL;
   pirg::FftDims F{};
   pirg::ChunkRule fin_u{}, fin_v{}, ext_u{}, ext_v{};
   pirg::PairList fin_pairs{}, ext_pairs{};

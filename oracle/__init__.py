@@ -56,6 +56,7 @@ class _CParams(ctypes.Structure):
         ("entry_size_multiple", ctypes.c_int),
         ("db_bytes", ctypes.c_int),
         ("mat_bytes", ctypes.c_int),
+        ("coeff_bits", ctypes.c_int),
     ]
 
 
@@ -84,6 +85,7 @@ class Params:
     entry_size_multiple: int = 1
     db_bytes: int = 2
     mat_bytes: int = 4
+    coeff_bits: int = 64   # CoeffType: 64 uint64_t, 32 uint32_t
 
     @property
     def d(self) -> int:
@@ -461,6 +463,8 @@ def _alloc_pack(P: Params, rows: int) -> np.ndarray:
     s = sizes(P)
     if P.mat_bytes == 4:
         return np.zeros((rows, s.pack_words_per_row), dtype=np.uint64)
+    if P.mat_bytes == 8:
+        return np.zeros((rows, s.pack_cols), dtype=np.uint64)
     return np.zeros((rows, s.pack_cols), dtype=np.int16)
 
 
@@ -512,7 +516,12 @@ def synthetic_material(P: Params, seed: int) -> Material:
     if P.mat_bytes == 2:
         bound = min(bound, 1 << 15)
     pv = rng.integers(-bound, bound, size=(s.M, s.pack_cols), dtype=np.int64)
-    pack = condense_i32(pv.astype(np.int32)) if P.mat_bytes == 4 else pv.astype(np.int16)
+    if P.mat_bytes == 4:
+        pack = condense_i32(pv.astype(np.int32))
+    elif P.mat_bytes == 8:
+        pack = pv.astype(np.int64).view(np.uint64)
+    else:
+        pack = pv.astype(np.int16)
     qmask = (1 << P.log_q) - 1 if P.log_q < 64 else (1 << 64) - 1
     a_tilde = (rng.integers(0, 1 << 63, size=(s.ell * s.t, s.d), dtype=np.uint64) * 2 +
                rng.integers(0, 2, size=(s.ell * s.t, s.d), dtype=np.uint64)) & np.uint64(qmask)

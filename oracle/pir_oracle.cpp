@@ -692,7 +692,7 @@ Poly InnerProductExactW(const std::vector<Poly>& u, const std::vector<Poly>& v,
 // lwes_to_rlwe.cc:60-74: chunk = min(20, floor((53 - log2 n - log2 d) / 2)).
 Poly InnerProductHelper(const std::vector<Poly>& a, const std::vector<Poly>& b,
                         int a_bits, int b_bits, bool a_is_signed,
-                        double* max_err) {
+                        double* max_err, int width = 64) {
   if (a.size() != b.size() || a.empty()) {
     InvalidArg("InnerProduct: size mismatch or empty");
   }
@@ -701,8 +701,8 @@ Poly InnerProductHelper(const std::vector<Poly>& a, const std::vector<Poly>& b,
        std::log2(static_cast<double>(a[0].size()))) /
       2.0);
   const int chunk_bits = std::min(20, max_allowed);
-  return InnerProductFft(a, b, a_bits, b_bits, chunk_bits, a_is_signed,
-                         max_err);
+  return InnerProductFftW(a, b, a_bits, b_bits, chunk_bits, a_is_signed, width,
+                          max_err);
 }
 
 // ---------------------------------------------------------------------------
@@ -798,17 +798,46 @@ void ParallelRows(int rows, F f) {
 }
 
 // The pack matrix of the server in either representation.
+// Matrix<uint64_t>::Multiply on a plain matrix (matrix.h:182-192): the
+// wraparound sum of entry times vector element.
+void MulU64(const u64* data, int rows, int cols, const u64* vec, u64* out) {
+  for (int i = 0; i < rows; ++i) {
+    const u64* row = data + static_cast<size_t>(i) * cols;
+    u64 acc = 0;
+    for (int j = 0; j < cols; ++j) acc += row[j] * vec[j];
+    out[i] = acc;
+  }
+}
+
 struct PackView {
   const u64* i32_cdata = nullptr;   // condensed int32 (mat_bytes == 4)
   const int16_t* i16 = nullptr;     // plain int16 (mat_bytes == 2)
+  const u64* u64p = nullptr;        // plain uint64 (mat_bytes == 8)
   void Multiply(int row0, int nrows, int cols, const u64* vec, u64* out) const {
     if (i32_cdata != nullptr) {
       CondensedMulI32Rows(i32_cdata, cols, row0, nrows, vec, out);
+    } else if (u64p != nullptr) {
+      MulU64(u64p + static_cast<size_t>(row0) * cols, nrows, cols, vec, out);
     } else {
       MulI16(i16 + static_cast<size_t>(row0) * cols, nrows, cols, vec, out);
     }
   }
 };
+
+// The pack matrix of a PirServer<Db, Coeff, Mat>: int32_t (condensed words,
+// mat_bytes 4; a plain uint32_t matrix has the same bytes), int16_t (2) or
+// uint64_t (8).
+PackView MakePackView(int mat_bytes, const void* pack) {
+  PackView pv;
+  if (mat_bytes == 4) {
+    pv.i32_cdata = static_cast<const u64*>(pack);
+  } else if (mat_bytes == 8) {
+    pv.u64p = static_cast<const u64*>(pack);
+  } else {
+    pv.i16 = static_cast<const int16_t*>(pack);
+  }
+  return pv;
+}
 
 // ---------------------------------------------------------------------------
 // Parameters.
@@ -816,12 +845,20 @@ struct Derived {
   int d, log_d, t, ell, k, e, r, S, K, M, n;
   int log_q, log_q1, log_q2;
   u64 qmask;
+  int width;  // bits of CoeffType
+  u64 wmask;  // 2^width - 1
 };
 
 Derived Derive(const orc_params* P) {
   if (P->log_degree < 1 || P->log_degree > 16) InvalidArg("Invalid log_degree.");
   if (P->log_q < 1 || P->log_q > 64) InvalidArg("Invalid log_q.");
+  if (P->coeff_bits != 0 && P->coeff_bits != 32 && P->coeff_bits != 64) {
+    InvalidArg("coeff_bits must be 32 or 64.");
+  }
   Derived D;
+  D.width = P->coeff_bits == 32 ? 32 : 64;
+  if (P->log_q > D.width) InvalidArg("log_q exceeds the coefficient width.");
+  D.wmask = D.width == 64 ? ~0ULL : ((1ULL << D.width) - 1);
   D.d = 1 << P->log_degree;
   D.log_d = P->log_degree;
   D.t = P->interp_degree;
@@ -864,12 +901,17 @@ void FinalizeMatrixPack(const orc_params* P, int d, const u64* b,
                         const u64* b_part, const std::vector<Poly>& y_vec_h,
                         const std::vector<Poly>& t_vec_h, Poly* out_b,
                         double* max_err) {
+  // CoeffType uint32_t: the sums wrap at 2^32 and the inner product is
+  // Polynomial<uint32_t>'s; LowBits to log_q <= 32 then reads the same bits.
+  const int width = P->coeff_bits == 32 ? 32 : 64;
+  const u64 wm = WidthMask(width);
   Poly b_agg(d);
-  for (int m = 0; m < d; ++m) b_agg[m] = b[m] + b_part[m];
+  for (int m = 0; m < d; ++m) b_agg[m] = (b[m] + b_part[m]) & wm;
   b_agg = LowBits(b_agg, P->log_q);
   Poly inner_h = InnerProductHelper(t_vec_h, y_vec_h, P->pack_log_digit,
-                                    P->log_q, true, max_err);
+                                    P->log_q, true, max_err, width);
   b_agg = Add(b_agg, inner_h);
+  for (auto& x : b_agg) x &= wm;
   *out_b = LowBits(b_agg, P->log_q);
 }
 
@@ -1014,7 +1056,9 @@ std::vector<i64> BuildPreprocessMatrix(const PreprocessPackOut& pp, int d,
         u64 val = fp[r];
         if (val >= threshold && log_q < 64) val = val - (1ULL << log_q);
         i64 v;
-        if (mat_bytes == 4) {
+        if (mat_bytes == 8) {
+          v = static_cast<i64>(val);
+        } else if (mat_bytes == 4) {
           v = static_cast<int32_t>(static_cast<uint32_t>(val));
         } else {
           v = static_cast<int16_t>(static_cast<uint16_t>(val));
@@ -1046,6 +1090,9 @@ void StorePack(const orc_params* P, const std::vector<i64>& m, int rows,
   if (P->mat_bytes == 4) {
     CondenseI32(m.data(), rows, cols,
                 static_cast<u64*>(dst) + row_offset * (cols / 2));
+  } else if (P->mat_bytes == 8) {
+    u64* o = static_cast<u64*>(dst) + row_offset * cols;
+    for (size_t i = 0; i < m.size(); ++i) o[i] = static_cast<u64>(m[i]);
   } else {
     int16_t* o = static_cast<int16_t*>(dst) + row_offset * cols;
     for (size_t i = 0; i < m.size(); ++i) o[i] = static_cast<int16_t>(m[i]);
@@ -1121,10 +1168,11 @@ void ExternalProduct(const orc_params* P, int d, const Poly& ct_a,
        std::log2(static_cast<double>(d))) /
       2.0);
   const int chunk_bits = std::min(20, max_allowed);
-  *out_a = InnerProductFft(u_joined, va, P->eval_log_digit, log_q, chunk_bits,
-                           false, max_err);
-  *out_b = InnerProductFft(u_joined, vb, P->eval_log_digit, log_q, chunk_bits,
-                           false, max_err);
+  const int width = P->coeff_bits == 32 ? 32 : 64;
+  *out_a = InnerProductFftW(u_joined, va, P->eval_log_digit, log_q, chunk_bits,
+                            false, width, max_err);
+  *out_b = InnerProductFftW(u_joined, vb, P->eval_log_digit, log_q, chunk_bits,
+                            false, width, max_err);
 }
 
 double NowMs() {
@@ -1780,12 +1828,7 @@ int orc_matrix_pack(const orc_params* P, const uint64_t* b,
   ORC_GUARD_BEGIN
   Derived D = Derive(P);
   const int cols = D.k * D.d;
-  PackView pv;
-  if (P->mat_bytes == 4) {
-    pv.i32_cdata = pack_matrix;
-  } else {
-    pv.i16 = reinterpret_cast<const int16_t*>(pack_matrix);
-  }
+  const PackView pv = MakePackView(P->mat_bytes, pack_matrix);
   Poly part(D.d);
   pv.Multiply(0, D.d, cols, y_vec_g, part.data());
   Poly ob;
@@ -1882,16 +1925,15 @@ int orc_process_response(const orc_params* P, const uint64_t* db_cdata,
   for (int l = 0; l < k; ++l) {
     for (int m = 0; m < d; ++m) y[static_cast<size_t>(l) * d + m] = y_vec_g[l][m];
   }
-  PackView pv;
-  if (P->mat_bytes == 4) {
-    pv.i32_cdata = static_cast<const u64*>(pack);
-  } else {
-    pv.i16 = static_cast<const int16_t*>(pack);
-  }
+  const PackView pv = MakePackView(P->mat_bytes, pack);
   std::vector<u64> b_part(D.M);
   ParallelRows(D.M, [&](int r0, int nr) {
     pv.Multiply(r0, nr, k * d, y.data(), b_part.data() + r0);
   });
+  // CoeffType uint32_t: both products are Polynomial<uint32_t> sums, i.e.
+  // the 64-bit sums mod 2^32 (wraparound arithmetic is a ring map).
+  for (auto& x : b_prime) x &= D.wmask;
+  for (auto& x : b_part) x &= D.wmask;
   double pack_ms = NowMs() - t1, eval_ms = 0.0;
   // pir_server.cc:256-303.
   for (int s = 0; s < ell; ++s) {
@@ -1904,6 +1946,7 @@ int orc_process_response(const orc_params* P, const uint64_t* db_cdata,
       FinalizeMatrixPack(P, d, b_prime.data() + off, b_part.data() + off, y_vec_h,
                          SplitPolys(t_vec_h + oi * k * d, k, d), &ob, &me);
       cts_a[i].assign(a_tilde + oi * d, a_tilde + (oi + 1) * d);
+      for (auto& x : cts_a[i]) x &= D.wmask;
       cts_b[i] = std::move(ob);
     }
     double te = NowMs();
@@ -1915,11 +1958,16 @@ int orc_process_response(const orc_params* P, const uint64_t* db_cdata,
       ExternalProduct(P, d, ra, rb, rgsw_a, rgsw_b, &pa, &pb, &me);
       ra = Add(pa, cts_a[i]);
       rb = Add(pb, cts_b[i]);
+      for (auto& x : ra) x &= D.wmask;  // uint32_t Add wraps at 2^32
+      for (auto& x : rb) x &= D.wmask;
     }
     eval_ms += NowMs() - te;
     // pir_server.cc:288-297.
+    // Rescale's result is cast to CoeffType (polynomial.cc:297-301).
     Poly oa = Rescale(ra, D.log_q, D.log_q1);
     Poly ob = Rescale(rb, D.log_q, D.log_q2);
+    for (auto& x : oa) x &= D.wmask;
+    for (auto& x : ob) x &= D.wmask;
     std::memcpy(out + static_cast<size_t>(s) * 2 * d, oa.data(), sizeof(u64) * d);
     std::memcpy(out + static_cast<size_t>(s) * 2 * d + d, ob.data(), sizeof(u64) * d);
   }
@@ -2094,6 +2142,15 @@ int orc_preprocess(const orc_params* P, const uint16_t* db_raw,
         dst[w] = word;
       }
     }
+  }
+  // CoeffType uint32_t: the material is Polynomial<uint32_t>. Every step
+  // above is a ring operation mod 2^64 or reads only bits below log_q <= 32,
+  // so the uint32_t material is this material mod 2^32 (a_tilde and
+  // second_dim_a are already below q; t_vec_h's signed digits wrap).
+  if (D.width < 64) {
+    const size_t lt = static_cast<size_t>(D.ell) * D.t;
+    for (size_t x = 0; x < lt * d; ++x) a_tilde[x] &= D.wmask;
+    for (size_t x = 0; x < lt * D.k * d; ++x) t_vec_h[x] &= D.wmask;
   }
   if (t > 1 && second_dim_a != nullptr) {
     for (int j = 0; j < 2 * D.e; ++j) {
@@ -2451,12 +2508,7 @@ int orc_vpir_process(const orc_params* P, int local_finalize,
     CondensedMulU16Rows(db_cdata, db_cols, 0, rows, first_dim, b_prime.data());
   }
   // Part 2: pack * y_vec_g.
-  PackView pv;
-  if (P->mat_bytes == 4) {
-    pv.i32_cdata = static_cast<const u64*>(pack);
-  } else {
-    pv.i16 = static_cast<const int16_t*>(pack);
-  }
+  const PackView pv = MakePackView(P->mat_bytes, pack);
   std::vector<u64> b_part(rows);
   pv.Multiply(0, rows, k * d, packing_key, b_part.data());
   // Part 3: complete packing.
@@ -2501,12 +2553,7 @@ int orc_vpir_process_matmul(const orc_params* P, int local_finalize,
   } else {
     CondensedMulU16Rows(db_cdata, db_cols, 0, rows, first_dim, b_prime.data());
   }
-  PackView pv;
-  if (P->mat_bytes == 4) {
-    pv.i32_cdata = static_cast<const u64*>(pack);
-  } else {
-    pv.i16 = static_cast<const int16_t*>(pack);
-  }
+  const PackView pv = MakePackView(P->mat_bytes, pack);
   std::vector<u64> b_part(rows);
   pv.Multiply(0, rows, k * d, packing_key, b_part.data());
   std::vector<u64> h(rows, 0);

@@ -10,9 +10,11 @@
  * baseline leg may load it; the product library (inspire_amd) never does.
  *
  * Every function cites the reference file:line it restates. Coefficients are
- * uint64_t (the reference's CoeffType=uint64_t instantiation); log_q is the
- * bit width of the power-of-two modulus (64 means q = 0, i.e. arithmetic mod
- * 2^64, as in RlweParams::LogModulus, crypto/encryption.h:46-49).
+ * held in uint64_t; orc_params.coeff_bits selects the reference's CoeffType
+ * (uint64_t, or uint32_t: every coefficient then wraps mod 2^32). log_q is
+ * the bit width of the power-of-two modulus (64, or 32 for uint32_t, means
+ * q = 0, i.e. arithmetic mod 2^width, as in RlweParams::LogModulus,
+ * crypto/encryption.h:46-49).
  *
  * Return convention: 0 = OK, otherwise an absl::StatusCode number
  * (3 INVALID_ARGUMENT, 9 FAILED_PRECONDITION, 13 INTERNAL) with a message
@@ -44,6 +46,7 @@ typedef struct {
   int entry_size_multiple; /* ell */
   int db_bytes;         /* 1 (uint8 DB) or 2 (uint16 DB) */
   int mat_bytes;        /* 4 (int32, condensed) or 2 (int16, plain) */
+  int coeff_bits;       /* CoeffType: 64 (or 0) uint64_t, 32 uint32_t */
 } orc_params;
 
 /* Derived sizes (crypto/pir/pir_params.h:71-83, preprocessing_server.h:127-146). */

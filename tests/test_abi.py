@@ -265,3 +265,44 @@ def test_preprocess_without_device_fails_loudly():
         ia.preprocess(to_gpu_params(P), db, first, pack_a)
     except ia.PirError as e:
         assert e.code == ia.UNAVAILABLE
+
+
+@pytest.mark.parametrize("over,msg", [
+    (dict(coeff_bits=16), "coeff_bits must be 32 or 64."),
+    (dict(coeff_bits=32), "log_q exceeds the coefficient width (32 bits)."),
+    (dict(coeff_bits=32, log_q=32, log_q1=31, log_q2=30, mat_bytes=8),
+     "mat_bytes 8 (a uint64_t pack matrix) needs 64-bit coefficients."),
+])
+def test_coefficient_type_validation(over, msg):
+    # CoeffType is uint32_t or uint64_t (pir_server.cc:806-818); a uint32_t
+    # modulus has at most 32 bits (RlweParams::LogModulus, encryption.h:46-49).
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62"])
+    gp = to_gpu_params(P)
+    gp = ia.PirParams(**{**{f: getattr(gp, f) for f in ia.PirParams.__dataclass_fields__},
+                         **over})
+    a = _view_args(P)
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.create(gp, a["db_matrices"], a["db_cols"], a["pack"], a["a_tilde"],
+                            a["t_vec_h"], a["second_dim_a"])
+    assert ei.value.code == ia.INVALID_ARGUMENT
+    assert ei.value.message == msg
+
+
+def test_uint64_pack_matrix_outside_int32_is_rejected_before_the_device():
+    # PirServer<Db, uint64_t, uint64_t>: entries are held as int32 on the GPU,
+    # which BuildPreprocessMatrix's centered values always fit.
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62Interp1"])
+    gp = ia.PirParams(**{**{f: getattr(to_gpu_params(P), f)
+                            for f in ia.PirParams.__dataclass_fields__}, "mat_bytes": 8})
+    s = O.sizes(P)
+    a = _view_args(P)
+    pack = [[np.zeros((s.d, s.k * s.d), np.uint64) for _ in range(s.t)] for _ in range(s.ell)]
+    pack[0][0][3, 5] = np.uint64((1 << 64) - (1 << 31))      # -2^31: fits
+    pack[0][0][4, 1] = np.uint64((1 << 64) - (1 << 62) - 16)  # -16 mod 2^62: fits
+    pack[0][0][7, 9] = np.uint64(1 << 31)                     # 2^31: does not
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.create(gp, a["db_matrices"], a["db_cols"], pack, a["a_tilde"],
+                            a["t_vec_h"], a["second_dim_a"])
+    assert ei.value.code == ia.INVALID_ARGUMENT
+    assert ei.value.message.startswith(f"pack matrix entry {7 * s.k * s.d + 9} of block 0 lies "
+                                       "outside [-2^31, 2^31) mod q")

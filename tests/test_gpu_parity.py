@@ -5,8 +5,8 @@ import pytest
 
 import inspire_amd as ia
 import oracle as O
-from tests.helpers import (PIR_SERVER_TEST_TYPES, gpu_request, pir_server_test_params,
-                           server_from_material)
+from tests.helpers import (PIR_SERVER_TEST_TYPES, PIR_SERVER_TEST_TYPES_U32, gpu_request,
+                           pir_server_test_params, server_from_material)
 
 pytestmark = pytest.mark.gpu
 
@@ -77,10 +77,16 @@ def _compare(P, mat, reqs):
             np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("name", sorted(PIR_SERVER_TEST_TYPES))
+ALL_TYPES = {**{n: (v, 64) for n, v in PIR_SERVER_TEST_TYPES.items()},
+             **{n: (v, 32) for n, v in PIR_SERVER_TEST_TYPES_U32.items()}}
+
+
+@pytest.mark.parametrize("name", sorted(ALL_TYPES))
 def test_process_pir_server_test_types(name):
-    # crypto/pir/server/pir_server_test.cc:246-275 with real preprocessing.
-    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES[name])
+    # crypto/pir/server/pir_server_test.cc:246-275 with real preprocessing,
+    # for all nine typed configurations (:55-143), uint32_t and uint64_t.
+    vals, coeff_bits = ALL_TYPES[name]
+    P = pir_server_test_params(*vals, coeff_bits=coeff_bits)
     s = O.sizes(P)
     db = O.fill_db_u16(11, P.num_entries * s.d * s.ell, P.plaintext_modulus)
     mat = O.preprocess(P, db, public_seed=5)
@@ -94,6 +100,31 @@ def test_process_pir_server_test_types(name):
             msg = O.client_decrypt(P, 900 + idx, got)
             np.testing.assert_array_equal(msg.astype(np.uint16),
                                           db[idx * s.d * s.ell:(idx + 1) * s.d * s.ell])
+
+
+@pytest.mark.parametrize("name,coeff_bits,mat_bytes", [
+    ("Uint64Mod62", 64, 8), ("Uint64Mod0", 64, 8),                 # Mat = uint64_t
+    ("Uint32Mod30", 32, 4), ("Uint32Mod0Interp1", 32, 4),          # Mat = uint32_t / int32_t
+])
+def test_unsigned_and_int32_pack_matrices(name, coeff_bits, mat_bytes):
+    # PirServer<uint16_t, Coeff, Mat> with Mat = CoeffType (pir_server.cc:806,
+    # :807) and <uint16_t, uint32_t, int32_t>: a uint32_t matrix has the bytes
+    # of the condensed int32_t one; a uint64_t matrix holds the NTT outputs
+    # over Z_2^64 and is reduced mod q on the way in.
+    types = PIR_SERVER_TEST_TYPES if coeff_bits == 64 else PIR_SERVER_TEST_TYPES_U32
+    lq, q1, q2, t, _ = types[name]
+    P = pir_server_test_params(lq, q1, q2, t, mat_bytes, coeff_bits=coeff_bits)
+    s = O.sizes(P)
+    db = O.fill_db_u16(13, P.num_entries * s.d * s.ell, P.plaintext_modulus)
+    mat = O.preprocess(P, db, public_seed=6)
+    with server_from_material(P, mat) as srv:
+        for idx in (0, 17, P.num_entries - 1):
+            req = O.client_request(P, 6, 500 + idx, idx)
+            want, _, _ = O.process_response(P, mat, req)
+            got = srv.process_response(gpu_request(req)).ciphertexts
+            np.testing.assert_array_equal(got, want)
+            msg = O.client_decrypt(P, 500 + idx, got)
+            np.testing.assert_array_equal(msg.astype(np.uint16), db[idx * s.d:(idx + 1) * s.d])
 
 
 @pytest.mark.parametrize("log_d,n,t,ell", [(8, 512, 2, 1), (8, 512, 4, 2), (11, 4096, 2, 1),

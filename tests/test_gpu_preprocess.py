@@ -7,7 +7,8 @@ import pytest
 
 import inspire_amd as ia
 import oracle as O
-from helpers import PIR_SERVER_TEST_TYPES, gpu_request, pir_server_test_params, to_gpu_params
+from helpers import (PIR_SERVER_TEST_TYPES, PIR_SERVER_TEST_TYPES_U32, gpu_request,
+                     pir_server_test_params, to_gpu_params)
 
 pytestmark = pytest.mark.gpu
 
@@ -142,6 +143,18 @@ def test_preprocess_matches_oracle(name, ell, n):
     s = O.sizes(P)
     db = O.fill_db_u16(17, P.num_entries * s.d * s.ell, P.plaintext_modulus)
     _check_material(_gpu_preprocess(P, db, 9), O.preprocess(P, db, public_seed=9))
+
+
+@pytest.mark.parametrize("name", sorted(PIR_SERVER_TEST_TYPES_U32))
+def test_preprocess_uint32_types_match_oracle(name):
+    # PreprocessingServer<uint16_t, uint32_t, int16_t> of pir_server_test.cc's
+    # Uint32 types: the material Polynomial<uint32_t> holds.
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES_U32[name], coeff_bits=32)
+    s = O.sizes(P)
+    db = O.fill_db_u16(19, P.num_entries * s.d * s.ell, P.plaintext_modulus)
+    got = _gpu_preprocess(P, db, 9)
+    assert int(got["t_vec_h"].max()) < 1 << 32
+    _check_material(got, O.preprocess(P, db, public_seed=9))
 
 
 @pytest.mark.parametrize("t,ell,mat_bytes", [(4, 1, 4), (1, 2, 2), (8, 2, 4)])

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from tests.helpers import PIR_SERVER_TEST_TYPES, pir_server_test_params
+from tests.helpers import PIR_SERVER_TEST_TYPES, PIR_SERVER_TEST_TYPES_U32, pir_server_test_params
 
 
 def _e2e(P, idxs, db_seed=1, pub=77):
@@ -31,6 +31,74 @@ def test_end_to_end(name):
     P = pir_server_test_params(*PIR_SERVER_TEST_TYPES[name])
     idxs = [0, 1, 10, P.num_entries - 2, P.num_entries - 1] + list(range(0, P.num_entries, 211))
     _e2e(P, idxs)
+
+
+@pytest.mark.parametrize("name", sorted(PIR_SERVER_TEST_TYPES_U32))
+def test_end_to_end_uint32(name):
+    # pir_server_test.cc:246-275 for the CoeffType = uint32_t types (:55-93):
+    # every coefficient wraps at 2^32 and the answers decrypt.
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES_U32[name], coeff_bits=32)
+    idxs = [0, 1, 10, P.num_entries - 2, P.num_entries - 1] + list(range(0, P.num_entries, 211))
+    _e2e(P, idxs)
+
+
+@pytest.mark.parametrize("name", sorted(PIR_SERVER_TEST_TYPES_U32))
+def test_uint32_answers_are_32_bit_and_differ_from_uint64_only_through_wraps(name):
+    # With t > 1 the Horner state and Rescale see the 2^32 wrap, so the
+    # uint64_t arithmetic on the same inputs answers differently; with t = 1
+    # every value before Rescale is already reduced mod q <= 2^32.
+    import dataclasses
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES_U32[name], coeff_bits=32)
+    s = O.sizes(P)
+    db = O.fill_db_u16(2, P.num_entries * s.d * s.ell, P.plaintext_modulus)
+    mat = O.preprocess(P, db, public_seed=3)
+    assert int(mat.t_vec_h.max()) < 1 << 32 and int(mat.a_tilde.max()) < 1 << 32
+    req = O.client_request(P, 3, 44, 7)
+    got, _, _ = O.process_response(P, mat, req)
+    assert int(got.max()) < 1 << 32
+    wide, _, _ = O.process_response(dataclasses.replace(P, coeff_bits=64), mat, req)
+    if P.interp_degree > 1:
+        assert not np.array_equal(got, wide)
+    else:
+        np.testing.assert_array_equal(got, wide)
+
+
+@pytest.mark.parametrize("name", ["Uint64Mod62", "Uint64Mod0"])
+def test_uint64_pack_matrix_answers_as_the_signed_one(name):
+    # PirServer<uint16_t, uint64_t, uint64_t> (pir_server.cc:806-818): the
+    # pack matrix holds BuildPreprocessMatrix's centered values in two's
+    # complement (lwes_to_rlwe.cc:210-222), so its products equal the int32_t
+    # matrix's mod 2^64 and the answers are the same, and decrypt.
+    import dataclasses
+    lq, q1, q2, t, _ = PIR_SERVER_TEST_TYPES[name]
+    P4 = pir_server_test_params(lq, q1, q2, t, 4)
+    P8 = dataclasses.replace(P4, mat_bytes=8)
+    s = O.sizes(P4)
+    db = O.fill_db_u16(4, P4.num_entries * s.d * s.ell, P4.plaintext_modulus)
+    m4 = O.preprocess(P4, db, public_seed=9)
+    m8 = O.preprocess(P8, db, public_seed=9)
+    # The entries are NTT outputs over Z_2^64 (not reduced mod q): equal to
+    # the int32 matrix's values mod 2^log_q, whose condensed words are the
+    # row-major int32 values.
+    assert m8.pack.shape == (s.M, s.pack_cols)
+    q = 1 << P4.log_q if P4.log_q < 64 else 1 << 64
+    centered = [((int(x) % q) + q // 2) % q - q // 2 for x in m8.pack.reshape(-1)[:4096]]
+    np.testing.assert_array_equal(np.array(centered, dtype=np.int64),
+                                  m4.pack.view(np.int32).reshape(-1)[:4096].astype(np.int64))
+    for idx in (0, 5, P4.num_entries - 1):
+        req = O.client_request(P4, 9, 70 + idx, idx)
+        a4, _, _ = O.process_response(P4, m4, req)
+        a8, _, _ = O.process_response(P8, m8, req)
+        np.testing.assert_array_equal(a8, a4)
+        msg = O.client_decrypt(P8, 70 + idx, a8)
+        np.testing.assert_array_equal(msg.astype(np.uint16), db[idx * s.d:(idx + 1) * s.d])
+
+
+def test_uint32_rejects_log_q_above_32():
+    import dataclasses
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62"], coeff_bits=32)
+    with pytest.raises(O.OracleError, match="coefficient width"):
+        O.sizes(dataclasses.replace(P))
 
 
 @pytest.mark.parametrize("name", ["Uint64Mod62", "Uint64Mod0Interp1"])
