@@ -183,7 +183,10 @@ def run_serving(rates, seconds: float, seed: int = 5):
 
                 def poller():
                     while len(done) < n:
-                        r = srv.poll(30000)
+                        try:
+                            r = srv.poll(30000)
+                        except ia.PirError:   # the loop was stopped
+                            break
                         if r is None:
                             break
                         done.append(r)
@@ -191,12 +194,17 @@ def run_serving(rates, seconds: float, seed: int = 5):
                 th = threading.Thread(target=poller)
                 th.start()
                 t0 = time.perf_counter()
-                for i in range(n):
-                    delay = t0 + arrivals[i] - time.perf_counter()
-                    if delay > 0:
-                        time.sleep(delay)
-                    srv.submit(pool[i % len(pool)], tag=i)
-                th.join()
+                try:
+                    for i in range(n):
+                        delay = t0 + arrivals[i] - time.perf_counter()
+                        if delay > 0:
+                            time.sleep(delay)
+                        srv.submit(pool[i % len(pool)], tag=i)
+                except BaseException:
+                    srv.serve_stop()          # wakes the poller
+                    raise
+                finally:
+                    th.join()
                 wall = time.perf_counter() - t0
                 st = srv.serve_stats()
                 bad = sum(1 for r in done if r[2] != 0)

@@ -16,8 +16,9 @@
  *
  * Threading: a pirg_server is externally synchronised (like the reference,
  * whose ProcessResponse is not safe to call concurrently), except for the
- * serving-loop calls pirg_serve_submit / pirg_serve_poll / pirg_serve_stats,
- * which are thread-safe and may run beside the other calls.
+ * serving-loop calls pirg_serve_submit / pirg_serve_poll / pirg_serve_stats /
+ * pirg_serve_stop, which are thread-safe and may run beside each other and
+ * beside the other calls (pirg_server_destroy excepted).
  */
 #ifndef PIRGPU_H_
 #define PIRGPU_H_
@@ -196,20 +197,25 @@ int pirg_serve_submit(pirg_server* server, const pirg_request_view* req,
 /* Waits up to timeout_ms for a finished request: returns 0 and fills tag,
  * status (its ProcessResponse status), latency_ms (submit to completion) and
  * out (ell x 2 x d, when status == 0); returns PIRG_DEADLINE_EXCEEDED when
- * none finished in time. Thread-safe. */
+ * none finished in time, and PIRG_FAILED_PRECONDITION when the loop is not
+ * running or was stopped while the call waited. Thread-safe. */
 int pirg_serve_poll(pirg_server* server, int timeout_ms, uint64_t* tag,
                     uint64_t* out, int* status, double* latency_ms, char* err,
                     size_t errlen);
 typedef struct {
   long long submitted, answered, batches;
   double mean_batch;            /* requests per launched batch */
-  double p50_ms, p99_ms, max_ms; /* submit-to-completion latency */
+  double p50_ms, p99_ms, max_ms; /* submit-to-completion latency, over the
+                                    latest 2^20 answers */
   double elapsed_ms;            /* since start or the last reset */
   int slots, max_batch;         /* after admission by free HBM */
 } pirg_serve_stats_t;
 int pirg_serve_stats(pirg_server* server, pirg_serve_stats_t* stats, int reset);
-/* Answers everything submitted, then stops the worker (unpolled results are
- * dropped). pirg_server_destroy stops a running loop itself. */
+/* Answers everything submitted, then stops the worker; threads waiting in
+ * pirg_serve_poll return (with a result still undelivered, or
+ * PIRG_FAILED_PRECONDITION), and results nobody polls are dropped. Calls
+ * already inside the loop finish safely: the loop is freed when the last of
+ * them returns. pirg_server_destroy stops a running loop itself. */
 int pirg_serve_stop(pirg_server* server);
 
 /* ---- Batched LWE-to-RLWE packing. MatrixPack (crypto/lwes_to_rlwe.cc:481-508,
@@ -357,6 +363,29 @@ int pirg_poly_from_proto_device(const uint64_t* d_words, long long pitch_words, 
  * in the reference. */
 int pirg_server_load_from_buffer(const pirg_params* params, const uint8_t* buffer,
                                  size_t size, pirg_server** out, char* err, size_t errlen);
+
+/* PirServer::LoadFromFile (pir_server.cc:794-804): LoadFromBuffer on the
+ * file's bytes (mapped, not copied). An unreadable file fails with "Failed to
+ * read file." (INTERNAL), as in the reference. */
+int pirg_server_load_from_file(const pirg_params* params, const char* file_path,
+                               pirg_server** out, char* err, size_t errlen);
+
+/* PirServer::SaveToFile (pir_server.cc:433-592): the image LoadFromBuffer
+ * reads, byte for byte as the reference writes it for the same material,
+ * written from the resident server (DB words and pack entries rebuilt from
+ * their HBM tiles and bit-packed on the GPU). Errors: "Failed to open file
+ * for writing: <path>", "Error writing to file: <path>" (INTERNAL). A
+ * server given a plain uint32_t pack matrix under mat_bytes 4 is written
+ * with the condensed int32_t header (5 bytes differ from the reference's
+ * Matrix<uint32_t> image; the entries are the same). */
+int pirg_server_save_to_file(pirg_server* server, const char* file_path, char* err,
+                             size_t errlen);
+/* The same image into a caller's buffer: *size receives its length; with
+ * buffer == NULL nothing else happens (no device work), otherwise capacity
+ * must be at least that length (INVALID_ARGUMENT). No reference counterpart:
+ * for callers that keep the image in memory. */
+int pirg_server_save_to_buffer(pirg_server* server, uint8_t* buffer, size_t capacity,
+                               size_t* size, char* err, size_t errlen);
 
 /* DeserializePirPreprocessedData + PirServer::Create
  * (crypto/pir/server/pir_preprocessed_data.h:93-148, pir_server.cc:52-124) on

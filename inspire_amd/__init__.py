@@ -345,6 +345,44 @@ class PirServer:
         return cls(handle, params)
 
     @classmethod
+    def load_from_file(cls, params: PirParams, path: str) -> "PirServer":
+        """PirServer::LoadFromFile (pir_server.cc:794-804)."""
+        handle = ctypes.c_void_p()
+        err = _err()
+        cp = params.c()
+        rc = lib().pirg_server_load_from_file(ctypes.byref(cp), os.fsencode(path),
+                                              ctypes.byref(handle), err,
+                                              ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return cls(handle, params)
+
+    def save_to_file(self, path: str) -> None:
+        """PirServer::SaveToFile (pir_server.cc:433-592), written from the
+        resident material."""
+        err = _err()
+        _check(lib().pirg_server_save_to_file(self._h, os.fsencode(path), err,
+                                              ctypes.c_size_t(len(err))), err)
+
+    def image_size(self) -> int:
+        """Bytes of the SaveToFile image (no device work)."""
+        n = ctypes.c_size_t()
+        err = _err()
+        _check(lib().pirg_server_save_to_buffer(self._h, None, ctypes.c_size_t(0), ctypes.byref(n),
+                                                err, ctypes.c_size_t(len(err))), err)
+        return n.value
+
+    def save_to_buffer(self) -> np.ndarray:
+        """The SaveToFile image as a uint8 array (pirg_server_save_to_buffer)."""
+        size = self.image_size()
+        out = np.empty(max(size, 1), dtype=np.uint8)
+        n = ctypes.c_size_t()
+        err = _err()
+        _check(lib().pirg_server_save_to_buffer(self._h, out.ctypes.data_as(ctypes.c_void_p),
+                                                ctypes.c_size_t(size), ctypes.byref(n), err,
+                                                ctypes.c_size_t(len(err))), err)
+        return out[:size]
+
+    @classmethod
     def create_preprocessed(cls, params: PirParams, db_raw: np.ndarray,
                             first_dim_a: np.ndarray, second_dim_a: Optional[np.ndarray],
                             pack_a: np.ndarray) -> "PirServer":

@@ -125,6 +125,45 @@ __global__ void PackToTilesKernel(const void* __restrict__ pack, int mat_bytes,
   }
 }
 
+// The inverse of the two conversions above, for SaveToFile: rows
+// [row0, row0 + nrows) of a tiled matrix back to row-major values (out is
+// nrows x cols, sign-extended to 64 bits when the top plane is signed, then
+// ANDed with mask). One thread per (row, k-block, 16-column group) reads that
+// group's 16 bytes in every plane.
+__global__ void TilesToValuesKernel(const int8_t* __restrict__ tiles, int cols, int row0,
+                                    int nrows, int kblocks, int planes, int signed_top,
+                                    u64 mask, u64* __restrict__ out) {
+  const long long total = static_cast<long long>(nrows) * kblocks * 4;
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
+       g += (long long)gridDim.x * blockDim.x) {
+    const int lg = static_cast<int>(g & 3);
+    const long long t = g >> 2;
+    const int kb = static_cast<int>(t % kblocks);
+    const int rr = static_cast<int>(t / kblocks);
+    const long long row = static_cast<long long>(row0) + rr;
+    const long long strip = row / 16;
+    const int lane = static_cast<int>(row % 16) + 16 * lg;
+    long long vals[16];
+    for (int b = 0; b < 16; ++b) vals[b] = 0;
+    for (int p = 0; p < planes; ++p) {
+      int8_t in[16];
+      *reinterpret_cast<int4*>(in) = *reinterpret_cast<const int4*>(
+          tiles + (TileIndex(strip, kb, p, kblocks, planes) * 64 + lane) * 16);
+      const bool top = signed_top && p == planes - 1;
+      for (int b = 0; b < 16; ++b) {
+        const long long byte = top ? static_cast<long long>(in[b])
+                                   : static_cast<long long>(static_cast<int>(in[b]) + 128);
+        vals[b] += byte * (1LL << (8 * p));
+      }
+    }
+    const int c0 = kb * 64 + 16 * lg;
+    u64* dst = out + static_cast<long long>(rr) * cols;
+    for (int b = 0; b < 16; ++b) {
+      if (c0 + b < cols) dst[c0 + b] = static_cast<u64>(vals[b]) & mask;
+    }
+  }
+}
+
 __device__ inline int BalancedDigit(u64 y, int c) {
   u64 v = y;
   int dig = 0;
@@ -1124,6 +1163,17 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
   hipLaunchKernelGGL(PackToTilesKernel, dim3(GridBlocks(total, 256)), dim3(256),
                      0, s, pack, mat_bytes, rows, cols, row0, nrows, L.strips,
                      L.kblocks, L.planes, tiles);
+  return hipGetLastError();
+}
+
+hipError_t LaunchTilesToValues(const int8_t* tiles, const LimbLayout& L, int row0, int nrows,
+                              bool signed_top, u64 mask, u64* out, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  if (row0 < 0 || row0 + nrows > L.rows) return hipErrorInvalidValue;
+  const long long total = static_cast<long long>(nrows) * L.kblocks * 4;
+  hipLaunchKernelGGL(TilesToValuesKernel, dim3(GridBlocks(total, 256)), dim3(256), 0, s,
+                     tiles, L.cols, row0, nrows, L.kblocks, L.planes, signed_top ? 1 : 0, mask,
+                     out);
   return hipGetLastError();
 }
 

@@ -14,6 +14,19 @@
 // All integers little-endian; polynomials bit-packed at log-modulus bits
 // (BitPackCoefficients). The walk here only records where each record lies;
 // the pack matrix and the polynomials are unpacked by the GPU.
+//
+// SaveToFile writes the same image from a resident server: the DB words and
+// the pack entries are rebuilt from their int8-plane tiles on the GPU and
+// bit-packed by the codec's encoder in row chunks, the polynomials are
+// bit-packed from the raw copies the server keeps, and the chunks stream
+// through a pinned buffer to the file (or a caller's buffer).
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -204,10 +217,251 @@ Error LoadFromBuffer(const pirg_params* params, const uint8_t* buffer, size_t si
                          a_dev, t_dev, s_dev, out);
 }
 
+// ---------------------------------------------------------------------------
+// PirServer::SaveToFile (crypto/pir/server/pir_server.cc:433-592).
+
+// Where the image goes: a file, a caller's buffer, or nowhere (counting).
+class ImageSink {
+ public:
+  explicit ImageSink(FILE* f) : f_(f) {}
+  ImageSink(uint8_t* buf, size_t cap) : buf_(buf), cap_(cap) {}
+  bool Write(const void* p, size_t n) {
+    if (n == 0) return true;
+    if (f_ != nullptr) {
+      if (std::fwrite(p, 1, n, f_) != n) return false;
+    } else if (buf_ != nullptr) {
+      if (n > cap_ - off_) return false;
+      std::memcpy(buf_ + off_, p, n);
+    }
+    off_ += n;
+    return true;
+  }
+  template <typename T>
+  bool Val(T v) {
+    return Write(&v, sizeof(T));
+  }
+  size_t written() const { return off_; }
+
+ private:
+  FILE* f_ = nullptr;
+  uint8_t* buf_ = nullptr;
+  size_t cap_ = 0;
+  size_t off_ = 0;
+};
+
+struct ImageShape {
+  int M, K, phys_db, pk_cols, pk_phys, pk_condensed, pk_bits, logm, nsda;
+  uint64_t db_words, pk_bytes, poly_bytes;
+};
+
+// pir_server.cc:441-483: the header fields of the two matrices.
+ImageShape ShapeOf(const pirg_server* S) {
+  ImageShape sh{};
+  sh.M = S->M;
+  sh.K = S->K;
+  sh.phys_db = S->P.db_bytes == 2 ? S->K / 4 : S->K / 8;
+  sh.db_words = static_cast<uint64_t>(sh.M) * sh.phys_db;
+  sh.pk_cols = S->k * S->d;
+  // Matrix<int32_t> is condensed (two entries per word); Matrix<int16_t> and
+  // Matrix<uint64_t> are plain (lwes_to_rlwe.cc:228-234).
+  sh.pk_condensed = S->mat_src_bytes == 4 ? 1 : 0;
+  sh.pk_phys = sh.pk_condensed ? (sh.pk_cols + 1) / 2 : sh.pk_cols;
+  int log2_d = 0;
+  while ((1 << log2_d) < S->d) ++log2_d;  // absl::bit_width(d - 1) for d >= 2
+  sh.pk_bits = S->P.pack_log_digit + log2_d;
+  sh.pk_bytes = (static_cast<uint64_t>(sh.M) * sh.pk_cols * sh.pk_bits + 7) / 8;
+  sh.logm = S->log_q;  // RlweParams::LogModulus: 64 (or 32) for modulus 0
+  sh.poly_bytes = (static_cast<uint64_t>(S->d) * sh.logm + 7) / 8;
+  sh.nsda = S->t > 1 ? 2 * S->e : 0;
+  return sh;
+}
+
+uint64_t ImageBytes(const pirg_server* S) {
+  const ImageShape sh = ShapeOf(S);
+  const uint64_t lt = static_cast<uint64_t>(S->ell) * S->t;
+  const uint64_t poly_rec = 16 + sh.poly_bytes;
+  return 4 + 4 + 1 + 4 + 8 + sh.db_words * 8 +            // DB
+         4 * 5 + 1 + 8 + sh.pk_bytes +                      // pack matrix
+         4 + 4 + 4ULL * S->ell +                            // log modulus, shards, slices
+         lt * (poly_rec + 8 + static_cast<uint64_t>(S->k) * poly_rec) +
+         8 + static_cast<uint64_t>(sh.nsda) * poly_rec;     // second_dim_a
+}
+
+Error WriteFailed(const std::string& where) {
+  return Error{PIRG_INTERNAL, "Error writing to file: " + where};
+}
+
+// Rows of a tiled matrix, chunk by chunk: tiles -> values (GPU) -> bit-packed
+// at `bits` (GPU) -> pinned host -> sink. `total_bytes` of the bit stream are
+// written (a chunk of a multiple of 64 rows ends on a word boundary).
+Error WriteTiledRows(const int8_t* tiles, const LimbLayout& L, bool signed_top, u64 mask,
+                     int bits, uint64_t total_bytes, hipStream_t s, ImageSink& out,
+                     const std::string& where) {
+  const size_t row_vals = static_cast<size_t>(L.cols);
+  int rows_per = static_cast<int>(std::max<size_t>(64, (size_t(512) << 20) / (row_vals * 8)));
+  rows_per = std::max(64, rows_per / 64 * 64);
+  rows_per = std::min(rows_per, (L.rows + 63) / 64 * 64);
+  const long long max_words = StreamWords(static_cast<long long>(rows_per) * L.cols, bits);
+  DevBuf vals, words;
+  HostBuf host;
+  Error er = vals.alloc(static_cast<size_t>(rows_per) * row_vals * 8);
+  if (!er.ok()) return er;
+  if ((er = words.alloc(static_cast<size_t>(max_words) * 8)).code) return er;
+  if ((er = host.alloc(static_cast<size_t>(max_words) * 8)).code) return er;
+  uint64_t left = total_bytes;
+  for (int r0 = 0; r0 < L.rows; r0 += rows_per) {
+    const int nr = std::min(rows_per, L.rows - r0);
+    const long long len = static_cast<long long>(nr) * L.cols;
+    const long long nw = StreamWords(len, bits);
+    PIRG_HIP(LaunchTilesToValues(tiles, L, r0, nr, signed_top, mask, vals.as<u64>(), s));
+    PIRG_HIP(LaunchBitPack(vals.as<u64>(), 1, len, bits, words.as<u64>(), nw, s));
+    PIRG_HIP(hipMemcpyAsync(host.p, words.p, static_cast<size_t>(nw) * 8,
+                            hipMemcpyDeviceToHost, s));
+    PIRG_HIP(hipStreamSynchronize(s));
+    const uint64_t n = std::min<uint64_t>(left, static_cast<uint64_t>(nw) * 8);
+    if (!out.Write(host.p, n)) return WriteFailed(where);
+    left -= n;
+  }
+  if (left != 0) return Internal("image matrix stream is short");
+  return Ok();
+}
+
+Error SaveImage(pirg_server* S, ImageSink& out, const std::string& where) {
+  const ImageShape sh = ShapeOf(S);
+  CallStream cs;
+  PIRG_HIP(cs.create());
+  // 1. The combined DB matrix, condensed (pir_server.cc:441-464).
+  bool ok = out.Val<int32_t>(sh.M) && out.Val<int32_t>(sh.phys_db) && out.Val<uint8_t>(1) &&
+            out.Val<int32_t>(sh.K) && out.Val<uint64_t>(sh.db_words);
+  if (!ok) return WriteFailed(where);
+  Error er = WriteTiledRows(S->db_tiles.as<int8_t>(), S->dbL, false, ~0ULL,
+                            8 * S->P.db_bytes, sh.db_words * 8, cs.s, out, where);
+  if (!er.ok()) return er;
+  // 2. The combined pack matrix, BitPackSigned at pack_log_digit + log2(d)
+  // bits (:466-516): each entry cast to its unsigned type, then masked.
+  ok = out.Val<int32_t>(sh.M) && out.Val<int32_t>(sh.pk_cols) && out.Val<int32_t>(sh.pk_phys) &&
+       out.Val<uint8_t>(static_cast<uint8_t>(sh.pk_condensed)) &&
+       out.Val<int32_t>(sh.pk_cols) && out.Val<int32_t>(sh.pk_bits) &&
+       out.Val<uint64_t>(sh.pk_bytes);
+  if (!ok) return WriteFailed(where);
+  const u64 type_mask = S->mat_src_bytes == 2 ? 0xFFFFULL
+                        : S->mat_src_bytes == 4 ? 0xFFFFFFFFULL : ~0ULL;
+  er = WriteTiledRows(S->pk_tiles.as<int8_t>(), S->pkL, true, type_mask, sh.pk_bits,
+                      sh.pk_bytes, cs.s, out, where);
+  if (!er.ok()) return er;
+  // 3-4. The polynomials, BitPackCoefficients at log-modulus bits
+  // (:518-583): a_tilde_agg, t_vec_h, second_dim_a, one stream each.
+  const int d = S->d, k = S->k, lt = S->ell * S->t;
+  const long long pitch = StreamWords(d, sh.logm);
+  const int npolys = lt + lt * k + sh.nsda;
+  DevBuf words;
+  HostBuf host;
+  if ((er = words.alloc(static_cast<size_t>(npolys) * pitch * 8)).code) return er;
+  if ((er = host.alloc(static_cast<size_t>(npolys) * pitch * 8)).code) return er;
+  u64* w = words.as<u64>();
+  PIRG_HIP(LaunchBitPack(S->a_tilde.as<u64>(), lt, d, sh.logm, w, pitch, cs.s));
+  PIRG_HIP(LaunchBitPack(S->t_raw.as<u64>(), lt * k, d, sh.logm, w + lt * pitch, pitch, cs.s));
+  if (sh.nsda > 0) {
+    PIRG_HIP(LaunchBitPack(S->sda_raw.as<u64>(), sh.nsda, d, sh.logm,
+                           w + static_cast<long long>(lt) * (1 + k) * pitch, pitch, cs.s));
+  }
+  PIRG_HIP(hipMemcpyAsync(host.p, words.p, static_cast<size_t>(npolys) * pitch * 8,
+                          hipMemcpyDeviceToHost, cs.s));
+  PIRG_HIP(hipStreamSynchronize(cs.s));
+  const uint8_t* hb = host.as<uint8_t>();
+  auto poly = [&](int idx) {
+    return out.Val<uint64_t>(d) && out.Val<uint64_t>(sh.poly_bytes) &&
+           out.Write(hb + static_cast<size_t>(idx) * pitch * 8, sh.poly_bytes);
+  };
+  ok = out.Val<int32_t>(sh.logm) && out.Val<int32_t>(S->ell);
+  for (int s = 0; ok && s < S->ell; ++s) {
+    ok = out.Val<int32_t>(S->t);
+    for (int i = 0; ok && i < S->t; ++i) {
+      const int oi = s * S->t + i;
+      ok = poly(oi) && out.Val<uint64_t>(k);
+      for (int j = 0; ok && j < k; ++j) ok = poly(lt + oi * k + j);
+    }
+  }
+  ok = ok && out.Val<uint64_t>(sh.nsda);
+  for (int j = 0; ok && j < sh.nsda; ++j) ok = poly(lt + lt * k + j);
+  if (!ok) return WriteFailed(where);
+  return Ok();
+}
+
 }  // namespace
 }  // namespace pirg
 
 using namespace pirg;
+
+extern "C" int pirg_server_save_to_file(pirg_server* S, const char* file_path, char* err,
+                                        size_t errlen) {
+  if (S == nullptr || file_path == nullptr) {
+    return Report(Invalid("server and file_path must not be null."), err, errlen);
+  }
+  FILE* f = std::fopen(file_path, "wb");
+  if (f == nullptr) {
+    return Report(Error{PIRG_INTERNAL, std::string("Failed to open file for writing: ") +
+                                           file_path},
+                  err, errlen);
+  }
+  ImageSink out(f);
+  Error er = SaveImage(S, out, file_path);
+  const bool closed = std::fclose(f) == 0;
+  if (er.ok() && !closed) er = WriteFailed(file_path);
+  return er.ok() ? 0 : Report(er, err, errlen);
+}
+
+extern "C" int pirg_server_save_to_buffer(pirg_server* S, uint8_t* buffer, size_t capacity,
+                                          size_t* size, char* err, size_t errlen) {
+  if (S == nullptr || size == nullptr) {
+    return Report(Invalid("server and size must not be null."), err, errlen);
+  }
+  const uint64_t need = ImageBytes(S);
+  *size = static_cast<size_t>(need);
+  if (buffer == nullptr) return 0;
+  if (capacity < need) {
+    return Report(Invalid("buffer holds " + std::to_string(capacity) + " bytes; the image has " +
+                          std::to_string(need) + "."),
+                  err, errlen);
+  }
+  ImageSink out(buffer, capacity);
+  Error er = SaveImage(S, out, "buffer");
+  if (er.ok() && out.written() != need) er = Internal("image size mismatch");
+  return er.ok() ? 0 : Report(er, err, errlen);
+}
+
+extern "C" int pirg_server_load_from_file(const pirg_params* params, const char* file_path,
+                                          pirg_server** out, char* err, size_t errlen) {
+  if (params == nullptr || file_path == nullptr || out == nullptr) {
+    return Report(Invalid("params, file_path and out must not be null."), err, errlen);
+  }
+  // file_util::GetFileContents's failure (pir_server.cc:799-801); the file is
+  // mapped rather than copied into memory.
+  const Error read_failed{PIRG_INTERNAL, "Failed to read file."};
+  const int fd = ::open(file_path, O_RDONLY);
+  if (fd < 0) return Report(read_failed, err, errlen);
+  struct stat st {};
+  if (::fstat(fd, &st) != 0) {
+    ::close(fd);
+    return Report(read_failed, err, errlen);
+  }
+  const size_t size = static_cast<size_t>(st.st_size);
+  void* map = nullptr;
+  if (size > 0) {
+    map = ::mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (map == MAP_FAILED) {
+      ::close(fd);
+      return Report(read_failed, err, errlen);
+    }
+  }
+  ::close(fd);
+  std::unique_ptr<pirg_server> S;
+  Error er = LoadFromBuffer(params, static_cast<const uint8_t*>(map), size, &S);
+  if (map != nullptr) ::munmap(map, size);
+  if (!er.ok()) return Report(er, err, errlen);
+  *out = S.release();
+  return 0;
+}
 
 extern "C" int pirg_server_load_from_buffer(const pirg_params* params, const uint8_t* buffer,
                                             size_t size, pirg_server** out, char* err,

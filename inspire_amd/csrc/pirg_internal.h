@@ -114,6 +114,11 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
                              int cols, int row0, int nrows,
                              const LimbLayout& L, int8_t* tiles,
                              hipStream_t s);
+// The inverse for SaveToFile: rows [row0, row0 + nrows) (row0 + nrows <=
+// L.rows) back to row-major values, nrows x L.cols u64 (the planes' sum;
+// sign-extended when signed_top, as for pack matrices; ANDed with mask).
+hipError_t LaunchTilesToValues(const int8_t* tiles, const LimbLayout& L, int row0, int nrows,
+                              bool signed_top, u64 mask, u64* out, hipStream_t s);
 // How one limb product over nq query vectors is launched. Columns of the
 // int8 product are (query, shift s) pairs, s < ns = ceil(out_bits / 8): column
 // (q, s) collects plane p times digit s - p of query q. Queries are grouped,

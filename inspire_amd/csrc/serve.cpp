@@ -11,6 +11,12 @@
 // admission twice: the number of slots and their batch capacity are chosen at
 // start from the free HBM, and a submit is refused (RESOURCE_EXHAUSTED) while
 // `max_outstanding` requests are submitted but not yet polled.
+//
+// Lifetime: the server holds the loop through a shared_ptr, and every
+// submit / poll / stats call works on its own copy of it, taken under the
+// server's serve_mu. pirg_serve_stop detaches the loop, stops it (waking any
+// thread blocked in poll) and drops its reference; the loop is destroyed when
+// the last call still inside it returns.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -113,38 +119,62 @@ class ServeLoop {
     return Ok();
   }
 
-  // True when a result was delivered within timeout_ms.
-  bool Poll(int timeout_ms, ServeResult* res) {
+  // PIRG_OK when a result was delivered within timeout_ms, else
+  // PIRG_DEADLINE_EXCEEDED, or PIRG_FAILED_PRECONDITION once the loop has
+  // stopped with nothing left to deliver.
+  int Poll(int timeout_ms, ServeResult* res) {
     std::unique_lock<std::mutex> lock(mu_);
     const auto until = Clock::now() + std::chrono::milliseconds(std::max(timeout_ms, 0));
     while (done_.empty()) {
+      if (stopped_) return PIRG_FAILED_PRECONDITION;
       if (cv_done_.wait_until(lock, until) == std::cv_status::timeout && done_.empty()) {
-        return false;
+        return stopped_ ? PIRG_FAILED_PRECONDITION : PIRG_DEADLINE_EXCEEDED;
       }
     }
     *res = std::move(done_.front());
     done_.pop_front();
     --outstanding_;
-    return true;
+    return PIRG_OK;
   }
 
+  // Answers everything submitted, joins the worker and wakes every poller.
+  // Safe to call more than once and from several threads.
   void Stop() {
     {
       std::lock_guard<std::mutex> lock(mu_);
-      if (stopping_ && !worker_.joinable()) return;
       stopping_ = true;
     }
     cv_work_.notify_all();
-    if (worker_.joinable()) worker_.join();
+    {
+      std::lock_guard<std::mutex> join_lock(join_mu_);
+      if (worker_.joinable()) worker_.join();
+    }
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      stopped_ = true;
+    }
+    cv_done_.notify_all();
   }
 
   void Stats(pirg_serve_stats_t* st, bool reset) {
-    std::lock_guard<std::mutex> lock(mu_);
-    st->submitted = submitted_;
-    st->answered = answered_;
-    st->batches = batches_;
-    st->mean_batch = batches_ > 0 ? static_cast<double>(batched_) / batches_ : 0.0;
-    std::vector<double> l = latencies_;
+    std::vector<double> l;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      st->submitted = submitted_;
+      st->answered = answered_;
+      st->batches = batches_;
+      st->mean_batch = batches_ > 0 ? static_cast<double>(batched_) / batches_ : 0.0;
+      l = latencies_;
+      st->slots = nslots_;
+      st->max_batch = max_batch_;
+      st->elapsed_ms = MsSince(started_, Clock::now());
+      if (reset) {
+        submitted_ = answered_ = batches_ = batched_ = 0;
+        latencies_.clear();
+        lat_next_ = 0;
+        started_ = Clock::now();
+      }
+    }
     std::sort(l.begin(), l.end());
     auto pct = [&](double p) {
       if (l.empty()) return 0.0;
@@ -154,14 +184,6 @@ class ServeLoop {
     st->p50_ms = pct(0.50);
     st->p99_ms = pct(0.99);
     st->max_ms = l.empty() ? 0.0 : l.back();
-    st->slots = nslots_;
-    st->max_batch = max_batch_;
-    st->elapsed_ms = MsSince(started_, Clock::now());
-    if (reset) {
-      submitted_ = answered_ = batches_ = batched_ = 0;
-      latencies_.clear();
-      started_ = Clock::now();
-    }
   }
 
  private:
@@ -179,7 +201,13 @@ class ServeLoop {
         r.out.assign(src, src + per);
       }
       r.latency_ms = MsSince(f.reqs[i].submitted, now);
-      latencies_.push_back(r.latency_ms);
+      // The latest kLatencyWindow latencies, as a ring.
+      if (latencies_.size() < kLatencyWindow) {
+        latencies_.push_back(r.latency_ms);
+      } else {
+        latencies_[lat_next_] = r.latency_ms;
+        lat_next_ = (lat_next_ + 1) % kLatencyWindow;
+      }
       done_.push_back(std::move(r));
       ++answered_;
     }
@@ -279,11 +307,22 @@ class ServeLoop {
   std::deque<ServeResult> done_;
   long long outstanding_ = 0;
   bool stopping_ = false;
+  bool stopped_ = false;  // the worker has finished; guarded by mu_
   long long submitted_ = 0, answered_ = 0, batches_ = 0, batched_ = 0;
+  // Percentiles cover the latest kLatencyWindow answers since start or reset.
+  static constexpr size_t kLatencyWindow = size_t(1) << 20;
   std::vector<double> latencies_;
+  size_t lat_next_ = 0;
   Clock::time_point started_;
+  std::mutex join_mu_;
   std::thread worker_;
 };
+
+// The server's loop, if serving, as a reference the caller keeps for the call.
+std::shared_ptr<ServeLoop> LoopOf(pirg_server* S) {
+  std::lock_guard<std::mutex> lock(S->serve_mu);
+  return S->serve;
+}
 
 }  // namespace pirg
 
@@ -292,8 +331,13 @@ using namespace pirg;
 pirg_server::pirg_server() = default;
 
 pirg_server::~pirg_server() {
-  if (serve) serve->Stop();
-  serve.reset();
+  std::shared_ptr<ServeLoop> loop;
+  {
+    std::lock_guard<std::mutex> lock(serve_mu);
+    loop.swap(serve);
+  }
+  if (loop) loop->Stop();
+  loop.reset();
   if (main.stream != nullptr) (void)hipStreamSynchronize(main.stream);
 }
 
@@ -302,14 +346,18 @@ extern "C" {
 int pirg_serve_start(pirg_server* S, int max_batch, int slots, int max_outstanding, char* err,
                      size_t errlen) {
   if (S == nullptr) return Report(Invalid("invalid server."), err, errlen);
-  if (S->serve) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is already serving"}, err, errlen);
   if (max_batch < 1 || slots < 1 || max_outstanding < 1) {
     return Report(Invalid("max_batch, slots and max_outstanding must be positive."), err, errlen);
   }
   if (!S->deferred.ok()) return Report(S->deferred, err, errlen);
-  std::unique_ptr<ServeLoop> loop(new ServeLoop(S, max_batch, slots, max_outstanding));
+  if (LoopOf(S)) {
+    return Report(Error{PIRG_FAILED_PRECONDITION, "the server is already serving"}, err, errlen);
+  }
+  std::shared_ptr<ServeLoop> loop = std::make_shared<ServeLoop>(S, max_batch, slots,
+                                                                max_outstanding);
   Error er = loop->Start();
   if (!er.ok()) return Report(er, err, errlen);
+  std::lock_guard<std::mutex> lock(S->serve_mu);
   S->serve = std::move(loop);
   return 0;
 }
@@ -317,17 +365,23 @@ int pirg_serve_start(pirg_server* S, int max_batch, int slots, int max_outstandi
 int pirg_serve_submit(pirg_server* S, const pirg_request_view* req, uint64_t tag, char* err,
                       size_t errlen) {
   if (S == nullptr || req == nullptr) return Report(Invalid("invalid server or request."), err, errlen);
-  if (!S->serve) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is not serving"}, err, errlen);
-  Error er = S->serve->Submit(*req, tag);
+  const std::shared_ptr<ServeLoop> loop = LoopOf(S);
+  if (!loop) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is not serving"}, err, errlen);
+  Error er = loop->Submit(*req, tag);
   return er.ok() ? 0 : Report(er, err, errlen);
 }
 
 int pirg_serve_poll(pirg_server* S, int timeout_ms, uint64_t* tag, uint64_t* out, int* status,
                     double* latency_ms, char* err, size_t errlen) {
   if (S == nullptr) return Report(Invalid("invalid server."), err, errlen);
-  if (!S->serve) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is not serving"}, err, errlen);
+  const std::shared_ptr<ServeLoop> loop = LoopOf(S);
+  if (!loop) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is not serving"}, err, errlen);
   ServeResult r;
-  if (!S->serve->Poll(timeout_ms, &r)) return PIRG_DEADLINE_EXCEEDED;
+  const int rc = loop->Poll(timeout_ms, &r);
+  if (rc == PIRG_FAILED_PRECONDITION) {
+    return Report(Error{PIRG_FAILED_PRECONDITION, "the serving loop stopped"}, err, errlen);
+  }
+  if (rc != PIRG_OK) return rc;
   if (tag) *tag = r.tag;
   if (status) *status = r.status;
   if (latency_ms) *latency_ms = r.latency_ms;
@@ -341,16 +395,20 @@ int pirg_serve_poll(pirg_server* S, int timeout_ms, uint64_t* tag, uint64_t* out
 
 int pirg_serve_stats(pirg_server* S, pirg_serve_stats_t* st, int reset) {
   if (S == nullptr || st == nullptr) return PIRG_INVALID_ARGUMENT;
-  if (!S->serve) return PIRG_FAILED_PRECONDITION;
-  S->serve->Stats(st, reset != 0);
+  const std::shared_ptr<ServeLoop> loop = LoopOf(S);
+  if (!loop) return PIRG_FAILED_PRECONDITION;
+  loop->Stats(st, reset != 0);
   return 0;
 }
 
 int pirg_serve_stop(pirg_server* S) {
   if (S == nullptr) return PIRG_INVALID_ARGUMENT;
-  if (!S->serve) return 0;
-  S->serve->Stop();
-  S->serve.reset();
+  std::shared_ptr<ServeLoop> loop;
+  {
+    std::lock_guard<std::mutex> lock(S->serve_mu);
+    loop.swap(S->serve);
+  }
+  if (loop) loop->Stop();
   return 0;
 }
 

@@ -204,6 +204,7 @@ Error DeriveServer(const pirg_params& P, pirg_server* S) {
   S->qmask = P.log_q >= 64 ? ~0ULL : ((1ULL << P.log_q) - 1);
   S->width = width;
   S->cmask = width >= 64 ? ~0ULL : ((1ULL << width) - 1);
+  S->mat_src_bytes = P.mat_bytes;
   // FinalizeMatrixPack's InnerProduct (lwes_to_rlwe.cc:472-474).
   Error er = MakeRules(S->k, d, P.pack_log_digit, P.log_q, ChunkFor(S->k, d), true,
                        &S->fin_u, &S->fin_v);
@@ -375,12 +376,18 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
     PIRG_HIP(CopySync(S->a_tilde.as<u64>() + static_cast<size_t>(oi) * d, a_parts[oi],
                       static_cast<size_t>(d) * 8, hipMemcpyDefault, S->main.stream));
   }
-  DevBuf th;
+  DevBuf& th = S->t_raw;
   er = th.alloc(static_cast<size_t>(lt) * S->k * d * 8);
   if (!er.ok()) return er;
   for (int oi = 0; oi < lt; ++oi) {
     PIRG_HIP(CopySync(th.as<u64>() + static_cast<size_t>(oi) * S->k * d, th_parts[oi],
                       static_cast<size_t>(S->k) * d * 8, hipMemcpyDefault, S->main.stream));
+  }
+  if (S->t > 1) {
+    er = S->sda_raw.alloc(static_cast<size_t>(2 * S->e) * d * 8);
+    if (!er.ok()) return er;
+    PIRG_HIP(CopySync(S->sda_raw.p, second_dim_a, static_cast<size_t>(2 * S->e) * d * 8,
+                      hipMemcpyDefault, S->main.stream));
   }
 
   // Twiddles.
@@ -407,11 +414,7 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
     PIRG_HIP(LaunchSpectra(th.as<u64>(), lt * S->k, 1, d, S->fin_u, S->F,
                            S->tw_fwd.as<double2>(), S->t_spec.as<double2>(), S->main.stream));
     if (S->t > 1) {
-      DevBuf sda;
-      er = sda.alloc(static_cast<size_t>(2 * S->e) * d * 8);
-      if (!er.ok()) return er;
-      PIRG_HIP(CopySync(sda.p, second_dim_a, static_cast<size_t>(2 * S->e) * d * 8,
-                        hipMemcpyDefault, S->main.stream));
+      const DevBuf& sda = S->sda_raw;
       er = S->sda_spec.alloc(static_cast<size_t>(2 * S->e) * S->ext_v.num_chunks * S->N *
                              sizeof(double2));
       if (!er.ok()) return er;
@@ -959,6 +962,15 @@ Error CreateFromImage(const pirg_params* params, const uint64_t* db_cdata, int d
   Error er = DeriveServer(*params, S.get());
   if (!er.ok()) return er;
   ApplyPlanOverrides(S.get());
+  if (S->P.mat_bytes == 8) {
+    // BitUnpackSigned sign-extends each entry from pack.bits bits: int32
+    // holds it whenever pack.bits <= 32.
+    if (pack.bits > 32) {
+      return Invalid("image pack entries are wider than 32 bits; the GPU path holds "
+                     "uint64_t pack matrices as int32.");
+    }
+    S->P.mat_bytes = 4;
+  }
   // The image holds the combined matrices: view them as ell shards.
   const int ell = S->ell, t = S->t, d = S->d, lt = ell * t;
   std::vector<int> rows(ell, t * d), cols(ell, db_cols), olen(ell, t);
@@ -1160,7 +1172,10 @@ extern "C" {
 int pirg_server_autotune(pirg_server* S, const int* widths, int nwidths, char* report,
                          size_t report_len) {
   if (S == nullptr || widths == nullptr || nwidths < 0) return PIRG_INVALID_ARGUMENT;
-  if (S->serve) return PIRG_FAILED_PRECONDITION;
+  {
+    std::lock_guard<std::mutex> lock(S->serve_mu);
+    if (S->serve) return PIRG_FAILED_PRECONDITION;
+  }
   std::string rep;
   for (int i = 0; i < nwidths; ++i) {
     if (widths[i] < 1) return PIRG_INVALID_ARGUMENT;
@@ -1180,6 +1195,9 @@ int pirg_server_autotune(pirg_server* S, const int* widths, int nwidths, char* r
 int pirg_server_create(const pirg_params* params,
                        const pirg_preprocessed_view* data, pirg_server** out,
                        char* err, size_t errlen) {
+  if (params == nullptr || data == nullptr || out == nullptr) {
+    return Report(Invalid("params, data and out must not be null."), err, errlen);
+  }
   std::unique_ptr<pirg_server> S;
   Error er = CreateFromParts(params, data, &S);
   if (!er.ok()) return Report(er, err, errlen);
@@ -1192,6 +1210,11 @@ int pirg_server_create_flat(const pirg_params* params, const uint64_t* db_cdata,
                             const uint64_t* a_tilde, const uint64_t* t_vec_h,
                             const uint64_t* second_dim_a, pirg_server** out,
                             char* err, size_t errlen) {
+  if (params == nullptr || out == nullptr || db_cdata == nullptr || pack == nullptr ||
+      a_tilde == nullptr || t_vec_h == nullptr) {
+    return Report(Invalid("params, out, db_cdata, pack, a_tilde and t_vec_h must not be null."),
+                  err, errlen);
+  }
   pirg_server probe;
   Error er = DeriveServer(*params, &probe);
   if (!er.ok()) return Report(er, err, errlen);
@@ -1242,6 +1265,9 @@ int pirg_process_batch(pirg_server* S, int nreq, const pirg_request_view* reqs,
                        uint64_t* const* outs, int* statuses, char* err,
                        size_t errlen) {
   if (S == nullptr || nreq < 0) return Report(Invalid("invalid server or batch size."), err, errlen);
+  if (nreq > 0 && (reqs == nullptr || outs == nullptr || statuses == nullptr)) {
+    return Report(Invalid("reqs, outs and statuses must hold nreq entries."), err, errlen);
+  }
   std::vector<int> idx;
   Error first_bad;
   for (int i = 0; i < nreq; ++i) {
@@ -1270,6 +1296,9 @@ int pirg_process_batch(pirg_server* S, int nreq, const pirg_request_view* reqs,
 
 int pirg_batch_stage(pirg_server* S, int nreq, const pirg_request_view* reqs,
                      char* err, size_t errlen) {
+  if (S == nullptr || nreq <= 0 || reqs == nullptr) {
+    return Report(Invalid("invalid server, batch size or requests."), err, errlen);
+  }
   std::vector<int> idx(nreq);
   for (int i = 0; i < nreq; ++i) {
     Error er = ValidateRequest(S, reqs[i].first_dim_len, reqs[i].second_dim_polys,
@@ -1289,6 +1318,7 @@ int pirg_batch_stage(pirg_server* S, int nreq, const pirg_request_view* reqs,
 }
 
 int pirg_batch_run(pirg_server* S, int nreq, char* err, size_t errlen) {
+  if (S == nullptr) return Report(Invalid("invalid server."), err, errlen);
   if (nreq <= 0 || nreq > S->staged) {
     return Report(Invalid("batch size exceeds the staged requests."), err, errlen);
   }
@@ -1303,6 +1333,7 @@ int pirg_batch_run(pirg_server* S, int nreq, char* err, size_t errlen) {
 
 int pirg_batch_fetch(pirg_server* S, int nreq, uint64_t* const* outs, char* err,
                      size_t errlen) {
+  if (S == nullptr || outs == nullptr) return Report(Invalid("invalid server or outputs."), err, errlen);
   if (nreq <= 0 || nreq > S->staged) {
     return Report(Invalid("batch size exceeds the staged requests."), err, errlen);
   }

@@ -165,6 +165,11 @@ struct pirg_server {
   // Scan configurations chosen by pirg_server_autotune, per batch size.
   std::map<int, pirg::ScanPlan> tuned_db, tuned_pk;
   pirg::DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
+  // t_vec_h ((ell*t) x k x d) and second_dim_a (2e x d, when t > 1) as given,
+  // for SaveToFile (the kernels read their spectra).
+  pirg::DevBuf t_raw, sda_raw;
+  // The pack matrix type the caller passed (mat_bytes 8 is held as int32).
+  int mat_src_bytes = 4;
   pirg::Slot main;
   int staged = 0;
   // pirg_process_batch / pirg_batch_run replay hipGraphs instead of eager
@@ -179,7 +184,10 @@ struct pirg_server {
   double db_ms = 0, pack_ms = 0, eval_ms = 0;
   double db_scan_ms = 0, pk_scan_ms = 0;
   long long scans = 0;
-  std::unique_ptr<pirg::ServeLoop> serve;
+  // The serving loop (see serve.cpp for its lifetime rules); serve_mu guards
+  // the pointer, not the loop.
+  std::mutex serve_mu;
+  std::shared_ptr<pirg::ServeLoop> serve;
   // Defined beside ServeLoop (serve.cpp), where its type is complete.
   pirg_server();
   ~pirg_server();

@@ -97,6 +97,10 @@ int pirg_vpir_server_create(const pirg_params* params, int local_finalize,
   if (params->interp_degree != 1) {
     return Report(Invalid("vPIR parameters have interpolation degree 1."), err, errlen);
   }
+  if (params->log_degree < 1 || params->log_degree > 12) {
+    return Report(Invalid("log_degree must be in [1, 12] on the GPU path (2d-point FFT in LDS)."),
+                  err, errlen);
+  }
   const int d = 1 << params->log_degree;
   const int ell = params->entry_size_multiple;
   if (db_rows != ell * d) return Report(Invalid("DB matrix must have ell * d rows."), err, errlen);

@@ -2361,12 +2361,19 @@ size_t orc_save_server(const orc_params* P, const uint64_t* db_cdata,
   w.Val<int32_t>(pcols);
   const int32_t bits = P->pack_log_digit + BitWidth(static_cast<u64>(d - 1));
   w.Val<int32_t>(bits);
+  // BitPackSigned casts each entry to its unsigned type before masking
+  // (pir_server.cc:318-321): 32, 16 or 64 value bits.
   std::vector<u64> entries(static_cast<size_t>(M) * pcols);
+  const int value_bits = P->mat_bytes == 8 ? 64 : (i32 ? 32 : 16);
   for (size_t i = 0; i < entries.size(); ++i) {
-    entries[i] = i32 ? static_cast<u64>(static_cast<i64>(static_cast<const int32_t*>(pack)[i]))
-                     : static_cast<u64>(static_cast<i64>(static_cast<const int16_t*>(pack)[i]));
+    if (P->mat_bytes == 8) {
+      entries[i] = static_cast<const u64*>(pack)[i];
+    } else {
+      entries[i] = i32 ? static_cast<u64>(static_cast<i64>(static_cast<const int32_t*>(pack)[i]))
+                       : static_cast<u64>(static_cast<i64>(static_cast<const int16_t*>(pack)[i]));
+    }
   }
-  std::vector<uint8_t> pk = BitPack(entries.data(), entries.size(), bits, 1, i32 ? 32 : 16);
+  std::vector<uint8_t> pk = BitPack(entries.data(), entries.size(), bits, 1, value_bits);
   w.Val<uint64_t>(pk.size());
   w.Put(pk.data(), pk.size());
   // 3. preprocessed outputs.
@@ -2437,10 +2444,13 @@ int orc_load_server(const orc_params* P, const uint8_t* buf, size_t size,
     InvalidArg("pack matrix bytes are short.");
   }
   std::vector<u64> ent(nent);
-  BitUnpack(buf + r.off, nent, bits, 1, P->mat_bytes == 4 ? 32 : 16, ent.data());
+  BitUnpack(buf + r.off, nent, bits, 1,
+            P->mat_bytes == 8 ? 64 : (P->mat_bytes == 4 ? 32 : 16), ent.data());
   r.off += plen;
   for (size_t i = 0; i < nent; ++i) {
-    if (P->mat_bytes == 4) {
+    if (P->mat_bytes == 8) {
+      static_cast<u64*>(pack)[i] = ent[i];
+    } else if (P->mat_bytes == 4) {
       static_cast<int32_t*>(pack)[i] = static_cast<int32_t>(ent[i]);
     } else {
       static_cast<int16_t*>(pack)[i] = static_cast<int16_t>(ent[i]);

@@ -306,3 +306,42 @@ def test_uint64_pack_matrix_outside_int32_is_rejected_before_the_device():
     assert ei.value.code == ia.INVALID_ARGUMENT
     assert ei.value.message.startswith(f"pack matrix entry {7 * s.k * s.d + 9} of block 0 lies "
                                        "outside [-2^31, 2^31) mod q")
+
+
+def test_entry_points_reject_null_and_negative_arguments():
+    # Checked before any allocation or device call, so no GPU is needed.
+    lib = ia.lib()
+    err = ctypes.create_string_buffer(256)
+    n = ctypes.c_size_t(256)
+    out = ctypes.c_void_p()
+    assert lib.pirg_server_create(None, None, ctypes.byref(out), err, n) == ia.INVALID_ARGUMENT
+    assert lib.pirg_server_create_flat(None, None, 0, None, None, None, None,
+                                       ctypes.byref(out), err, n) == ia.INVALID_ARGUMENT
+    assert lib.pirg_process_batch(None, 1, None, None, None, err, n) == ia.INVALID_ARGUMENT
+    assert lib.pirg_batch_stage(None, -1, None, err, n) == ia.INVALID_ARGUMENT
+    assert lib.pirg_batch_run(None, 1, err, n) == ia.INVALID_ARGUMENT
+    assert lib.pirg_batch_fetch(None, 1, None, err, n) == ia.INVALID_ARGUMENT
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62Interp1"])
+    gp = ia.PirParams(**{**{f: getattr(to_gpu_params(P), f)
+                            for f in ia.PirParams.__dataclass_fields__}, "log_degree": 40})
+    cp = gp.c()
+    rc = lib.pirg_vpir_server_create(ctypes.byref(cp), 0, None, 0, 0, None, 0, None, None,
+                                     ctypes.byref(out), err, n)
+    assert rc == ia.INVALID_ARGUMENT
+    assert b"log_degree" in err.value
+
+
+def test_load_from_file_and_save_errors(tmp_path):
+    # LoadFromFile's read failure (pir_server.cc:799-801) and SaveToFile's
+    # open failure (:437-440), before any device work.
+    P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62"])
+    with pytest.raises(ia.PirError) as ei:
+        ia.PirServer.load_from_file(to_gpu_params(P), str(tmp_path / "missing.bin"))
+    assert ei.value.code == ia.INTERNAL
+    assert ei.value.message == "Failed to read file."
+    lib = ia.lib()
+    err = ctypes.create_string_buffer(256)
+    assert lib.pirg_server_save_to_file(None, b"/tmp/x", err, 256) == ia.INVALID_ARGUMENT
+    size = ctypes.c_size_t()
+    assert lib.pirg_server_save_to_buffer(None, None, 0, ctypes.byref(size), err,
+                                          256) == ia.INVALID_ARGUMENT

@@ -6,6 +6,7 @@ must equal, bit for bit, the same request answered alone, and the oracle's
 ProcessResponse on the same inputs.
 """
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -232,3 +233,45 @@ def test_library_calls_while_serving(material, requests):
     for t in (0, tag - 1):
         want, _, _ = O.process_response(P, material, requests[t % len(requests)])
         np.testing.assert_array_equal(served[t], want)
+
+
+def test_stop_wakes_a_blocked_poller(material, requests):
+    # pirg_serve_stop while another thread waits in pirg_serve_poll: the
+    # poller returns at once (FAILED_PRECONDITION, nothing left to deliver)
+    # instead of using a freed loop; a result still undelivered at the stop
+    # is handed to the poller first.
+    with server_from_material(P, material) as srv:
+        srv.serve_start(max_batch=4, slots=1, max_outstanding=8)
+        srv.submit(gpu_request(requests[0]), tag=7)
+        first = srv.poll(30000)
+        assert first is not None and first[0] == 7
+        srv.submit(gpu_request(requests[1]), tag=8)
+        got = {}
+
+        def poller():
+            t0 = time.perf_counter()
+            outcomes = []
+            for _ in range(2):
+                try:
+                    outcomes.append(srv.poll(30000))
+                except ia.PirError as e:
+                    outcomes.append(e)
+            got["outcomes"] = outcomes
+            got["s"] = time.perf_counter() - t0
+
+        th = threading.Thread(target=poller)
+        th.start()
+        time.sleep(0.5)
+        srv.serve_stop()
+        th.join(20)
+        assert not th.is_alive()
+        assert got["s"] < 10
+        r, e = got["outcomes"]
+        assert r is not None and r[0] == 8 and r[2] == 0
+        assert isinstance(e, ia.PirError) and e.code == ia.FAILED_PRECONDITION
+        with pytest.raises(ia.PirError) as ei:
+            srv.submit(gpu_request(requests[2]), tag=9)
+        assert ei.value.code == ia.FAILED_PRECONDITION
+        srv.serve_stop()  # stopping twice is harmless
+    want, _, _ = O.process_response(P, material, requests[1])
+    np.testing.assert_array_equal(r[1].ciphertexts, want)
