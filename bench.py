@@ -44,6 +44,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 import inspire_amd as ia  # noqa: E402
+from inspire_amd import probes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 INT8_PEAK_TOPS = 5000.0        # MI355X dense INT8 MFMA
@@ -149,6 +150,136 @@ def cpu_baseline(P, mat, req, max_queries=3, budget_s=30.0, threads=1):
             break
     O.set_matrix_threads(1)
     return len(times) / sum(times), len(times), out, stages
+
+
+def run_reference_keys(srv: ia.PirServer, reqs, iters: int = 20):
+    """pir_metrics_tool's server keys (crypto/pir/benchmarking/
+    pir_metrics_tool.cc:243-319, :435-447), per request, for the
+    single-request path through pirg_process (PirServer::ProcessResponse's
+    drop-in, graph replay by default): wall time per call including the
+    request's host-to-device and the response's device-to-host copies, and
+    the GetDbMultiplyTime / GetPackingTime / GetPolyEvalTime split."""
+    srv.use_graphs(True)
+    for i in range(3):
+        srv.process_response(reqs[i % len(reqs)])
+    a0 = srv.stage_times_ms()
+    lat = []
+    for i in range(iters):
+        s0 = time.perf_counter()
+        srv.process_response(reqs[i % len(reqs)])
+        lat.append((time.perf_counter() - s0) * 1e3)
+    a1 = srv.stage_times_ms()
+    return {"iterations": iters,
+            "server_proc_resp_time_ms": float(np.mean(lat)),
+            "server_proc_resp_db_mult_time_ms": (a1[0] - a0[0]) / iters,
+            "server_proc_resp_packing_time_ms": (a1[1] - a0[1]) / iters,
+            "server_proc_resp_poly_eval_time_ms": (a1[2] - a0[2]) / iters,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "qps": 1e3 / float(np.mean(lat))}
+
+
+def run_e2e(srv: ia.PirServer, reqs, batches: int, submitters: int = 4, chunk: int = 32,
+            slots: int = 2):
+    """Answers with the requests in host memory: every request goes in through
+    pirg_serve_submit_batch (copied from the caller's arrays, `chunk` per
+    call, from `submitters` threads) and every response comes back through
+    pirg_serve_poll_batch. Two serving slots of len(reqs) requests each, so
+    one batch's host staging and transfers overlap the other slot's graph
+    replay."""
+    B = len(reqs)
+    n = B * batches
+    srv.serve_start(max_batch=B, slots=slots, max_outstanding=n + 2 * B)
+    try:
+        for i in range(2 * B):                 # capture both slots' graphs
+            srv.submit(reqs[i % B], tag=i)
+        for _ in range(2 * B):
+            assert srv.poll(60000) is not None
+        srv.serve_stats(reset=True)
+        done = []
+
+        def poller():
+            while len(done) < n:
+                try:
+                    r = srv.poll_batch(B, 30000)
+                except ia.PirError:
+                    break
+                if r is None:
+                    break
+                done.extend(r[2])
+
+        def submitter(j):
+            for c0 in range(j * chunk, n, submitters * chunk):
+                ids = range(c0, min(n, c0 + chunk))
+                st = srv.submit_batch([reqs[i % B] for i in ids], list(ids))
+                assert not any(st)
+
+        th = threading.Thread(target=poller)
+        subs = [threading.Thread(target=submitter, args=(j,)) for j in range(submitters)]
+        t0 = time.perf_counter()
+        th.start()
+        try:
+            for s in subs:
+                s.start()
+            for s in subs:
+                s.join()
+            submit_s = time.perf_counter() - t0
+        except BaseException:
+            srv.serve_stop()
+            raise
+        finally:
+            th.join()
+        wall = time.perf_counter() - t0
+        st = srv.serve_stats()
+    finally:
+        srv.serve_stop()
+    return {"requests": n, "batch": B, "slots": st["slots"], "answered": len(done),
+            "failed": sum(1 for x in done if x != 0), "e2e_qps": len(done) / wall,
+            "submit_s": submit_s, "wall_s": wall,
+            "mean_batch": st["mean_batch"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+            "bytes_in_per_request": int(reqs[0].first_dimension_query.nbytes +
+                                        reqs[0].second_dimension_query.nbytes +
+                                        reqs[0].packing_key.nbytes)}
+
+
+def run_small_config(name: str, n: int, log_d: int, t: int, B: int, steps: int, warmup: int,
+                     cpu: bool):
+    """One of the smaller BASELINE workloads on its own material: the plain
+    single request (B = 1) or a B-request batch as one graph launch, requests
+    staged in HBM, with the scan-kernel rooflines; the first answer checked
+    against the oracle and the batch's answers against single answers."""
+    P = ia.metrics_tool_params(n=n, log_d=log_d, t=t)
+    d = P.degree
+    r = n // t
+    K = (r + 31) // 32 * 32
+    M = t * d
+    mat = synthetic_material(P, seed=100 + log_d + t)
+    reqs = synthetic_requests(P, max(B, 2), seed=200 + t)
+    out = {"workload": f"{name}: n={n} d={d} t={t} ell=1, batch {B}"}
+    with server_from(P, mat) as srv:
+        srv.autotune(sorted({1, B}))
+        single = time_width(srv, reqs, 1, max(steps, 20), warmup)
+        res = single if B == 1 else time_width(srv, reqs, B, steps, warmup)
+        keys = run_reference_keys(srv, reqs, iters=20)
+        first = srv.process_response(reqs[0]).ciphertexts
+    db_bytes, pk_bytes = M * K * 2, M * 2 * d * 4
+    out.update({"qps": res["qps"], "ms_per_batch": res["ms_per_batch"],
+                "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
+                "single_qps": single["qps"], "batched_over_single": res["qps"] / single["qps"],
+                "matches_single": res["matches_single"], "reference_keys": keys,
+                "db_scan_ms": res["db_scan_ms"], "pack_scan_ms": res["pack_scan_ms"]})
+    if B == 1:
+        out["db_scan_hbm_frac"] = db_bytes / (res["db_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        out["pack_scan_hbm_frac"] = pk_bytes / (res["pack_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    else:
+        tops = 2.0 * M * K * B * 13 / (res["db_scan_ms"] * 1e-3) / 1e12
+        out["db_scan_mfma_frac"] = tops / INT8_PEAK_TOPS
+        out["db_scan_hbm_frac"] = db_bytes / (res["db_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    if cpu:
+        qps, nq, cpu_out, _ = cpu_baseline(P, mat, reqs[0], max_queries=5, budget_s=10.0)
+        out["cpu_baseline"] = {"value": qps, "unit": "queries/s", "cores": 1, "kind": "port",
+                               "sample": f"{nq} request(s) through the oracle ProcessResponse"}
+        out["bit_identical"] = bool(np.array_equal(cpu_out, first))
+    return out
 
 
 def run_serving(rates, seconds: float, seed: int = 5):
@@ -582,9 +713,15 @@ def main():
     ap.add_argument("--only-preprocess", action="store_true",
                     help="run only the preprocessing section (for rocprofv3)")
     ap.add_argument("--cpu-queries", type=int, default=3)
-    ap.add_argument("--serve-rates", default="100,500,1000,2000,5000")
+    ap.add_argument("--serve-rates", default="100,500,1000,2000,5000,10000,20000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--pack-jobs", type=int, default=512)
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the n=4096 plain and n=65536 64-request workloads")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the host-resident end-to-end serving measurement")
+    ap.add_argument("--e2e-batches", type=int, default=8)
+    ap.add_argument("--e2e-slots", type=int, default=2)
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: two eager answers each at widths 1, 64 and "
                          "256, a copy and a read calibration kernel")
@@ -622,8 +759,8 @@ def main():
                         "ms_per_batch": res["ms_per_batch"],
                         "matches_single": res["matches_single"]}
         run_codec(steps=2, cpu=False)
-        copy_ms = ia.copy_probe(2 << 30, repeats=3)
-        read_ms = ia.read_probe(4 << 30, repeats=3)
+        copy_ms = probes.copy_probe(2 << 30, repeats=3)
+        read_ms = probes.read_probe(4 << 30, repeats=3)
         print(json.dumps({"profile": True, "copy_bytes": 2 << 30, "copy_ms": copy_ms,
                           "read_bytes": 4 << 30, "read_ms": read_ms, "widths": scans}))
         srv.close()
@@ -641,8 +778,8 @@ def main():
     single = per_width[1] if 1 in per_width else time_width(srv, reqs, 1, 5, args.warmup)
     best_B = max((b for b in per_width if b > 1), key=lambda b: per_width[b]["qps"])
     best = per_width[best_B]
-    read_ms = ia.read_probe(4 << 30, repeats=4)
-    mfma_ms, mfma_tops = ia.mfma_probe(2048, 4096)
+    read_ms = probes.read_probe(4 << 30, repeats=4)
+    mfma_ms, mfma_tops = probes.mfma_probe(2048, 4096)
 
     db_bytes = M * K * 2
     pairs = 13  # (plane, digit) pairs with 8 (p + c) < 52
@@ -689,6 +826,19 @@ def main():
                                   f"run's stage ms: db {pst[0]:.0f}, pack {pst[1]:.0f}, "
                                   f"eval {pst[2]:.0f})",
                         "bit_identical": bool(np.array_equal(pout, gpu_out))}
+    ref_keys = run_reference_keys(srv, reqs, iters=20)
+    log(f"pirg_process (host in/out): {ref_keys['server_proc_resp_time_ms']:.3f} ms "
+        f"(db {ref_keys['server_proc_resp_db_mult_time_ms']:.3f}, "
+        f"packing {ref_keys['server_proc_resp_packing_time_ms']:.3f}, "
+        f"poly eval {ref_keys['server_proc_resp_poly_eval_time_ms']:.3f})")
+    e2e = None
+    if not args.no_e2e:
+        host_reqs = synthetic_requests(P, best_B, seed=11)
+        e2e = run_e2e(srv, host_reqs, args.e2e_batches, slots=args.e2e_slots)
+        e2e["over_value"] = e2e["e2e_qps"] / best["qps"]
+        log(f"end to end (host-resident requests): {e2e['e2e_qps']:.0f} q/s = "
+            f"{e2e['over_value']:.2f} x value, mean batch {e2e['mean_batch']:.1f}")
+        del host_reqs
     want0 = srv.process_response(reqs[0]).ciphertexts
     srv.close()
     load = None
@@ -700,6 +850,18 @@ def main():
     codec = run_codec(cpu=not args.no_cpu)
     log(f"codec: encode {codec['encode_gbs']:.0f} GB/s, decode {codec['decode_gbs']:.0f} GB/s, "
         f"exact {codec['round_trip_exact']}")
+
+    small = None
+    if not args.no_small:
+        small = {
+            "plain_n4096": run_small_config("plain PIR, one query", 4096, 11, 2, 1, args.steps,
+                                            args.warmup, cpu=not args.no_cpu),
+            "batched64_n65536": run_small_config("batched PIR, 64 queries", 65536, 11, 4, 64,
+                                                 args.steps, args.warmup, cpu=not args.no_cpu),
+        }
+        for k, v in small.items():
+            log(f"{k}: {v['qps']:.0f} q/s, p50 {v['p50_ms']:.3f} ms, p99 {v['p99_ms']:.3f} ms, "
+                f"x{v['batched_over_single']:.1f} over single")
 
     vpir = None
     if not args.no_vpir:
@@ -744,6 +906,9 @@ def main():
         "p50_ms_batch": best["p50_ms"],
         "p99_ms_batch": best["p99_ms"],
         "batches_match_single": all(w["matches_single"] for w in widths_out),
+        "e2e": e2e,
+        "reference_keys_single": ref_keys,
+        "small_configs": small,
         "roofline": roofline,
         "roofline_single_scan": roofline_single,
         "mfma_probe_tops": mfma_tops,

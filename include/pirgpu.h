@@ -167,10 +167,11 @@ int pirg_batch_fetch(pirg_server* server, int nreq, uint64_t* const* outs,
 int pirg_server_autotune(pirg_server* server, const int* widths, int nwidths,
                          char* report, size_t report_len);
 
-/* Makes pirg_process_batch / pirg_batch_run replay one hipGraph per batch
- * size (captured on first use: the whole answer pipeline as one graph launch)
- * instead of launching the kernels one by one. Off by default; graph replays
- * do not add to the stage times below. No reference counterpart. */
+/* Whether pirg_process / pirg_process_batch / pirg_batch_run replay one
+ * hipGraph per batch size (captured on first use: the whole answer pipeline
+ * as one graph launch) or launch the kernels one by one. On by default; both
+ * add to the stage times below (graphs carry event-record nodes at the stage
+ * boundaries). No reference counterpart. */
 int pirg_server_use_graphs(pirg_server* server, int enable);
 
 /* Graphs captured from now on bracket the DB and pack scans with event-record
@@ -202,6 +203,22 @@ int pirg_serve_submit(pirg_server* server, const pirg_request_view* req,
 int pirg_serve_poll(pirg_server* server, int timeout_ms, uint64_t* tag,
                     uint64_t* out, int* status, double* latency_ms, char* err,
                     size_t errlen);
+/* n submits in one call (tags[i] for reqs[i]): statuses[i] receives each
+ * request's validation status (0 = queued; err holds the first failure's
+ * message). The valid requests are copied (by several threads for large
+ * calls) and queued together; when they do not all fit under
+ * max_outstanding, none is queued and PIRG_RESOURCE_EXHAUSTED is returned.
+ * Thread-safe. */
+int pirg_serve_submit_batch(pirg_server* server, int n, const pirg_request_view* reqs,
+                            const uint64_t* tags, int* statuses, char* err, size_t errlen);
+/* Up to max finished requests in one call: waits up to timeout_ms for the
+ * first, then takes whatever else is finished. *count receives how many;
+ * for each i < *count, tags[i], statuses[i], latency_ms[i] (may be NULL) and,
+ * when statuses[i] == 0, outs[i] (ell x 2 x d) are filled. Return codes as
+ * pirg_serve_poll. Thread-safe. */
+int pirg_serve_poll_batch(pirg_server* server, int max, int timeout_ms, uint64_t* tags,
+                          uint64_t* const* outs, int* statuses, double* latency_ms, int* count,
+                          char* err, size_t errlen);
 typedef struct {
   long long submitted, answered, batches;
   double mean_batch;            /* requests per launched batch */
@@ -300,23 +317,6 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
 int pirg_inner_product_fft_noalloc(const uint64_t* u, const uint64_t* v, int n, int d,
                                    int u_bits, int v_bits, int chunk_bits, int u_is_signed,
                                    uint64_t* out, void* stream, char* err, size_t errlen);
-
-/* Streams `bytes` through a plain device-to-device copy kernel `repeats`
- * times and reports the average kernel time. No reference counterpart: the
- * known-size kernel that calibrates HBM byte counters and bandwidth. */
-int pirg_copy_probe(size_t bytes, int repeats, double* ms);
-
-/* Sweeps a plain read-only kernel (8 loads of 16 bytes in flight per thread)
- * `repeats` times over `bytes` within one launch and reports the time per
- * sweep: the achievable read bandwidth the scans are compared with (HBM for
- * large buffers, the caches for small ones). No reference counterpart. */
-int pirg_read_probe(size_t bytes, int repeats, double* ms);
-
-/* Runs blocks x 256 threads of register-only v_mfma_i32_16x16x64_i8 (16
- * independent accumulators per wave, `iters` rounds) and reports the kernel
- * time and the int8 rate in TOP/s: the achievable MFMA rate the batched scan
- * is compared with. No reference counterpart. */
-int pirg_mfma_probe(int blocks, int iters, double* ms, double* tops);
 
 /* The twiddle table the FFT kernels use for an n-point transform, as the
  * interleaved (re, im) pairs of FftPlan's twiddles_forward_ /

@@ -534,6 +534,40 @@ class PirServer:
                                      ctypes.c_size_t(len(err)))
         _check(rc, err)
 
+    def submit_batch(self, requests: Sequence[PirRequest], tags: Sequence[int]) -> List[int]:
+        """Queues many requests in one call (pirg_serve_submit_batch); returns
+        each one's validation status (0 = queued)."""
+        n = len(requests)
+        ras = [_RequestArrays(r, self.params.degree) for r in requests]
+        views = (_CRequest * max(1, n))(*[ra.view() for ra in ras])
+        tg = (ctypes.c_uint64 * max(1, n))(*tags)
+        st = (ctypes.c_int * max(1, n))()
+        err = _err()
+        rc = lib().pirg_serve_submit_batch(self._h, n, views, tg, st, err,
+                                           ctypes.c_size_t(len(err)))
+        _check(rc, err)
+        return list(st)[:n]
+
+    def poll_batch(self, max_results: int, timeout_ms: int = 1000):
+        """Up to max_results finished requests as (tags, responses array
+        (count, ell, 2, d), statuses, latencies); None when none finished in
+        time."""
+        outs = np.zeros((max_results,) + self._out_shape, dtype=np.uint64)
+        ptrs = (_u64p * max_results)(*[outs[i].ctypes.data_as(_u64p)
+                                        for i in range(max_results)])
+        tags = (ctypes.c_uint64 * max_results)()
+        st = (ctypes.c_int * max_results)()
+        lat = (ctypes.c_double * max_results)()
+        count = ctypes.c_int()
+        err = _err()
+        rc = lib().pirg_serve_poll_batch(self._h, max_results, int(timeout_ms), tags, ptrs, st, lat,
+                                         ctypes.byref(count), err, ctypes.c_size_t(len(err)))
+        if rc == DEADLINE_EXCEEDED:
+            return None
+        _check(rc, err)
+        n = count.value
+        return list(tags)[:n], outs[:n], list(st)[:n], list(lat)[:n]
+
     def poll(self, timeout_ms: int = 1000):
         """Next finished request as (tag, response or None, status, latency_ms,
         message), or None when nothing finished within timeout_ms."""
@@ -962,35 +996,6 @@ def preprocess(params: PirParams, db_raw, first_dim_a, pack_a) -> dict:
     return dict(db=db, db_cols=K, pack=pack, a_tilde=a_tilde, t_vec_h=t_vec_h,
                 stage_ms=dict(zip(("interpolate", "hints", "preprocess_pack", "total"),
                                   list(ms))))
-
-
-def copy_probe(nbytes: int, repeats: int = 10) -> float:
-    """Average ms of a plain nbytes device copy kernel (counter calibration)."""
-    ms = ctypes.c_double()
-    rc = lib().pirg_copy_probe(ctypes.c_size_t(nbytes), repeats, ctypes.byref(ms))
-    if rc != 0:
-        raise PirError(rc, "copy probe failed")
-    return ms.value
-
-
-def read_probe(nbytes: int, repeats: int = 10) -> float:
-    """Average ms of a plain nbytes streaming-read kernel (achievable read
-    bandwidth)."""
-    ms = ctypes.c_double()
-    rc = lib().pirg_read_probe(ctypes.c_size_t(nbytes), repeats, ctypes.byref(ms))
-    if rc != 0:
-        raise PirError(rc, "read probe failed")
-    return ms.value
-
-
-def mfma_probe(blocks: int = 1024, iters: int = 4096):
-    """(ms, TOP/s) of a register-only int8 MFMA stream (achievable MFMA rate)."""
-    ms = ctypes.c_double()
-    tops = ctypes.c_double()
-    rc = lib().pirg_mfma_probe(blocks, iters, ctypes.byref(ms), ctypes.byref(tops))
-    if rc != 0:
-        raise PirError(rc, "mfma probe failed")
-    return ms.value, tops.value
 
 
 def version() -> str:

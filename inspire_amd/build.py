@@ -1,4 +1,6 @@
-"""Builds libpirgpu.so (HIP kernels + the C ABI of include/pirgpu.h) in-tree.
+"""Builds libpirgpu.so (HIP kernels + the C ABI of include/pirgpu.h) and
+libpirgpu_probes.so (the calibration kernels of include/pirgpu_probes.h,
+used by measurements only) in-tree.
 
     python -m inspire_amd.build        # or inspire_amd.build.build()
 
@@ -19,6 +21,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libpirgpu.so")
+PROBES_LIB_PATH = os.path.join(HERE, "libpirgpu_probes.so")
 
 SOURCES = ["limb_kernels.hip", "fft_kernels.hip", "codec.hip", "server.cpp", "serve.cpp",
            "pack.cpp", "loader.cpp", "proto_loader.cpp", "vpir.cpp", "preprocess_kernels.hip",
@@ -88,6 +91,15 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd))
     _run(cmd)
     os.replace(tmp, LIB_PATH)
+    # The calibration probes: their own library, so the answer path's ABI
+    # holds only the reference's interface and its batching / serving forms.
+    tmp = PROBES_LIB_PATH + ".tmp"
+    cmd = [cc, *COMMON_FLAGS, "-x", "hip", "-shared", os.path.join(CSRC, "probes.hip"),
+           "-o", tmp]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, PROBES_LIB_PATH)
     return LIB_PATH
 
 

@@ -1063,85 +1063,6 @@ hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan
 
 }  // namespace
 
-// Plain streaming copy (one 16-byte load and store per element); the
-// reference kernel for calibrating HBM byte counters.
-__global__ void CopyProbeKernel(const int4* __restrict__ src, int4* __restrict__ dst,
-                                long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    dst[i] = src[i];
-  }
-}
-
-// Plain streaming read: `passes` sweeps over the buffer, every 16-byte
-// element loaded once per sweep (8 non-temporal loads in flight per thread)
-// and folded by XOR; one word per thread is written. The reference kernel for
-// the achievable read bandwidth (from HBM for large buffers, from the caches
-// for small ones).
-__global__ __launch_bounds__(256) void ReadProbeKernel(const v4i* __restrict__ src,
-                                                       long long n, int passes,
-                                                       int* __restrict__ out) {
-  const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  int acc = 0;
-  for (int pass = 0; pass < passes; ++pass) {
-    long long i = tid;
-    for (; i + 7 * stride < n; i += 8 * stride) {
-      v4i v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
-    }
-    for (; i < n; i += stride) {
-      const v4i v = __builtin_nontemporal_load(src + i);
-      acc ^= v.x ^ v.y ^ v.z ^ v.w;
-    }
-  }
-  out[tid] = acc;
-}
-
-// Register-only v_mfma_i32_16x16x64_i8 stream: 16 independent accumulators
-// per wave, `iters` rounds. The reference kernel for the achievable int8 MFMA
-// rate (one output word per thread keeps the work observable).
-__global__ __launch_bounds__(256) void MfmaProbeKernel(int iters, int* __restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  v4i acc[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) acc[j] = v4i{0, 0, 0, 0};
-  const v4i a = v4i{lane, 1, 2, 3};
-  const v4i b = v4i{3, lane, 1, 2};
-  for (int it = 0; it < iters; ++it) {
-#pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc[j], 0, 0, 0);
-  }
-  int s = 0;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
-  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
-}
-
-hipError_t LaunchMfmaProbe(int blocks, int iters, int* out, hipStream_t s) {
-  hipLaunchKernelGGL(MfmaProbeKernel, dim3(blocks), dim3(256), 0, s, iters, out);
-  return hipGetLastError();
-}
-
-int ReadProbeOutWords() { return 2048 * 256; }
-
-hipError_t LaunchReadProbe(const void* src, size_t bytes, int passes, int* out,
-                          hipStream_t s) {
-  const long long n = static_cast<long long>(bytes / 16);
-  hipLaunchKernelGGL(ReadProbeKernel, dim3(2048), dim3(256), 0, s,
-                     static_cast<const v4i*>(src), n, passes, out);
-  return hipGetLastError();
-}
-
-hipError_t LaunchCopyProbe(const void* src, void* dst, size_t bytes, hipStream_t s) {
-  const long long n = static_cast<long long>(bytes / 16);
-  hipLaunchKernelGGL(CopyProbeKernel, dim3(4096), dim3(256), 0, s,
-                     static_cast<const int4*>(src), static_cast<int4*>(dst), n);
-  return hipGetLastError();
-}
 
 hipError_t LaunchDbToTiles(const u64* cdata, int db_bytes, int rows, int cols,
                            int row0, int nrows, const LimbLayout& L,

@@ -175,13 +175,6 @@ hipError_t LaunchVectorSums(const u64* vec, int nq, int stride, int len, u64 c_o
 // combined over (plane, digit) pairs with shift < 8 * ns.
 hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                       const int8_t* btiles, u64* partial, hipStream_t s);
-hipError_t LaunchCopyProbe(const void* src, void* dst, size_t bytes, hipStream_t s);
-// `passes` streaming reads of `bytes`; writes ReadProbeOutWords() ints to out.
-int ReadProbeOutWords();
-hipError_t LaunchReadProbe(const void* src, size_t bytes, int passes, int* out,
-                          hipStream_t s);
-// blocks x 256 threads of register-only int8 MFMAs; out holds blocks*256 ints.
-hipError_t LaunchMfmaProbe(int blocks, int iters, int* out, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // FFT-side launchers (fft_kernels.hip). N = 2d, twiddles = N/2 complex.

@@ -38,10 +38,14 @@ double MsSince(Clock::time_point t0, Clock::time_point t1) {
 
 }  // namespace
 
+// A queued request, as its record (RequestWords words: first dimension,
+// second dimension, packing key): in the loop's pinned record pool (rec >= 0)
+// or, when the pool is exhausted, in a heap buffer.
 struct ServeRequest {
   uint64_t tag = 0;
   Clock::time_point submitted;
-  std::vector<u64> fd, sd, pk;
+  int rec = -1;
+  std::vector<u64> data;
 };
 
 struct ServeResult {
@@ -55,7 +59,30 @@ struct ServeResult {
 struct InFlight {
   Slot* slot = nullptr;
   std::vector<ServeRequest> reqs;
+  std::vector<int> recs;  // pool records to free when the batch retires
 };
+
+// Runs fn(lo, hi) over [0, n) on up to max_threads threads, one per 8 MB
+// of the `bytes` the ranges copy (one thread's memcpy moves about 10 GB/s,
+// slower than the link and the GPU work a large batch feeds).
+template <typename Fn>
+void CopyInParallel(size_t n, size_t bytes, size_t max_threads, Fn fn) {
+  const size_t threads = std::min<size_t>(
+      {std::max<size_t>(1, std::thread::hardware_concurrency()), max_threads, n,
+       bytes / (size_t(8) << 20) + 1});
+  if (threads <= 1) {
+    fn(size_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const size_t per = (n + threads - 1) / threads;
+  for (size_t t = 1; t < threads; ++t) {
+    const size_t lo = std::min(n, t * per), hi = std::min(n, lo + per);
+    if (lo < hi) pool.emplace_back(fn, lo, hi);
+  }
+  fn(size_t(0), std::min(n, per));
+  for (auto& th : pool) th.join();
+}
 
 class ServeLoop {
  public:
@@ -81,6 +108,21 @@ class ServeLoop {
     if (static_cast<double>(WorkspaceBytes(S_, max_batch_)) * nslots_ > budget) {
       return Error{PIRG_RESOURCE_EXHAUSTED, "not enough free device memory for a serving slot"};
     }
+    // The pinned record pool: submits write requests here and batches copy
+    // them to HBM from here, so a request crosses host memory once. A record
+    // for every request that may be outstanding, within 4 GiB of pinned
+    // memory (at least the batches in flight plus one being gathered);
+    // requests beyond it wait in heap records.
+    rw_ = RequestWords(S_);
+    const size_t cap_records = std::max<size_t>(
+        static_cast<size_t>(max_batch_) * (nslots_ + 1), (size_t(4) << 30) / (rw_ * 8));
+    const size_t nrec = std::min(static_cast<size_t>(max_outstanding_), cap_records);
+    {
+      Error pe = records_.alloc(nrec * rw_ * 8);
+      if (!pe.ok()) return pe;
+      free_recs_.resize(nrec);
+      for (size_t i = 0; i < nrec; ++i) free_recs_[i] = static_cast<int>(nrec - 1 - i);
+    }
     for (int i = 0; i < nslots_; ++i) {
       slots_.emplace_back(new Slot());
       Error er = slots_.back()->Init();
@@ -95,25 +137,71 @@ class ServeLoop {
   }
 
   Error Submit(const pirg_request_view& r, uint64_t tag) {
-    Error er = ValidateRequest(S_, r.first_dim_len, r.second_dim_polys, r.packing_key_polys);
+    int status = 0;
+    Error er = SubmitMany(1, &r, &tag, &status);
     if (!er.ok()) return er;
-    ServeRequest q;
-    q.tag = tag;
-    q.fd.assign(r.first_dim, r.first_dim + r.first_dim_len);
-    if (S_->t > 1) q.sd.assign(r.second_dim, r.second_dim + r.second_dim_polys * S_->d);
-    q.pk.assign(r.packing_key, r.packing_key + r.packing_key_polys * S_->d);
+    if (status != 0) {
+      return ValidateRequest(S_, r.first_dim_len, r.second_dim_polys, r.packing_key_polys);
+    }
+    return Ok();
+  }
+
+  // Validates every request (statuses[i], ProcessResponse's checks), copies
+  // the valid ones (several threads for large calls) and queues them under
+  // one lock. Queues none when they do not all fit under max_outstanding.
+  Error SubmitMany(int n, const pirg_request_view* reqs, const uint64_t* tags, int* statuses) {
+    std::vector<int> ok;
+    ok.reserve(n);
+    for (int i = 0; i < n; ++i) {
+      const pirg_request_view& r = reqs[i];
+      statuses[i] =
+          ValidateRequest(S_, r.first_dim_len, r.second_dim_polys, r.packing_key_polys).code;
+      if (statuses[i] == 0) ok.push_back(i);
+    }
+    if (ok.empty()) return Ok();
     {
       std::lock_guard<std::mutex> lock(mu_);
       if (stopping_) return Error{PIRG_FAILED_PRECONDITION, "the serving loop is stopping"};
-      if (outstanding_ >= max_outstanding_) {
+      if (outstanding_ + static_cast<long long>(ok.size()) > max_outstanding_) {
         return Error{PIRG_RESOURCE_EXHAUSTED,
                      "serving queue is full (" + std::to_string(outstanding_) +
                          " requests submitted and not yet polled)"};
       }
-      ++outstanding_;
-      ++submitted_;
-      q.submitted = Clock::now();
-      pending_.push_back(std::move(q));
+      // Reserved now, so concurrent submits cannot overshoot while copying.
+      outstanding_ += static_cast<long long>(ok.size());
+    }
+    std::vector<ServeRequest> qs(ok.size());
+    {
+      std::lock_guard<std::mutex> lock(pool_mu_);
+      for (auto& q : qs) {
+        if (free_recs_.empty()) break;
+        q.rec = free_recs_.back();
+        free_recs_.pop_back();
+      }
+    }
+    auto copy = [&](size_t lo, size_t hi) {
+      for (size_t j = lo; j < hi; ++j) {
+        ServeRequest& q = qs[j];
+        q.tag = tags[ok[j]];
+        u64* dst;
+        if (q.rec >= 0) {
+          dst = records_.as<u64>() + static_cast<size_t>(q.rec) * rw_;
+        } else {
+          q.data = TakeBuffer(rw_);
+          dst = q.data.data();
+        }
+        WriteRecord(S_, reqs[ok[j]], dst);
+      }
+    };
+    CopyInParallel(qs.size(), qs.size() * rw_ * 8, 8, copy);
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      const Clock::time_point now = Clock::now();
+      for (auto& q : qs) {
+        q.submitted = now;
+        pending_.push_back(std::move(q));
+      }
+      submitted_ += static_cast<long long>(qs.size());
     }
     cv_work_.notify_one();
     return Ok();
@@ -134,6 +222,25 @@ class ServeLoop {
     *res = std::move(done_.front());
     done_.pop_front();
     --outstanding_;
+    return PIRG_OK;
+  }
+
+  // Up to max results: waits up to timeout_ms for the first, then takes
+  // whatever else is ready. Return codes as Poll.
+  int PollMany(int max, int timeout_ms, std::vector<ServeResult>* res) {
+    std::unique_lock<std::mutex> lock(mu_);
+    const auto until = Clock::now() + std::chrono::milliseconds(std::max(timeout_ms, 0));
+    while (done_.empty()) {
+      if (stopped_) return PIRG_FAILED_PRECONDITION;
+      if (cv_done_.wait_until(lock, until) == std::cv_status::timeout && done_.empty()) {
+        return stopped_ ? PIRG_FAILED_PRECONDITION : PIRG_DEADLINE_EXCEEDED;
+      }
+    }
+    while (!done_.empty() && static_cast<int>(res->size()) < max) {
+      res->push_back(std::move(done_.front()));
+      done_.pop_front();
+      --outstanding_;
+    }
     return PIRG_OK;
   }
 
@@ -212,20 +319,66 @@ class ServeLoop {
       ++answered_;
     }
     free_.push_back(f.slot);
+    if (!f.recs.empty()) {
+      std::lock_guard<std::mutex> plock(pool_mu_);
+      // Reversed, so later takes pop them in ascending order and a batch's
+      // records stay contiguous (one copy each run).
+      free_recs_.insert(free_recs_.end(), f.recs.rbegin(), f.recs.rend());
+      f.recs.clear();
+    }
   }
 
-  Error Launch(Slot& slot, std::vector<ServeRequest>& reqs) {
-    std::vector<pirg_request_view> views(reqs.size());
-    std::vector<int> idx(reqs.size());
-    for (size_t i = 0; i < reqs.size(); ++i) {
-      const ServeRequest& q = reqs[i];
-      views[i] = pirg_request_view{q.fd.data(), q.fd.size(), q.sd.data(),
-                                   q.sd.size() / static_cast<size_t>(S_->d), q.pk.data(),
-                                   q.pk.size() / static_cast<size_t>(S_->d)};
-      idx[i] = static_cast<int>(i);
+  // A buffer of at least n words from the pool (or a new one).
+  std::vector<u64> TakeBuffer(size_t n) {
+    std::vector<u64> b;
+    {
+      std::lock_guard<std::mutex> lock(pool_mu_);
+      if (!pool_.empty()) {
+        b = std::move(pool_.back());
+        pool_.pop_back();
+      }
     }
+    b.resize(n);
+    return b;
+  }
+
+  void GiveBuffer(std::vector<u64>&& b) {
+    std::lock_guard<std::mutex> lock(pool_mu_);
+    if (pool_.size() < static_cast<size_t>(max_batch_) * (nslots_ + 1)) {
+      pool_.push_back(std::move(b));
+    }
+  }
+
+  Error Launch(Slot& slot, InFlight& f) {
+    std::vector<ServeRequest>& reqs = f.reqs;
     const int n = static_cast<int>(reqs.size());
-    Error er = StageRequests(S_, slot, views.data(), idx);
+    // Pool records go to HBM as they are; heap records (pool exhausted) are
+    // moved into the slot's pinned buffer at their batch position first.
+    std::vector<const u64*> recs(reqs.size());
+    size_t heap = 0;
+    for (size_t i = 0; i < reqs.size(); ++i) {
+      ServeRequest& q = reqs[i];
+      if (q.rec >= 0) {
+        recs[i] = records_.as<u64>() + static_cast<size_t>(q.rec) * rw_;
+        f.recs.push_back(q.rec);
+      } else {
+        recs[i] = slot.ws.h_req.as<u64>() + i * rw_;
+        ++heap;
+      }
+    }
+    if (heap > 0) {
+      CopyInParallel(reqs.size(), heap * rw_ * 8, 16, [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+          if (reqs[i].rec < 0) {
+            std::memcpy(const_cast<u64*>(recs[i]), reqs[i].data.data(), rw_ * 8);
+          }
+        }
+      });
+      for (auto& q : reqs) {
+        if (q.rec < 0) GiveBuffer(std::move(q.data));
+      }
+    }
+    Error er = StageRecords(S_, slot, recs.data(), n);
     if (!er.ok()) return er;
     er = RunBatch(S_, slot, n, true);
     if (!er.ok()) return er;
@@ -273,7 +426,7 @@ class ServeLoop {
           ++batches_;
           batched_ += static_cast<long long>(n);
         }
-        Error er = Launch(*f.slot, f.reqs);
+        Error er = Launch(*f.slot, f);
         if (!er.ok()) {
           (void)hipStreamSynchronize(f.slot->stream);
           Deliver(f, er.code, er.msg);
@@ -316,6 +469,15 @@ class ServeLoop {
   Clock::time_point started_;
   std::mutex join_mu_;
   std::thread worker_;
+  // The pinned record pool and its free records, and heap record buffers
+  // for reuse when the pool is exhausted (a large request would otherwise
+  // cost a fresh allocation and its page faults on every submit); all under
+  // pool_mu_.
+  size_t rw_ = 0;
+  HostBuf records_;
+  std::mutex pool_mu_;
+  std::vector<int> free_recs_;
+  std::vector<std::vector<u64>> pool_;
 };
 
 // The server's loop, if serving, as a reference the caller keeps for the call.
@@ -369,6 +531,59 @@ int pirg_serve_submit(pirg_server* S, const pirg_request_view* req, uint64_t tag
   if (!loop) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is not serving"}, err, errlen);
   Error er = loop->Submit(*req, tag);
   return er.ok() ? 0 : Report(er, err, errlen);
+}
+
+int pirg_serve_submit_batch(pirg_server* S, int n, const pirg_request_view* reqs,
+                            const uint64_t* tags, int* statuses, char* err, size_t errlen) {
+  if (S == nullptr || n < 0 || (n > 0 && (reqs == nullptr || tags == nullptr ||
+                                          statuses == nullptr))) {
+    return Report(Invalid("invalid server, count, requests, tags or statuses."), err, errlen);
+  }
+  const std::shared_ptr<ServeLoop> loop = LoopOf(S);
+  if (!loop) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is not serving"}, err, errlen);
+  Error er = loop->SubmitMany(n, reqs, tags, statuses);
+  if (!er.ok()) return Report(er, err, errlen);
+  for (int i = 0; i < n; ++i) {
+    if (statuses[i] != 0) {
+      (void)Report(ValidateRequest(S, reqs[i].first_dim_len, reqs[i].second_dim_polys,
+                                   reqs[i].packing_key_polys),
+                   err, errlen);
+      break;
+    }
+  }
+  return 0;
+}
+
+int pirg_serve_poll_batch(pirg_server* S, int max, int timeout_ms, uint64_t* tags,
+                          uint64_t* const* outs, int* statuses, double* latency_ms, int* count,
+                          char* err, size_t errlen) {
+  if (S == nullptr || max < 1 || tags == nullptr || outs == nullptr || statuses == nullptr ||
+      count == nullptr) {
+    return Report(Invalid("invalid server, count or output arrays."), err, errlen);
+  }
+  *count = 0;
+  const std::shared_ptr<ServeLoop> loop = LoopOf(S);
+  if (!loop) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is not serving"}, err, errlen);
+  std::vector<ServeResult> rs;
+  const int rc = loop->PollMany(max, timeout_ms, &rs);
+  if (rc == PIRG_FAILED_PRECONDITION) {
+    return Report(Error{PIRG_FAILED_PRECONDITION, "the serving loop stopped"}, err, errlen);
+  }
+  if (rc != PIRG_OK) return rc;
+  bool reported = false;
+  for (size_t i = 0; i < rs.size(); ++i) {
+    tags[i] = rs[i].tag;
+    statuses[i] = rs[i].status;
+    if (latency_ms) latency_ms[i] = rs[i].latency_ms;
+    if (rs[i].status == 0) {
+      if (outs[i]) std::memcpy(outs[i], rs[i].out.data(), rs[i].out.size() * 8);
+    } else if (!reported && err != nullptr && errlen > 0) {
+      std::snprintf(err, errlen, "%s", rs[i].msg.c_str());
+      reported = true;
+    }
+  }
+  *count = static_cast<int>(rs.size());
+  return 0;
 }
 
 int pirg_serve_poll(pirg_server* S, int timeout_ms, uint64_t* tag, uint64_t* out, int* status,

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "server_internal.h"
@@ -461,9 +462,7 @@ std::vector<size_t> WorkspaceSizes(const pirg_server* S, int cap) {
   const int nu_e = S->t > 1 ? S->ext_u.num_chunks : 0;
   const int nv_e = S->t > 1 ? S->ext_v.num_chunks : 0;
   return {
-      nq * S->K * 8,
-      nq * 2 * S->e * d * 8,
-      nq * 2 * S->k * d * 8,
+      nq * RequestWords(S) * 8,
       bt_db,
       bt_pk,
       corr_words * 8,
@@ -506,7 +505,7 @@ Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap) {
   const int nu_e = S->t > 1 ? S->ext_u.num_chunks : 0;
   const int nv_e = S->t > 1 ? S->ext_v.num_chunks : 0;
   const std::vector<size_t> sizes = WorkspaceSizes(S, cap);
-  DevBuf* bufs[] = {&W.fd,       &W.sd,         &W.pk,          &W.bt_db,      &W.bt_pk,
+  DevBuf* bufs[] = {&W.req,                                      &W.bt_db,      &W.bt_pk,
                     &W.corr_db,  &W.corr_pk,    &W.db_partial,  &W.pk_partial, &W.yh_spec,
                     &W.sdb_spec, &W.d_spec,     &W.fin_partial, &W.ext_partial, &W.ct_b,
                     &W.st_a,     &W.st_b,       &W.out,         &W.fin_tab,    &W.ext_tab,
@@ -516,11 +515,7 @@ Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap) {
     Error er = bufs[i]->alloc(sizes[i]);
     if (!er.ok()) return er;
   }
-  Error er = W.h_fd.alloc(nq * S->K * 8);
-  if (!er.ok()) return er;
-  er = W.h_sd.alloc(nq * 2 * S->e * d * 8);
-  if (!er.ok()) return er;
-  er = W.h_pk.alloc(nq * 2 * S->k * d * 8);
+  Error er = W.h_req.alloc(nq * RequestWords(S) * 8);
   if (!er.ok()) return er;
   er = W.h_out.alloc(nq * S->ell * 2 * d * 8);
   if (!er.ok()) return er;
@@ -570,36 +565,70 @@ Error ValidateRequest(const pirg_server* S, size_t fd_len, size_t sd_polys,
   return Ok();
 }
 
-// Copies validated requests (reqs[idx[i]]) into the pinned staging buffers
-// and on to HBM.
+size_t RecordSdOffset(const pirg_server* S) { return static_cast<size_t>(S->K); }
+size_t RecordPkOffset(const pirg_server* S) {
+  return static_cast<size_t>(S->K) + (S->t > 1 ? static_cast<size_t>(2 * S->e) * S->d : 0);
+}
+size_t RequestWords(const pirg_server* S) {
+  return RecordPkOffset(S) + static_cast<size_t>(2 * S->k) * S->d;
+}
+
+void WriteRecord(const pirg_server* S, const pirg_request_view& R, u64* dst) {
+  const size_t d = S->d;
+  std::memcpy(dst, R.first_dim, R.first_dim_len * 8);
+  std::memset(dst + R.first_dim_len, 0, (S->K - R.first_dim_len) * 8);
+  if (S->t > 1) {
+    std::memcpy(dst + RecordSdOffset(S), R.second_dim, static_cast<size_t>(2 * S->e) * d * 8);
+  }
+  std::memcpy(dst + RecordPkOffset(S), R.packing_key, static_cast<size_t>(2 * S->k) * d * 8);
+}
+
+Error StageRecords(pirg_server* S, Slot& slot, const u64* const* records, int nq) {
+  const size_t rw = RequestWords(S);
+  u64* dev = slot.ws.req.as<u64>();
+  // Records that follow each other in host memory go as one copy.
+  for (int i = 0; i < nq;) {
+    int j = i + 1;
+    while (j < nq && records[j] == records[j - 1] + rw) ++j;
+    PIRG_HIP(hipMemcpyAsync(dev + static_cast<size_t>(i) * rw, records[i],
+                            static_cast<size_t>(j - i) * rw * 8, hipMemcpyHostToDevice,
+                            slot.stream));
+    i = j;
+  }
+  return Ok();
+}
+
+// Copies validated requests (reqs[idx[i]]) as records into the pinned
+// staging buffer and on to HBM.
 Error StageRequests(pirg_server* S, Slot& slot, const pirg_request_view* reqs,
                     const std::vector<int>& idx) {
   Workspace& W = slot.ws;
-  const size_t d = S->d;
   const int nq = static_cast<int>(idx.size());
-  u64* hfd = W.h_fd.as<u64>();
-  u64* hsd = W.h_sd.as<u64>();
-  u64* hpk = W.h_pk.as<u64>();
-  for (int i = 0; i < nq; ++i) {
-    const pirg_request_view& R = reqs[idx[i]];
-    u64* f = hfd + static_cast<size_t>(i) * S->K;
-    std::memcpy(f, R.first_dim, R.first_dim_len * 8);
-    std::memset(f + R.first_dim_len, 0, (S->K - R.first_dim_len) * 8);
-    if (S->t > 1) {
-      std::memcpy(hsd + static_cast<size_t>(i) * 2 * S->e * d, R.second_dim,
-                  static_cast<size_t>(2 * S->e) * d * 8);
+  const size_t rw = RequestWords(S);
+  u64* h = W.h_req.as<u64>();
+  auto stage = [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) WriteRecord(S, reqs[idx[i]], h + static_cast<size_t>(i) * rw);
+  };
+  // A large batch (hundreds of MB at n = 2^20) is copied into the pinned
+  // buffer by several threads: one thread's memcpy would take longer than
+  // the batch's GPU work.
+  const size_t total = rw * 8 * static_cast<size_t>(nq);
+  const int threads = static_cast<int>(std::min<size_t>(
+      {static_cast<size_t>(std::max(1u, std::thread::hardware_concurrency())), 16,
+       total / (size_t(8) << 20) + 1, static_cast<size_t>(nq)}));
+  if (threads <= 1) {
+    stage(0, nq);
+  } else {
+    std::vector<std::thread> pool;
+    const int per = (nq + threads - 1) / threads;
+    for (int t = 1; t < threads; ++t) {
+      const int lo = std::min(nq, t * per), hi = std::min(nq, lo + per);
+      if (lo < hi) pool.emplace_back(stage, lo, hi);
     }
-    std::memcpy(hpk + static_cast<size_t>(i) * 2 * S->k * d, R.packing_key,
-                static_cast<size_t>(2 * S->k) * d * 8);
+    stage(0, std::min(nq, per));
+    for (auto& th : pool) th.join();
   }
-  PIRG_HIP(hipMemcpyAsync(W.fd.p, hfd, static_cast<size_t>(nq) * S->K * 8,
-                          hipMemcpyHostToDevice, slot.stream));
-  if (S->t > 1) {
-    PIRG_HIP(hipMemcpyAsync(W.sd.p, hsd, static_cast<size_t>(nq) * 2 * S->e * d * 8,
-                            hipMemcpyHostToDevice, slot.stream));
-  }
-  PIRG_HIP(hipMemcpyAsync(W.pk.p, hpk, static_cast<size_t>(nq) * 2 * S->k * d * 8,
-                          hipMemcpyHostToDevice, slot.stream));
+  PIRG_HIP(hipMemcpyAsync(W.req.p, h, total, hipMemcpyHostToDevice, slot.stream));
   return Ok();
 }
 
@@ -614,8 +643,13 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   const int d = S->d;
   const int lt = S->ell * S->t;
   unsigned long long* margin = S->margin.as<unsigned long long>();
+  // Stage boundaries: eager runs record them; under capture the two inner
+  // boundaries become event-record nodes of the graph (RunBatch records the
+  // outer two around each replay), so replays report stage times too.
   auto mark = [&](hipEvent_t e) -> hipError_t {
-    return timed ? hipEventRecord(e, st) : hipSuccess;
+    if (timed) return hipEventRecord(e, st);
+    if (e == slot.ev[0] || e == slot.ev[3]) return hipSuccess;
+    return hipEventRecordWithFlags(e, st, hipEventRecordExternal);
   };
   // Scan brackets: eager runs record them directly; under capture with
   // graph_scan_events they become event-record nodes of the graph.
@@ -628,8 +662,13 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   // DB product (pir_server.cc:215-229).
   const ScanPlan pd = PlanFor(S, true, nq);
   const ScanPlan pp = PlanFor(S, false, nq);
-  PIRG_HIP(LaunchQueryDigits(W.fd.as<u64>(), S->K, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
-  PIRG_HIP(LaunchVectorSums(W.fd.as<u64>(), nq, S->K, S->K, S->dbL.c_off,
+  // Request records (RequestWords apart): first_dim, second_dim, packing key.
+  const int rw = static_cast<int>(RequestWords(S));
+  const u64* rec_fd = W.req.as<u64>();
+  const u64* rec_sd = W.req.as<u64>() + RecordSdOffset(S);
+  const u64* rec_pk = W.req.as<u64>() + RecordPkOffset(S);
+  PIRG_HIP(LaunchQueryDigits(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
+  PIRG_HIP(LaunchVectorSums(rec_fd, nq, rw, S->K, S->dbL.c_off,
                             W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
   PIRG_HIP(kmark(slot.kev[0]));
   PIRG_HIP(LaunchScan(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
@@ -637,9 +676,8 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   PIRG_HIP(kmark(slot.kev[1]));
   PIRG_HIP(mark(slot.ev[1]));
   // Pack product (pir_server.cc:237-248): y = concat(y_vec_g).
-  PIRG_HIP(LaunchQueryDigits(W.pk.as<u64>(), 2 * S->k * d, S->k * d, S->pkL, pp,
-                              W.bt_pk.as<int8_t>(), st));
-  PIRG_HIP(LaunchVectorSums(W.pk.as<u64>(), nq, 2 * S->k * d, S->k * d, S->pkL.c_off,
+  PIRG_HIP(LaunchQueryDigits(rec_pk, rw, S->k * d, S->pkL, pp, W.bt_pk.as<int8_t>(), st));
+  PIRG_HIP(LaunchVectorSums(rec_pk, nq, rw, S->k * d, S->pkL.c_off,
                             W.corr_pk.as<u64>() + nq, W.corr_pk.as<u64>(), st));
   PIRG_HIP(kmark(slot.kev[2]));
   PIRG_HIP(LaunchScan(S->pk_tiles.as<int8_t>(), S->pkL, pp, W.bt_pk.as<int8_t>(),
@@ -648,9 +686,9 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279). A
   // local-finalize server (vPIR, vpir_server.cc:148-160) adds no inner product.
   if (!S->local_finalize) {
-    PIRG_HIP(LaunchSpectra(W.pk.as<u64>() + static_cast<size_t>(S->k) * d, nq * S->k, S->k,
-                           2LL * S->k * d, S->fin_v, S->F, S->tw_fwd.as<double2>(),
-                           W.yh_spec.as<double2>(), st));
+    PIRG_HIP(LaunchSpectra(rec_pk + static_cast<size_t>(S->k) * d, nq * S->k, S->k, rw,
+                           S->fin_v, S->F, S->tw_fwd.as<double2>(), W.yh_spec.as<double2>(),
+                           st));
     const size_t n_fin_cap = static_cast<size_t>(W.cap) * lt;
     void* const* tab = W.fin_tab.as<void* const>();
     PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(tab),
@@ -687,9 +725,8 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   PIRG_HIP(mark(slot.ev[2]));
   // EvalPoly (pir_server.cc:126-150) and the modulus switch.
   if (S->t > 1) {
-    PIRG_HIP(LaunchSpectra(W.sd.as<u64>(), nq * 2 * S->e, 2 * S->e, 2LL * S->e * d,
-                           S->ext_v, S->F, S->tw_fwd.as<double2>(),
-                           W.sdb_spec.as<double2>(), st));
+    PIRG_HIP(LaunchSpectra(rec_sd, nq * 2 * S->e, 2 * S->e, rw, S->ext_v, S->F,
+                           S->tw_fwd.as<double2>(), W.sdb_spec.as<double2>(), st));
     const size_t n_ext_cap = static_cast<size_t>(W.cap) * S->ell * 2;
     void* const* et = W.ext_tab.as<void* const>();
     for (int i = S->t - 2; i >= 0; --i) {
@@ -785,18 +822,28 @@ void CollectTiming(pirg_server* S, Slot& slot) {
   if (hipEventSynchronize(slot.ev[3]) != hipSuccess) return;
   slot.timing_pending = false;
   if (slot.graph_run) {
-    // A graph replay has no per-stage events; its scan brackets are nodes of
-    // the graph when it was captured with graph_scan_events.
-    if (!slot.graph_scan_events) return;
-    float ka = 0, kb = 0;
-    if (hipEventElapsedTime(&ka, slot.kev[0], slot.kev[1]) != hipSuccess ||
-        hipEventElapsedTime(&kb, slot.kev[2], slot.kev[3]) != hipSuccess) {
+    // A replay's stage boundaries: ev[0] / ev[3] around the launch, ev[1] /
+    // ev[2] nodes inside the graph; its scan brackets are nodes too when it
+    // was captured with graph_scan_events.
+    float a = 0, b = 0, c = 0;
+    if (hipEventElapsedTime(&a, slot.ev[0], slot.ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&b, slot.ev[1], slot.ev[2]) != hipSuccess ||
+        hipEventElapsedTime(&c, slot.ev[2], slot.ev[3]) != hipSuccess) {
       return;
     }
+    float ka = 0, kb = 0;
+    const bool scans = slot.graph_scan_events &&
+                       hipEventElapsedTime(&ka, slot.kev[0], slot.kev[1]) == hipSuccess &&
+                       hipEventElapsedTime(&kb, slot.kev[2], slot.kev[3]) == hipSuccess;
     std::lock_guard<std::mutex> lock(S->mu);
-    S->db_scan_ms += ka;
-    S->pk_scan_ms += kb;
-    S->scans += 1;
+    S->db_ms += a;
+    S->pack_ms += b;
+    S->eval_ms += c;
+    if (scans) {
+      S->db_scan_ms += ka;
+      S->pk_scan_ms += kb;
+      S->scans += 1;
+    }
     return;
   }
   float a = 0, b = 0, c = 0;
@@ -1464,96 +1511,6 @@ int pirg_inner_product_fft(const uint64_t* u, const uint64_t* v, int n, int d,
   };
   Error er = run();
   return er.ok() ? 0 : Report(er, err, errlen);
-}
-
-int pirg_copy_probe(size_t bytes, int repeats, double* ms) {
-  auto run = [&]() -> Error {
-    Error er = CheckDevice();
-    if (!er.ok()) return er;
-    DevBuf a, b;
-    if ((er = a.alloc(bytes)).code) return er;
-    if ((er = b.alloc(bytes)).code) return er;
-    CallStream cs;
-    PIRG_HIP(cs.create());
-    PIRG_HIP(FillSync(a.p, 1, bytes, cs.s));
-    hipEvent_t e0, e1;
-    PIRG_HIP(hipEventCreate(&e0));
-    PIRG_HIP(hipEventCreate(&e1));
-    PIRG_HIP(LaunchCopyProbe(a.p, b.p, bytes, cs.s));
-    PIRG_HIP(hipEventRecord(e0, cs.s));
-    for (int i = 0; i < repeats; ++i) PIRG_HIP(LaunchCopyProbe(a.p, b.p, bytes, cs.s));
-    PIRG_HIP(hipEventRecord(e1, cs.s));
-    PIRG_HIP(hipEventSynchronize(e1));
-    float t = 0;
-    PIRG_HIP(hipEventElapsedTime(&t, e0, e1));
-    if (ms) *ms = t / std::max(repeats, 1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return Ok();
-  };
-  Error er = run();
-  return er.code;
-}
-
-int pirg_read_probe(size_t bytes, int repeats, double* ms) {
-  auto run = [&]() -> Error {
-    Error er = CheckDevice();
-    if (!er.ok()) return er;
-    DevBuf a, o;
-    if ((er = a.alloc(bytes)).code) return er;
-    if ((er = o.alloc(static_cast<size_t>(ReadProbeOutWords()) * 4)).code) return er;
-    CallStream cs;
-    PIRG_HIP(cs.create());
-    PIRG_HIP(FillSync(a.p, 1, bytes, cs.s));
-    hipEvent_t e0, e1;
-    PIRG_HIP(hipEventCreate(&e0));
-    PIRG_HIP(hipEventCreate(&e1));
-    const int passes = std::max(repeats, 1);
-    PIRG_HIP(LaunchReadProbe(a.p, bytes, 1, o.as<int>(), cs.s));
-    PIRG_HIP(hipEventRecord(e0, cs.s));
-    PIRG_HIP(LaunchReadProbe(a.p, bytes, passes, o.as<int>(), cs.s));
-    PIRG_HIP(hipEventRecord(e1, cs.s));
-    PIRG_HIP(hipEventSynchronize(e1));
-    float t = 0;
-    PIRG_HIP(hipEventElapsedTime(&t, e0, e1));
-    if (ms) *ms = t / passes;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return Ok();
-  };
-  Error er = run();
-  return er.code;
-}
-
-int pirg_mfma_probe(int blocks, int iters, double* ms, double* tops) {
-  auto run = [&]() -> Error {
-    Error er = CheckDevice();
-    if (!er.ok()) return er;
-    if (blocks <= 0 || iters <= 0) return Invalid("blocks and iters must be positive.");
-    DevBuf o;
-    if ((er = o.alloc(static_cast<size_t>(blocks) * 256 * 4)).code) return er;
-    CallStream cs;
-    PIRG_HIP(cs.create());
-    hipEvent_t e0, e1;
-    PIRG_HIP(hipEventCreate(&e0));
-    PIRG_HIP(hipEventCreate(&e1));
-    PIRG_HIP(LaunchMfmaProbe(blocks, iters, o.as<int>(), cs.s));
-    PIRG_HIP(hipEventRecord(e0, cs.s));
-    PIRG_HIP(LaunchMfmaProbe(blocks, iters, o.as<int>(), cs.s));
-    PIRG_HIP(hipEventRecord(e1, cs.s));
-    PIRG_HIP(hipEventSynchronize(e1));
-    float t = 0;
-    PIRG_HIP(hipEventElapsedTime(&t, e0, e1));
-    if (ms) *ms = t;
-    // 4 waves per block, 16 MFMAs per round, 16 x 16 x 64 x 2 ops each.
-    const double ops = static_cast<double>(blocks) * 4 * iters * 16 * 32768.0;
-    if (tops) *tops = ops / (t * 1e-3) / 1e12;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return Ok();
-  };
-  Error er = run();
-  return er.code;
 }
 
 int pirg_fft_twiddles(int n, double* fwd, double* bwd) {

@@ -103,7 +103,7 @@ struct HostBuf {
 // Per-batch working memory, sized for `cap` requests.
 struct Workspace {
   int cap = 0;
-  DevBuf fd, sd, pk;             // staged requests
+  DevBuf req;                    // staged requests, one record each (RequestWords)
   DevBuf bt_db, bt_pk;           // B-operand tiles
   DevBuf corr_db, corr_pk;
   DevBuf db_partial, pk_partial;
@@ -112,7 +112,7 @@ struct Workspace {
   DevBuf ct_b, st_a, st_b, out;
   DevBuf fin_tab, ext_tab;       // IpKernel pointer tables
   DevBuf mac_acc;                // accumulated spectra of small IpKernel launches
-  HostBuf h_fd, h_sd, h_pk, h_out;
+  HostBuf h_req, h_out;
 };
 
 // One execution context: a workspace, its stream, timing events and the
@@ -172,9 +172,10 @@ struct pirg_server {
   int mat_src_bytes = 4;
   pirg::Slot main;
   int staged = 0;
-  // pirg_process_batch / pirg_batch_run replay hipGraphs instead of eager
-  // launches (the serving loop always does).
-  bool use_graphs = false;
+  // pirg_process / pirg_process_batch / pirg_batch_run replay one hipGraph
+  // per batch size instead of launching eagerly (the serving loop always
+  // does). On by default; pirg_server_use_graphs(server, 0) turns it off.
+  bool use_graphs = true;
   // FinalizeMatrixPack without the t_vec_h inner product (vPIR's local finalize).
   bool local_finalize = false;
   // Cumulative stage times (GetDbMultiplyTime / GetPackingTime /
@@ -195,6 +196,17 @@ struct pirg_server {
 
 namespace pirg {
 
+// A staged request's record: first_dim zero-padded to K words, then (when
+// t > 1) the 2e second-dimension polynomials, then the 2k packing-key
+// polynomials (y_vec_g, y_vec_h). Batches hold one record per request.
+size_t RequestWords(const pirg_server* S);
+size_t RecordSdOffset(const pirg_server* S);
+size_t RecordPkOffset(const pirg_server* S);
+// Writes a validated request's record to dst (RequestWords words).
+void WriteRecord(const pirg_server* S, const pirg_request_view& r, u64* dst);
+// Enqueues the host-to-device copies of nq records (host memory, pinned for
+// an asynchronous copy) into the slot's workspace, record i at position i.
+Error StageRecords(pirg_server* S, Slot& slot, const u64* const* records, int nq);
 // pir_server.cc:161-194 request validation, in the reference's order.
 Error ValidateRequest(const pirg_server* S, size_t fd_len, size_t sd_polys, size_t pk_polys);
 // Bytes of device workspace a slot needs for `cap` requests.

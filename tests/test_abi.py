@@ -29,6 +29,18 @@ def test_libpirgpu_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_probes_library_exports_its_header_and_the_product_does_not():
+    # The calibration probes live in their own library, outside the answer
+    # path's ABI.
+    from inspire_amd import probes
+    names = _declared(os.path.join(ROOT, "include", "pirgpu_probes.h"), "pirg_")
+    assert names == ["pirg_copy_probe", "pirg_mfma_probe", "pirg_read_probe"]
+    plib = probes.lib()
+    assert all(hasattr(plib, n) for n in names)
+    product = set(_declared(os.path.join(ROOT, "include", "pirgpu.h"), "pirg_"))
+    assert not product & set(names)
+
+
 def test_liboracle_exports_every_declared_symbol():
     names = _declared(os.path.join(ROOT, "oracle", "pir_oracle.h"), "orc_")
     lib = O.lib()

@@ -275,3 +275,36 @@ def test_stop_wakes_a_blocked_poller(material, requests):
         srv.serve_stop()  # stopping twice is harmless
     want, _, _ = O.process_response(P, material, requests[1])
     np.testing.assert_array_equal(r[1].ciphertexts, want)
+
+
+def test_batched_submit_and_poll(material, requests):
+    # pirg_serve_submit_batch / pirg_serve_poll_batch: per-request validation
+    # statuses, all-or-nothing admission, and answers equal to the oracle's.
+    with server_from_material(P, material) as srv:
+        srv.serve_start(max_batch=8, slots=2, max_outstanding=24)
+        try:
+            bad = O.Request(requests[0].first_dim, requests[0].second_dim,
+                            requests[0].packing_key[:-1])
+            batch = [gpu_request(r) for r in requests[:10]] + [gpu_request(bad)]
+            st = srv.submit_batch(batch, list(range(11)))
+            assert st[:10] == [0] * 10 and st[10] == ia.INVALID_ARGUMENT
+            with pytest.raises(ia.PirError) as ei:   # 10 + 15 > 24: none queued
+                srv.submit_batch([gpu_request(r) for r in requests[10:25]],
+                                 list(range(100, 115)))
+            assert ei.value.code == ia.RESOURCE_EXHAUSTED
+            got = {}
+            while len(got) < 10:
+                r = srv.poll_batch(16, 30000)
+                assert r is not None
+                tags, outs, sts, lats = r
+                for tg, o, s_, lt in zip(tags, outs, sts, lats):
+                    assert s_ == 0 and lt > 0
+                    got[tg] = o.copy()
+            assert sorted(got) == list(range(10))
+            assert srv.poll_batch(4, 50) is None
+        finally:
+            srv.serve_stop()
+    for i in (0, 9):
+        want, _, _ = O.process_response(P, material, requests[i])
+        np.testing.assert_array_equal(got[i], want)
+
