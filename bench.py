@@ -241,6 +241,49 @@ def run_e2e(srv: ia.PirServer, reqs, batches: int, submitters: int = 4, chunk: i
                                         reqs[0].packing_key.nbytes)}
 
 
+def run_e2e_records(srv: ia.PirServer, reqs, batches: int, slots: int = 2):
+    """As run_e2e, with the requests held as records in pinned host memory
+    (HostRecords, written once before the timed region, as a server that
+    receives requests into pinned buffers holds them): each batch of
+    len(reqs) records goes in through one pirg_serve_submit_records call, with
+    no host copy, and travels to HBM as one DMA. The answers of three records
+    are checked against pirg_process on the same request."""
+    B = len(reqs)
+    n = B * batches
+    with ia.HostRecords(srv, B) as hr:
+        for i, r in enumerate(reqs):
+            srv.write_record(r, hr.array[i])
+        rows = [hr.array[i] for i in range(B)]
+        srv.serve_start(max_batch=B, slots=slots, max_outstanding=n + 2 * B)
+        try:
+            srv.submit_records(rows + rows, list(range(2 * B)))   # warm both slots
+            for _ in range(2 * B):
+                assert srv.poll(60000) is not None
+            srv.serve_stats(reset=True)
+            done = {}
+            t0 = time.perf_counter()
+            for b in range(batches):
+                srv.submit_records(rows, list(range(b * B, (b + 1) * B)))
+            submit_s = time.perf_counter() - t0
+            while len(done) < n:
+                r = srv.poll_batch(B, 30000)
+                if r is None:
+                    break
+                for tg, o, s_ in zip(r[0], r[1], r[2]):
+                    done[tg] = (s_, o.copy() if tg % B in (0, B // 2, B - 1) else None)
+            wall = time.perf_counter() - t0
+            st = srv.serve_stats()
+        finally:
+            srv.serve_stop()
+    same = all(np.array_equal(done[b * B + i][1], srv.process_response(reqs[i]).ciphertexts)
+               for i in (0, B // 2, B - 1) for b in (0, batches - 1) if b * B + i in done)
+    return {"requests": n, "batch": B, "slots": st["slots"], "answered": len(done),
+            "failed": sum(1 for s_, _ in done.values() if s_ != 0),
+            "e2e_qps": len(done) / wall, "submit_s": submit_s, "wall_s": wall,
+            "mean_batch": st["mean_batch"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+            "record_bytes": srv.record_words() * 8, "matches_process": bool(same)}
+
+
 def run_small_config(name: str, n: int, log_d: int, t: int, B: int, steps: int, warmup: int,
                      cpu: bool):
     """One of the smaller BASELINE workloads on its own material: the plain
@@ -838,6 +881,12 @@ def main():
         e2e["over_value"] = e2e["e2e_qps"] / best["qps"]
         log(f"end to end (host-resident requests): {e2e['e2e_qps']:.0f} q/s = "
             f"{e2e['over_value']:.2f} x value, mean batch {e2e['mean_batch']:.1f}")
+        zc = run_e2e_records(srv, host_reqs, args.e2e_batches, slots=args.e2e_slots)
+        zc["over_value"] = zc["e2e_qps"] / best["qps"]
+        e2e["zero_copy_records"] = zc
+        log(f"end to end (pinned request records, zero-copy submit): {zc['e2e_qps']:.0f} q/s = "
+            f"{zc['over_value']:.2f} x value, mean batch {zc['mean_batch']:.1f}, "
+            f"answers match pirg_process: {zc['matches_process']}")
         del host_reqs
     want0 = srv.process_response(reqs[0]).ciphertexts
     srv.close()

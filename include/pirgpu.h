@@ -219,6 +219,29 @@ int pirg_serve_submit_batch(pirg_server* server, int n, const pirg_request_view*
 int pirg_serve_poll_batch(pirg_server* server, int max, int timeout_ms, uint64_t* tags,
                           uint64_t* const* outs, int* statuses, double* latency_ms, int* count,
                           char* err, size_t errlen);
+/* Zero-copy submission (no counterpart in the reference: a serving form of
+ * PirServer::ProcessResponse, crypto/pir/server/pir_server.cc:152-306, for
+ * callers that receive requests straight into pinned host memory).
+ *
+ * A request record holds one request in pirg_request_record_words(server)
+ * words: the first dimension zero-padded to the DB's padded column count,
+ * the second dimension (t > 1), the packing key. pirg_request_record
+ * validates a request as ProcessResponse does (pir_server.cc:161-194) and
+ * writes its record. Records in memory from pirg_host_alloc go to
+ * pirg_serve_submit_records, which queues them without a host copy (records
+ * adjacent in memory travel to HBM as one copy); each must stay allocated
+ * and unchanged until its response is polled. pirg_serve_submit_records
+ * returns PIRG_INVALID_ARGUMENT for a record outside such memory and
+ * PIRG_RESOURCE_EXHAUSTED past max_outstanding, queuing none; responses come
+ * back through pirg_serve_poll / pirg_serve_poll_batch. pirg_host_free takes
+ * what pirg_host_alloc returned. Thread-safe. */
+int pirg_host_alloc(size_t bytes, void** out, char* err, size_t errlen);
+void pirg_host_free(void* p);
+size_t pirg_request_record_words(const pirg_server* server);
+int pirg_request_record(const pirg_server* server, const pirg_request_view* req,
+                        uint64_t* record, char* err, size_t errlen);
+int pirg_serve_submit_records(pirg_server* server, int n, const uint64_t* const* records,
+                              const uint64_t* tags, char* err, size_t errlen);
 typedef struct {
   long long submitted, answered, batches;
   double mean_batch;            /* requests per launched batch */

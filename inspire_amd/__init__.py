@@ -63,7 +63,42 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_double), ctypes.c_char_p,
                                          ctypes.c_size_t]
+        _lib.pirg_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.c_char_p, ctypes.c_size_t]
+        _lib.pirg_host_free.argtypes = [ctypes.c_void_p]
+        _lib.pirg_host_free.restype = None
+        _lib.pirg_request_record_words.argtypes = [ctypes.c_void_p]
+        _lib.pirg_request_record_words.restype = ctypes.c_size_t
     return _lib
+
+
+class HostRecords:
+    """n request records of a server's layout in pinned host memory
+    (pirg_host_alloc), as the (n, words) uint64 array `array`. Records written
+    with PirServer.write_record are submitted without a host copy by
+    PirServer.submit_records; keep them (and this object) until their
+    responses are polled."""
+
+    def __init__(self, server: "PirServer", n: int):
+        self.words = int(lib().pirg_request_record_words(server._h))
+        p = ctypes.c_void_p()
+        err = _err()
+        _check(lib().pirg_host_alloc(n * self.words * 8, ctypes.byref(p), err, len(err)), err)
+        self._p = p.value
+        buf = (ctypes.c_uint64 * (n * self.words)).from_address(self._p)
+        self.array = np.ctypeslib.as_array(buf).reshape(n, self.words)
+
+    def close(self) -> None:
+        if self._p is not None:
+            self.array = None
+            lib().pirg_host_free(self._p)
+            self._p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class _CServeStats(ctypes.Structure):
@@ -547,6 +582,33 @@ class PirServer:
                                            ctypes.c_size_t(len(err)))
         _check(rc, err)
         return list(st)[:n]
+
+    def record_words(self) -> int:
+        """Words of one request record (pirg_request_record_words)."""
+        return int(lib().pirg_request_record_words(self._h))
+
+    def write_record(self, request: PirRequest, record: np.ndarray) -> None:
+        """Validates a request as ProcessResponse does and writes its record
+        into `record` (a row of HostRecords.array)."""
+        if record.dtype != np.uint64 or not record.flags.c_contiguous or \
+                record.size < self.record_words():
+            raise ValueError("record must be a contiguous uint64 row of record_words() words")
+        ra = _RequestArrays(request, self.params.degree)
+        view = ra.view()
+        err = _err()
+        _check(lib().pirg_request_record(self._h, ctypes.byref(view),
+                                         record.ctypes.data_as(_u64p), err,
+                                         ctypes.c_size_t(len(err))), err)
+
+    def submit_records(self, records: Sequence[np.ndarray], tags: Sequence[int]) -> None:
+        """Queues records written by write_record into HostRecords memory,
+        with no host copy (pirg_serve_submit_records): all or none."""
+        n = len(records)
+        ptrs = (_u64p * max(1, n))(*[r.ctypes.data_as(_u64p) for r in records])
+        tg = (ctypes.c_uint64 * max(1, n))(*tags)
+        err = _err()
+        _check(lib().pirg_serve_submit_records(self._h, n, ptrs, tg, err,
+                                               ctypes.c_size_t(len(err))), err)
 
     def poll_batch(self, max_results: int, timeout_ms: int = 1000):
         """Up to max_results finished requests as (tags, responses array

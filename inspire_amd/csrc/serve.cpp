@@ -22,6 +22,8 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <mutex>
 #include <thread>
 
 #include "server_internal.h"
@@ -39,14 +41,30 @@ double MsSince(Clock::time_point t0, Clock::time_point t1) {
 }  // namespace
 
 // A queued request, as its record (RequestWords words: first dimension,
-// second dimension, packing key): in the loop's pinned record pool (rec >= 0)
-// or, when the pool is exhausted, in a heap buffer.
+// second dimension, packing key): the caller's own record in pinned memory
+// (ext), one of the loop's pinned record pool (rec >= 0) or, when the pool
+// is exhausted, a heap buffer.
 struct ServeRequest {
   uint64_t tag = 0;
   Clock::time_point submitted;
+  const u64* ext = nullptr;
   int rec = -1;
   std::vector<u64> data;
 };
+
+// Pinned host blocks handed out by pirg_host_alloc, by start address.
+std::mutex g_host_mu;
+std::map<uintptr_t, size_t> g_host_blocks;
+
+// Whether [p, p + bytes) lies inside one block from pirg_host_alloc.
+bool InHostBlock(const void* p, size_t bytes) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  std::lock_guard<std::mutex> lock(g_host_mu);
+  auto it = g_host_blocks.upper_bound(a);
+  if (it == g_host_blocks.begin()) return false;
+  --it;
+  return a - it->first <= it->second && bytes <= it->second - (a - it->first);
+}
 
 struct ServeResult {
   uint64_t tag = 0;
@@ -129,6 +147,13 @@ class ServeLoop {
       if (!er.ok()) return er;
       er = EnsureWorkspace(S_, *slots_.back(), max_batch_);
       if (!er.ok()) return er;
+      // Every batch size the loop launches has its graph before the first
+      // request arrives: a capture in the loop would stall every batch
+      // behind it.
+      for (int n = 1; n <= max_batch_; n = Bucket(n) + 1) {
+        er = PrepareGraph(S_, *slots_.back(), Bucket(n));
+        if (!er.ok()) return er;
+      }
       free_.push_back(slots_.back().get());
     }
     started_ = Clock::now();
@@ -159,17 +184,8 @@ class ServeLoop {
       if (statuses[i] == 0) ok.push_back(i);
     }
     if (ok.empty()) return Ok();
-    {
-      std::lock_guard<std::mutex> lock(mu_);
-      if (stopping_) return Error{PIRG_FAILED_PRECONDITION, "the serving loop is stopping"};
-      if (outstanding_ + static_cast<long long>(ok.size()) > max_outstanding_) {
-        return Error{PIRG_RESOURCE_EXHAUSTED,
-                     "serving queue is full (" + std::to_string(outstanding_) +
-                         " requests submitted and not yet polled)"};
-      }
-      // Reserved now, so concurrent submits cannot overshoot while copying.
-      outstanding_ += static_cast<long long>(ok.size());
-    }
+    Error re = Reserve(ok.size());
+    if (!re.ok()) return re;
     std::vector<ServeRequest> qs(ok.size());
     {
       std::lock_guard<std::mutex> lock(pool_mu_);
@@ -194,16 +210,31 @@ class ServeLoop {
       }
     };
     CopyInParallel(qs.size(), qs.size() * rw_ * 8, 8, copy);
-    {
-      std::lock_guard<std::mutex> lock(mu_);
-      const Clock::time_point now = Clock::now();
-      for (auto& q : qs) {
-        q.submitted = now;
-        pending_.push_back(std::move(q));
+    Enqueue(qs);
+    return Ok();
+  }
+
+  // Queues n records the caller wrote (pirg_request_record) into memory from
+  // pirg_host_alloc: they are copied to HBM from where they lie, with no
+  // host copy. All or none are queued.
+  Error SubmitRecords(int n, const u64* const* records, const uint64_t* tags) {
+    for (int i = 0; i < n; ++i) {
+      if (records[i] == nullptr || !InHostBlock(records[i], rw_ * 8)) {
+        return Invalid("record " + std::to_string(i) + " does not lie, whole (" +
+                       std::to_string(rw_ * 8) +
+                       " bytes), in memory from pirg_host_alloc.");
       }
-      submitted_ += static_cast<long long>(qs.size());
     }
-    cv_work_.notify_one();
+    if (!S_->deferred.ok()) return S_->deferred;
+    if (n == 0) return Ok();
+    Error re = Reserve(static_cast<size_t>(n));
+    if (!re.ok()) return re;
+    std::vector<ServeRequest> qs(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+      qs[i].tag = tags[i];
+      qs[i].ext = records[i];
+    }
+    Enqueue(qs);
     return Ok();
   }
 
@@ -297,9 +328,11 @@ class ServeLoop {
   void Deliver(InFlight& f, int status, const std::string& msg) {
     const size_t per = ResponseWords(S_);
     const Clock::time_point now = Clock::now();
-    std::lock_guard<std::mutex> lock(mu_);
+    // The responses leave the slot's pinned buffer before the lock is taken:
+    // submitters and pollers are not held up by the copies.
+    std::vector<ServeResult> rs(f.reqs.size());
     for (size_t i = 0; i < f.reqs.size(); ++i) {
-      ServeResult r;
+      ServeResult& r = rs[i];
       r.tag = f.reqs[i].tag;
       r.status = status;
       r.msg = msg;
@@ -308,6 +341,9 @@ class ServeLoop {
         r.out.assign(src, src + per);
       }
       r.latency_ms = MsSince(f.reqs[i].submitted, now);
+    }
+    std::lock_guard<std::mutex> lock(mu_);
+    for (auto& r : rs) {
       // The latest kLatencyWindow latencies, as a ring.
       if (latencies_.size() < kLatencyWindow) {
         latencies_.push_back(r.latency_ms);
@@ -326,6 +362,46 @@ class ServeLoop {
       free_recs_.insert(free_recs_.end(), f.recs.rbegin(), f.recs.rend());
       f.recs.clear();
     }
+  }
+
+  // Counts n requests as outstanding, or refuses them all.
+  Error Reserve(size_t n) {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (stopping_) return Error{PIRG_FAILED_PRECONDITION, "the serving loop is stopping"};
+    if (outstanding_ + static_cast<long long>(n) > max_outstanding_) {
+      return Error{PIRG_RESOURCE_EXHAUSTED,
+                   "serving queue is full (" + std::to_string(outstanding_) +
+                       " requests submitted and not yet polled)"};
+    }
+    // Reserved now, so concurrent submits cannot overshoot while copying.
+    outstanding_ += static_cast<long long>(n);
+    return Ok();
+  }
+
+  // Queues reserved requests for the worker.
+  void Enqueue(std::vector<ServeRequest>& qs) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      const Clock::time_point now = Clock::now();
+      for (auto& q : qs) {
+        q.submitted = now;
+        pending_.push_back(std::move(q));
+      }
+      submitted_ += static_cast<long long>(qs.size());
+    }
+    cv_work_.notify_one();
+  }
+
+  // The batch size n requests run at: powers of two up to 16, then
+  // multiples of 16, at most max_batch. The padding reads zero or earlier
+  // valid records of the slot and its answers are not fetched; a scan costs
+  // little more for up to 15 more queries, and the loop needs one graph per
+  // size (at most 20 for 256 requests).
+  int Bucket(int n) const {
+    int b = 1;
+    while (b < n && b < 16) b <<= 1;
+    if (n > 16) b = (n + 15) / 16 * 16;
+    return std::min(b, max_batch_);
   }
 
   // A buffer of at least n words from the pool (or a new one).
@@ -358,7 +434,9 @@ class ServeLoop {
     size_t heap = 0;
     for (size_t i = 0; i < reqs.size(); ++i) {
       ServeRequest& q = reqs[i];
-      if (q.rec >= 0) {
+      if (q.ext != nullptr) {
+        recs[i] = q.ext;
+      } else if (q.rec >= 0) {
         recs[i] = records_.as<u64>() + static_cast<size_t>(q.rec) * rw_;
         f.recs.push_back(q.rec);
       } else {
@@ -366,22 +444,35 @@ class ServeLoop {
         ++heap;
       }
     }
+    auto on_heap = [](const ServeRequest& q) { return q.ext == nullptr && q.rec < 0; };
     if (heap > 0) {
       CopyInParallel(reqs.size(), heap * rw_ * 8, 16, [&](size_t lo, size_t hi) {
         for (size_t i = lo; i < hi; ++i) {
-          if (reqs[i].rec < 0) {
+          if (on_heap(reqs[i])) {
             std::memcpy(const_cast<u64*>(recs[i]), reqs[i].data.data(), rw_ * 8);
           }
         }
       });
       for (auto& q : reqs) {
-        if (q.rec < 0) GiveBuffer(std::move(q.data));
+        if (on_heap(q)) GiveBuffer(std::move(q.data));
       }
     }
     Error er = StageRecords(S_, slot, recs.data(), n);
     if (!er.ok()) return er;
-    er = RunBatch(S_, slot, n, true);
+    // A batch whose DB scan is MFMA-bound (a WideScan plan) gains nothing
+    // from sharing the GPU: it runs its kernels after the previous batch's,
+    // in launch order, while its own copies go ahead. Two such batches
+    // running together would finish together, and both slots would then
+    // stage their next batches with the GPU idle. Smaller batches, whose
+    // HBM-bound scans and latency-bound tails do gain, overlap freely.
+    const int nb = Bucket(n);
+    if (last_ != nullptr && last_ != &slot && PlanFor(S_, true, nb).wide) {
+      PIRG_HIP(hipStreamWaitEvent(slot.stream, last_->graph_done, 0));
+    }
+    er = RunBatch(S_, slot, nb, true);
     if (!er.ok()) return er;
+    PIRG_HIP(hipEventRecord(slot.graph_done, slot.stream));
+    last_ = &slot;
     er = EnqueueFetch(S_, slot, n);
     if (!er.ok()) return er;
     PIRG_HIP(hipEventRecord(slot.done, slot.stream));
@@ -454,6 +545,7 @@ class ServeLoop {
   long long max_outstanding_;
   std::vector<std::unique_ptr<Slot>> slots_;
   std::deque<Slot*> free_;  // guarded by mu_
+  Slot* last_ = nullptr;    // the slot of the latest launched batch (worker only)
   std::mutex mu_;
   std::condition_variable cv_work_, cv_done_;
   std::deque<ServeRequest> pending_;
@@ -551,6 +643,64 @@ int pirg_serve_submit_batch(pirg_server* S, int n, const pirg_request_view* reqs
       break;
     }
   }
+  return 0;
+}
+
+int pirg_host_alloc(size_t bytes, void** out, char* err, size_t errlen) {
+  if (out == nullptr || bytes == 0) return Report(Invalid("bytes must be positive and out non-null."), err, errlen);
+  *out = nullptr;
+  void* p = nullptr;
+  const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess || p == nullptr) {
+    return Report(Error{PIRG_RESOURCE_EXHAUSTED, std::string("hipHostMalloc of ") +
+                                                     std::to_string(bytes) + " bytes: " +
+                                                     hipGetErrorString(e)},
+                  err, errlen);
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_host_mu);
+    g_host_blocks[reinterpret_cast<uintptr_t>(p)] = bytes;
+  }
+  *out = p;
+  return 0;
+}
+
+void pirg_host_free(void* p) {
+  if (p == nullptr) return;
+  {
+    std::lock_guard<std::mutex> lock(g_host_mu);
+    auto it = g_host_blocks.find(reinterpret_cast<uintptr_t>(p));
+    if (it == g_host_blocks.end()) return;
+    g_host_blocks.erase(it);
+  }
+  (void)hipHostFree(p);
+}
+
+size_t pirg_request_record_words(const pirg_server* S) {
+  return S == nullptr ? 0 : RequestWords(S);
+}
+
+int pirg_request_record(const pirg_server* S, const pirg_request_view* req, uint64_t* record,
+                        char* err, size_t errlen) {
+  if (S == nullptr || req == nullptr || record == nullptr) {
+    return Report(Invalid("server, request and record must not be null."), err, errlen);
+  }
+  Error er = ValidateRequest(S, req->first_dim_len, req->second_dim_polys,
+                             req->packing_key_polys);
+  if (!er.ok()) return Report(er, err, errlen);
+  WriteRecord(S, *req, record);
+  return 0;
+}
+
+int pirg_serve_submit_records(pirg_server* S, int n, const uint64_t* const* records,
+                              const uint64_t* tags, char* err, size_t errlen) {
+  if (S == nullptr || n < 0 || (n > 0 && (records == nullptr || tags == nullptr))) {
+    return Report(Invalid("invalid server, count, records or tags."), err, errlen);
+  }
+  const std::shared_ptr<ServeLoop> loop = LoopOf(S);
+  if (!loop) return Report(Error{PIRG_FAILED_PRECONDITION, "the server is not serving"}, err, errlen);
+  Error er = loop->SubmitRecords(n, records, tags);
+  if (!er.ok()) return Report(er, err, errlen);
   return 0;
 }
 

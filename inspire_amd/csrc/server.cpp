@@ -76,6 +76,11 @@ Slot::~Slot() {
     if (x != nullptr) (void)hipEventDestroy(x);
   }
   if (done != nullptr) (void)hipEventDestroy(done);
+  if (graph_done != nullptr) (void)hipEventDestroy(graph_done);
+  if (side != nullptr) (void)hipStreamSynchronize(side);
+  if (fork_ev != nullptr) (void)hipEventDestroy(fork_ev);
+  if (join_ev != nullptr) (void)hipEventDestroy(join_ev);
+  if (side != nullptr) (void)hipStreamDestroy(side);
   if (stream != nullptr) (void)hipStreamDestroy(stream);
 }
 
@@ -85,6 +90,10 @@ Error Slot::Init() {
   for (auto& x : ev) PIRG_HIP(hipEventCreate(&x));
   for (auto& x : kev) PIRG_HIP(hipEventCreate(&x));
   PIRG_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  PIRG_HIP(hipEventCreateWithFlags(&graph_done, hipEventDisableTiming));
+  PIRG_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  PIRG_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+  PIRG_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
   return Ok();
 }
 
@@ -515,6 +524,9 @@ Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap) {
     Error er = bufs[i]->alloc(sizes[i]);
     if (!er.ok()) return er;
   }
+  // Zeroed once: a batch padded past its requests (the serving loop rounds
+  // batch sizes up) then reads zero records, or earlier valid ones.
+  PIRG_HIP(FillSync(W.req.p, 0, sizes[0], slot.stream));
   Error er = W.h_req.alloc(nq * RequestWords(S) * 8);
   if (!er.ok()) return er;
   er = W.h_out.alloc(nq * S->ell * 2 * d * 8);
@@ -654,9 +666,9 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   // Scan brackets: eager runs record them directly; under capture with
   // graph_scan_events they become event-record nodes of the graph.
   const bool scan_nodes = !timed && slot.graph_scan_events;
-  auto kmark = [&](hipEvent_t e) -> hipError_t {
-    if (timed) return hipEventRecord(e, st);
-    return scan_nodes ? hipEventRecordWithFlags(e, st, hipEventRecordExternal) : hipSuccess;
+  auto kmark = [&](hipEvent_t e, hipStream_t s) -> hipError_t {
+    if (timed) return hipEventRecord(e, s);
+    return scan_nodes ? hipEventRecordWithFlags(e, s, hipEventRecordExternal) : hipSuccess;
   };
   PIRG_HIP(mark(slot.ev[0]));
   // DB product (pir_server.cc:215-229).
@@ -667,36 +679,80 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   const u64* rec_fd = W.req.as<u64>();
   const u64* rec_sd = W.req.as<u64>() + RecordSdOffset(S);
   const u64* rec_pk = W.req.as<u64>() + RecordPkOffset(S);
-  PIRG_HIP(LaunchQueryDigits(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
-  PIRG_HIP(LaunchVectorSums(rec_fd, nq, rw, S->K, S->dbL.c_off,
-                            W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
-  PIRG_HIP(kmark(slot.kev[0]));
-  PIRG_HIP(LaunchScan(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
-                      W.db_partial.as<u64>(), st));
-  PIRG_HIP(kmark(slot.kev[1]));
-  PIRG_HIP(mark(slot.ev[1]));
-  // Pack product (pir_server.cc:237-248): y = concat(y_vec_g).
-  PIRG_HIP(LaunchQueryDigits(rec_pk, rw, S->k * d, S->pkL, pp, W.bt_pk.as<int8_t>(), st));
-  PIRG_HIP(LaunchVectorSums(rec_pk, nq, rw, S->k * d, S->pkL.c_off,
-                            W.corr_pk.as<u64>() + nq, W.corr_pk.as<u64>(), st));
-  PIRG_HIP(kmark(slot.kev[2]));
-  PIRG_HIP(LaunchScan(S->pk_tiles.as<int8_t>(), S->pkL, pp, W.bt_pk.as<int8_t>(),
-                      W.pk_partial.as<u64>(), st));
-  PIRG_HIP(kmark(slot.kev[3]));
-  // FinalizeMatrixPack for every (shard, i) (pir_server.cc:256-279). A
-  // local-finalize server (vPIR, vpir_server.cc:148-160) adds no inner product.
-  if (!S->local_finalize) {
+  // Pack product (pir_server.cc:237-248), y = concat(y_vec_g): its query
+  // digits, then the scan.
+  auto pack_digits = [&](hipStream_t s) -> Error {
+    PIRG_HIP(LaunchQueryDigits(rec_pk, rw, S->k * d, S->pkL, pp, W.bt_pk.as<int8_t>(), s));
+    PIRG_HIP(LaunchVectorSums(rec_pk, nq, rw, S->k * d, S->pkL.c_off,
+                              W.corr_pk.as<u64>() + nq, W.corr_pk.as<u64>(), s));
+    return Ok();
+  };
+  auto pack_scan = [&]() -> Error {
+    PIRG_HIP(kmark(slot.kev[2], st));
+    PIRG_HIP(LaunchScan(S->pk_tiles.as<int8_t>(), S->pkL, pp, W.bt_pk.as<int8_t>(),
+                        W.pk_partial.as<u64>(), st));
+    PIRG_HIP(kmark(slot.kev[3], st));
+    return Ok();
+  };
+  // FinalizeMatrixPack's inner products for every (shard, i)
+  // (pir_server.cc:256-279), from the request's y_vec_h alone. A
+  // local-finalize server (vPIR, vpir_server.cc:148-160) adds none.
+  auto finalize_ip = [&](hipStream_t s) -> Error {
+    if (S->local_finalize) return Ok();
     PIRG_HIP(LaunchSpectra(rec_pk + static_cast<size_t>(S->k) * d, nq * S->k, S->k, rw,
                            S->fin_v, S->F, S->tw_fwd.as<double2>(), W.yh_spec.as<double2>(),
-                           st));
+                           s));
     const size_t n_fin_cap = static_cast<size_t>(W.cap) * lt;
     void* const* tab = W.fin_tab.as<void* const>();
     PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(tab),
                       reinterpret_cast<const double2* const*>(tab + n_fin_cap),
                       reinterpret_cast<u64* const*>(tab + 2 * n_fin_cap),
                       nq * lt, lt, S->k, S->fin_u.num_chunks, S->fin_v.num_chunks,
-                      S->fin_pairs, S->F, S->tw_bwd.as<double2>(), margin, st,
+                      S->fin_pairs, S->F, S->tw_bwd.as<double2>(), margin, s,
                       W.mac_acc.as<double2>()));
+    return Ok();
+  };
+  // EvalPoly's RGSW spectra (pir_server.cc:126-150), from the request alone.
+  auto second_dim_spectra = [&](hipStream_t s) -> Error {
+    PIRG_HIP(LaunchSpectra(rec_sd, nq * 2 * S->e, 2 * S->e, rw, S->ext_v, S->F,
+                           S->tw_fwd.as<double2>(), W.sdb_spec.as<double2>(), s));
+    return Ok();
+  };
+  // While the DB scan is bound by HBM (a NarrowScan plan), the
+  // latency-bound kernels that need only the request (the packing key's
+  // digits, FinalizeMatrixPack's inner products, the second dimension's
+  // spectra) run beside it on the side stream, forked from and joined back
+  // into the slot's stream; the pack scan, bound by HBM too, follows the DB
+  // scan. An MFMA-bound scan keeps every CU busy: then all of it stays in line.
+  const bool fork = !pd.wide && S->fork_branches;
+  if (fork) {
+    PIRG_HIP(hipEventRecord(slot.fork_ev, st));
+    PIRG_HIP(hipStreamWaitEvent(slot.side, slot.fork_ev, 0));
+    Error er = pack_digits(slot.side);
+    if (er.ok()) er = finalize_ip(slot.side);
+    if (er.ok() && S->t > 1) er = second_dim_spectra(slot.side);
+    if (!er.ok()) return er;
+    PIRG_HIP(hipEventRecord(slot.join_ev, slot.side));
+  }
+  PIRG_HIP(LaunchQueryDigits(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
+  PIRG_HIP(LaunchVectorSums(rec_fd, nq, rw, S->K, S->dbL.c_off,
+                            W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
+  PIRG_HIP(kmark(slot.kev[0], st));
+  PIRG_HIP(LaunchScan(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
+                      W.db_partial.as<u64>(), st));
+  PIRG_HIP(kmark(slot.kev[1], st));
+  PIRG_HIP(mark(slot.ev[1]));
+  {
+    Error er;
+    if (fork) {
+      PIRG_HIP(hipStreamWaitEvent(st, slot.join_ev, 0));
+      er = pack_scan();
+    } else {
+      er = pack_digits(st);
+      if (er.ok()) er = pack_scan();
+      if (er.ok()) er = finalize_ip(st);
+    }
+    if (!er.ok()) return er;
   }
   FinalizeArgs FA{};
   FA.db_partial = W.db_partial.as<u64>();
@@ -725,8 +781,10 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
   PIRG_HIP(mark(slot.ev[2]));
   // EvalPoly (pir_server.cc:126-150) and the modulus switch.
   if (S->t > 1) {
-    PIRG_HIP(LaunchSpectra(rec_sd, nq * 2 * S->e, 2 * S->e, rw, S->ext_v, S->F,
-                           S->tw_fwd.as<double2>(), W.sdb_spec.as<double2>(), st));
+    if (!fork) {
+      Error er = second_dim_spectra(st);
+      if (!er.ok()) return er;
+    }
     const size_t n_ext_cap = static_cast<size_t>(W.cap) * S->ell * 2;
     void* const* et = W.ext_tab.as<void* const>();
     for (int i = S->t - 2; i >= 0; --i) {
@@ -783,6 +841,26 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
 
 }  // namespace
 
+Error PrepareGraph(pirg_server* S, Slot& slot, int nq) {
+  if (nq <= 0 || nq > slot.ws.cap) return Invalid("batch size exceeds the slot's workspace.");
+  if (slot.graphs.count(nq) != 0) return Ok();
+  PIRG_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal));
+  Error er = EnqueueBatch(S, slot, nq, false);
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(slot.stream, &graph);
+  if (!er.ok()) {
+    if (graph != nullptr) (void)hipGraphDestroy(graph);
+    return er;
+  }
+  PIRG_HIP(ce);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  PIRG_HIP(ie);
+  slot.graphs.emplace(nq, exec);
+  return Ok();
+}
+
 Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph) {
   if (nq <= 0 || nq > slot.ws.cap) return Invalid("batch size exceeds the slot's workspace.");
   if (!use_graph) {
@@ -792,23 +870,9 @@ Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph) {
     slot.graph_run = false;
     return Ok();
   }
+  Error pe = PrepareGraph(S, slot, nq);
+  if (!pe.ok()) return pe;
   auto it = slot.graphs.find(nq);
-  if (it == slot.graphs.end()) {
-    PIRG_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal));
-    Error er = EnqueueBatch(S, slot, nq, false);
-    hipGraph_t graph = nullptr;
-    const hipError_t ce = hipStreamEndCapture(slot.stream, &graph);
-    if (!er.ok()) {
-      if (graph != nullptr) (void)hipGraphDestroy(graph);
-      return er;
-    }
-    PIRG_HIP(ce);
-    hipGraphExec_t exec = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    PIRG_HIP(ie);
-    it = slot.graphs.emplace(nq, exec).first;
-  }
   PIRG_HIP(hipEventRecord(slot.ev[0], slot.stream));
   PIRG_HIP(hipGraphLaunch(it->second, slot.stream));
   PIRG_HIP(hipEventRecord(slot.ev[3], slot.stream));

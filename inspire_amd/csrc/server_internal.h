@@ -124,6 +124,12 @@ struct Slot {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // stage boundaries
   hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};  // DB / pack scans
   hipEvent_t done = nullptr;                                 // serving: batch finished
+  hipEvent_t graph_done = nullptr;  // serving: the batch's kernels finished
+  // The pack branch of a small batch (packing key digits, pack scan,
+  // FinalizeMatrixPack's spectra, second-dimension spectra) runs on `side`
+  // beside the DB scan, forked from and joined back into `stream`.
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   bool timing_pending = false;
   bool graph_run = false;  // the pending timing came from a graph launch
   // Graphs captured on this slot bracket the two scans with event-record
@@ -158,6 +164,8 @@ struct pirg_server {
   pirg::LimbLayout dbL{}, pkL{};
   // Batches of at least this many requests use the MFMA-bound WideScan.
   int wide_min = 10;
+  // Small batches run their pack branch beside the DB scan.
+  bool fork_branches = true;
   // Default plans may use the digit-major WideScan layout, and for two-plane
   // matrices its tall kernel.
   bool digit_major = true;
@@ -220,6 +228,9 @@ Error StageRequests(pirg_server* S, Slot& slot, const pirg_request_view* reqs,
 // eagerly (with per-stage timing) or as a replay of the slot's hipGraph for
 // nq (captured on first use).
 Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph);
+// Captures and instantiates the slot's hipGraph for nq requests, if it has
+// none, without running it.
+Error PrepareGraph(pirg_server* S, Slot& slot, int nq);
 // Enqueues the device-to-host copy of nq responses into the slot's pinned
 // output buffer.
 Error EnqueueFetch(pirg_server* S, Slot& slot, int nq);

@@ -343,6 +343,23 @@ def test_entry_points_reject_null_and_negative_arguments():
     assert b"log_degree" in err.value
 
 
+def test_record_submission_entry_points_reject_bad_arguments():
+    lib = ia.lib()
+    err = ctypes.create_string_buffer(256)
+    n = ctypes.c_size_t(256)
+    p = ctypes.c_void_p()
+    assert lib.pirg_host_alloc(0, ctypes.byref(p), err, n) == ia.INVALID_ARGUMENT
+    assert lib.pirg_host_alloc(64, None, err, n) == ia.INVALID_ARGUMENT
+    lib.pirg_host_free(None)
+    # A pointer pirg_host_alloc did not return is ignored, not freed.
+    junk = ctypes.create_string_buffer(64)
+    lib.pirg_host_free(ctypes.cast(junk, ctypes.c_void_p))
+    assert lib.pirg_request_record_words(None) == 0
+    assert lib.pirg_request_record(None, None, None, err, n) == ia.INVALID_ARGUMENT
+    assert lib.pirg_serve_submit_records(None, 1, None, None, err, n) == ia.INVALID_ARGUMENT
+    assert lib.pirg_serve_submit_records(None, -1, None, None, err, n) == ia.INVALID_ARGUMENT
+
+
 def test_load_from_file_and_save_errors(tmp_path):
     # LoadFromFile's read failure (pir_server.cc:799-801) and SaveToFile's
     # open failure (:437-440), before any device work.

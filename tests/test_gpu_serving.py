@@ -308,3 +308,56 @@ def test_batched_submit_and_poll(material, requests):
         want, _, _ = O.process_response(P, material, requests[i])
         np.testing.assert_array_equal(got[i], want)
 
+
+
+def test_zero_copy_records(material, requests):
+    # pirg_serve_submit_records: records written by pirg_request_record into
+    # pinned memory from pirg_host_alloc are answered as the same requests
+    # submitted by copy (and as the oracle), mixed in one loop; records
+    # outside that memory, or running past its end, are refused whole.
+    reqs = [gpu_request(r) for r in requests[:12]]
+    with server_from_material(P, material) as srv:
+        alone = [srv.process_response(r).ciphertexts for r in reqs]
+        words = srv.record_words()
+        with ia.HostRecords(srv, 12) as hr:
+            assert hr.array.shape == (12, words)
+            for i, r in enumerate(reqs):
+                srv.write_record(r, hr.array[i])
+            bad = O.Request(requests[0].first_dim, requests[0].second_dim,
+                            requests[0].packing_key[:-1])
+            with pytest.raises(ia.PirError) as ei:
+                srv.write_record(gpu_request(bad), hr.array[0].copy())
+            assert ei.value.code == ia.INVALID_ARGUMENT
+            srv.serve_start(max_batch=8, slots=2, max_outstanding=64)
+            try:
+                outside = np.zeros(words, dtype=np.uint64)
+                with pytest.raises(ia.PirError) as ei:
+                    srv.submit_records([hr.array[0], outside], [0, 1])
+                assert ei.value.code == ia.INVALID_ARGUMENT
+                # A record starting inside the block but ending past it.
+                flat = hr.array.reshape(-1)
+                with pytest.raises(ia.PirError) as ei:
+                    srv.submit_records([flat[11 * words + 1:]], [0])
+                assert ei.value.code == ia.INVALID_ARGUMENT
+                # Records (adjacent and not, each twice) and copied requests.
+                order = [0, 1, 2, 3, 7, 5, 11, 10, 9, 4, 6, 8]
+                srv.submit_records([hr.array[i] for i in order], order)
+                srv.submit_batch(reqs[:4], [100, 101, 102, 103])
+                srv.submit_records([hr.array[i] for i in order[::-1]],
+                                   [200 + i for i in order[::-1]])
+                got = {}
+                while len(got) < 28:
+                    r = srv.poll_batch(16, 30000)
+                    assert r is not None
+                    for tg, o, s_ in zip(r[0], r[1], r[2]):
+                        assert s_ == 0
+                        got[tg] = o.copy()
+            finally:
+                srv.serve_stop()
+    for i in range(12):
+        np.testing.assert_array_equal(got[i], alone[i])
+        np.testing.assert_array_equal(got[200 + i], alone[i])
+    for i in range(4):
+        np.testing.assert_array_equal(got[100 + i], alone[i])
+    want, _, _ = O.process_response(P, material, requests[5])
+    np.testing.assert_array_equal(got[5], want)
