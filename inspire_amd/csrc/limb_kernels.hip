@@ -1087,6 +1087,35 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
   return hipGetLastError();
 }
 
+namespace {
+
+// 16 bytes per lane and step; words an odd count leaves to lane 0.
+__global__ __launch_bounds__(256) void CopyWordsKernel(const u64* __restrict__ src,
+                                                       u64* __restrict__ dst, long long pairs,
+                                                       long long words) {
+  const long long stride = static_cast<long long>(gridDim.x) * 256;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < pairs; i += stride) {
+    const ulonglong2 v = reinterpret_cast<const ulonglong2*>(src)[i];
+    reinterpret_cast<ulonglong2*>(dst)[i] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (words & 1)) dst[words - 1] = src[words - 1];
+}
+
+}  // namespace
+
+hipError_t LaunchCopyWords(const u64* src, u64* dst, size_t words, hipStream_t s) {
+  if (words == 0) return hipSuccess;
+  // Both ends 16-byte aligned (allocations and record strides are).
+  if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) {
+    return hipErrorInvalidValue;
+  }
+  const long long pairs = static_cast<long long>(words / 2);
+  const int blocks = static_cast<int>(std::min<long long>(1024, std::max<long long>(1, (pairs + 255) / 256)));
+  hipLaunchKernelGGL(CopyWordsKernel, dim3(blocks), dim3(256), 0, s, src, dst, pairs,
+                     static_cast<long long>(words));
+  return hipGetLastError();
+}
+
 hipError_t LaunchTilesToValues(const int8_t* tiles, const LimbLayout& L, int row0, int nrows,
                               bool signed_top, u64 mask, u64* out, hipStream_t s) {
   if (nrows <= 0) return hipSuccess;

@@ -119,6 +119,10 @@ hipError_t LaunchPackToTiles(const void* pack, int mat_bytes, int rows,
 // sign-extended when signed_top, as for pack matrices; ANDed with mask).
 hipError_t LaunchTilesToValues(const int8_t* tiles, const LimbLayout& L, int row0, int nrows,
                               bool signed_top, u64 mask, u64* out, hipStream_t s);
+// Copies `words` u64 from src to dst with a kernel: either side may be
+// pinned host memory, so a small transfer runs in line with the kernels
+// around it instead of on a copy engine that must synchronise with them.
+hipError_t LaunchCopyWords(const u64* src, u64* dst, size_t words, hipStream_t s);
 // How one limb product over nq query vectors is launched. Columns of the
 // int8 product are (query, shift s) pairs, s < ns = ceil(out_bits / 8): column
 // (q, s) collects plane p times digit s - p of query q. Queries are grouped,

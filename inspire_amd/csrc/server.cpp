@@ -7,13 +7,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -57,7 +61,11 @@ void HostBuf::reset() {
 Error HostBuf::alloc(size_t n) {
   reset();
   if (n == 0) n = 8;
-  hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+  // Coherent: kernels that read the buffer (host_io batches) see what the
+  // host wrote before the launch, never a line cached by an earlier read,
+  // and their writes reach host memory directly. Copy engines and the host
+  // are unaffected.
+  hipError_t e = hipHostMalloc(&p, n, hipHostMallocCoherent);
   if (e != hipSuccess) {
     p = nullptr;
     return Error{PIRG_RESOURCE_EXHAUSTED, "pinned host allocation failed"};
@@ -610,37 +618,155 @@ Error StageRecords(pirg_server* S, Slot& slot, const u64* const* records, int nq
   return Ok();
 }
 
+namespace {
+
+// Words [lo, hi) of request R's record, written to rec + lo.
+void WriteRecordPart(const pirg_server* S, const pirg_request_view& R, u64* rec, size_t lo,
+                     size_t hi) {
+  struct Seg {
+    const u64* src;  // nullptr: zeros
+    size_t len;
+  };
+  const size_t d = S->d;
+  const size_t sd_len = S->t > 1 ? static_cast<size_t>(2 * S->e) * d : 0;
+  const Seg segs[4] = {{R.first_dim, R.first_dim_len},
+                       {nullptr, static_cast<size_t>(S->K) - R.first_dim_len},
+                       {R.second_dim, sd_len},
+                       {R.packing_key, static_cast<size_t>(2 * S->k) * d}};
+  size_t off = 0;
+  for (const Seg& g : segs) {
+    const size_t a = std::max(lo, off), b = std::min(hi, off + g.len);
+    if (a < b) {
+      if (g.src != nullptr) {
+        std::memcpy(rec + a, g.src + (a - off), (b - a) * 8);
+      } else {
+        std::memset(rec + a, 0, (b - a) * 8);
+      }
+    }
+    off += g.len;
+  }
+}
+
+// Worker threads for host copies into pinned buffers, started once per
+// process: one thread copies about 10 GB/s from memory the caches do not
+// hold, so a single 1.4 MB request takes over 100 us alone, and a thread
+// started per call would cost as much as it saves.
+class HostCopyPool {
+ public:
+  static HostCopyPool& Get() {
+    static HostCopyPool pool;
+    return pool;
+  }
+
+  // Runs fn(i) for every i < n on the pool's threads and the caller, and
+  // returns when all have run. Concurrent calls share the threads.
+  void Run(size_t n, const std::function<void(size_t)>& fn) {
+    auto job = std::make_shared<Job>();
+    job->fn = &fn;
+    job->n = n;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      job_ = job;
+      ++gen_;
+    }
+    cv_.notify_all();
+    Work(*job);
+    while (job->done.load(std::memory_order_acquire) < n) std::this_thread::yield();
+    std::lock_guard<std::mutex> lock(mu_);
+    if (job_ == job) job_.reset();
+  }
+
+ private:
+  // A worker that takes a job after its last index was claimed runs nothing,
+  // so fn (the caller's) is called only while Run waits.
+  struct Job {
+    const std::function<void(size_t)>* fn = nullptr;
+    size_t n = 0;
+    std::atomic<size_t> next{0}, done{0};
+  };
+
+  HostCopyPool() {
+    const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
+    const unsigned workers = std::min(7u, hw - 1);
+    for (unsigned i = 0; i < workers; ++i) threads_.emplace_back([this] { Loop(); });
+  }
+  ~HostCopyPool() {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      quit_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+
+  static void Work(Job& j) {
+    for (;;) {
+      const size_t i = j.next.fetch_add(1, std::memory_order_relaxed);
+      if (i >= j.n) return;
+      (*j.fn)(i);
+      j.done.fetch_add(1, std::memory_order_release);
+    }
+  }
+
+  void Loop() {
+    unsigned long long seen = 0;
+    for (;;) {
+      std::shared_ptr<Job> j;
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_.wait(lock, [&] { return quit_ || gen_ != seen; });
+        if (quit_) return;
+        seen = gen_;
+        j = job_;
+      }
+      if (j) Work(*j);
+    }
+  }
+
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::shared_ptr<Job> job_;
+  unsigned long long gen_ = 0;
+  bool quit_ = false;
+  std::vector<std::thread> threads_;
+};
+
+}  // namespace
+
+void WriteRecords(pirg_server* S, Slot& slot, const pirg_request_view* reqs,
+                  const std::vector<int>& idx) {
+  const size_t nq = idx.size();
+  const size_t rw = RequestWords(S);
+  u64* h = slot.ws.h_req.as<u64>();
+  const size_t total = nq * rw;
+  // 128 KB pieces, each within one record or across two, on the copy pool
+  // from 256 KB up.
+  constexpr size_t kPiece = size_t(16) << 10;
+  auto piece = [&](size_t c) {
+    const size_t lo = c * kPiece, hi = std::min(total, lo + kPiece);
+    for (size_t w = lo; w < hi;) {
+      const size_t q = w / rw, r0 = w - q * rw;
+      const size_t r1 = std::min(rw, r0 + (hi - w));
+      WriteRecordPart(S, reqs[idx[q]], h + q * rw, r0, r1);
+      w += r1 - r0;
+    }
+  };
+  const size_t pieces = (total + kPiece - 1) / kPiece;
+  if (total * 8 < (size_t(256) << 10)) {
+    for (size_t c = 0; c < pieces; ++c) piece(c);
+  } else {
+    HostCopyPool::Get().Run(pieces, piece);
+  }
+}
+
 // Copies validated requests (reqs[idx[i]]) as records into the pinned
 // staging buffer and on to HBM.
 Error StageRequests(pirg_server* S, Slot& slot, const pirg_request_view* reqs,
                     const std::vector<int>& idx) {
-  Workspace& W = slot.ws;
-  const int nq = static_cast<int>(idx.size());
-  const size_t rw = RequestWords(S);
-  u64* h = W.h_req.as<u64>();
-  auto stage = [&](int lo, int hi) {
-    for (int i = lo; i < hi; ++i) WriteRecord(S, reqs[idx[i]], h + static_cast<size_t>(i) * rw);
-  };
-  // A large batch (hundreds of MB at n = 2^20) is copied into the pinned
-  // buffer by several threads: one thread's memcpy would take longer than
-  // the batch's GPU work.
-  const size_t total = rw * 8 * static_cast<size_t>(nq);
-  const int threads = static_cast<int>(std::min<size_t>(
-      {static_cast<size_t>(std::max(1u, std::thread::hardware_concurrency())), 16,
-       total / (size_t(8) << 20) + 1, static_cast<size_t>(nq)}));
-  if (threads <= 1) {
-    stage(0, nq);
-  } else {
-    std::vector<std::thread> pool;
-    const int per = (nq + threads - 1) / threads;
-    for (int t = 1; t < threads; ++t) {
-      const int lo = std::min(nq, t * per), hi = std::min(nq, lo + per);
-      if (lo < hi) pool.emplace_back(stage, lo, hi);
-    }
-    stage(0, std::min(nq, per));
-    for (auto& th : pool) th.join();
-  }
-  PIRG_HIP(hipMemcpyAsync(W.req.p, h, total, hipMemcpyHostToDevice, slot.stream));
+  WriteRecords(S, slot, reqs, idx);
+  const size_t total = RequestWords(S) * 8 * idx.size();
+  PIRG_HIP(hipMemcpyAsync(slot.ws.req.p, slot.ws.h_req.p, total, hipMemcpyHostToDevice,
+                          slot.stream));
   return Ok();
 }
 
@@ -649,7 +775,7 @@ namespace {
 // ProcessResponse for nq staged requests (pir_server.cc:207-305), enqueued on
 // the slot's stream. `timed` records the stage and scan-kernel events (it is
 // off while the sequence is captured into a graph).
-Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
+Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io) {
   Workspace& W = slot.ws;
   hipStream_t st = slot.stream;
   const int d = S->d;
@@ -671,6 +797,10 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
     return scan_nodes ? hipEventRecordWithFlags(e, s, hipEventRecordExternal) : hipSuccess;
   };
   PIRG_HIP(mark(slot.ev[0]));
+  if (host_io) {
+    PIRG_HIP(LaunchCopyWords(W.h_req.as<u64>(), W.req.as<u64>(),
+                             static_cast<size_t>(nq) * RequestWords(S), st));
+  }
   // DB product (pir_server.cc:215-229).
   const ScanPlan pd = PlanFor(S, true, nq);
   const ScanPlan pp = PlanFor(S, false, nq);
@@ -835,17 +965,24 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed) {
       if (i == 0) PIRG_HIP(LaunchExtCombine(EA, st));
     }
   }
+  if (host_io) {
+    PIRG_HIP(LaunchCopyWords(W.out.as<u64>(), W.h_out.as<u64>(),
+                             static_cast<size_t>(nq) * ResponseWords(S), st));
+  }
   PIRG_HIP(mark(slot.ev[3]));
   return Ok();
 }
 
 }  // namespace
 
-Error PrepareGraph(pirg_server* S, Slot& slot, int nq) {
+// Graphs of a slot, by batch size and whether they move their own host I/O.
+int GraphKey(int nq, bool host_io) { return 2 * nq + (host_io ? 1 : 0); }
+
+Error PrepareGraph(pirg_server* S, Slot& slot, int nq, bool host_io) {
   if (nq <= 0 || nq > slot.ws.cap) return Invalid("batch size exceeds the slot's workspace.");
-  if (slot.graphs.count(nq) != 0) return Ok();
+  if (slot.graphs.count(GraphKey(nq, host_io)) != 0) return Ok();
   PIRG_HIP(hipStreamBeginCapture(slot.stream, hipStreamCaptureModeThreadLocal));
-  Error er = EnqueueBatch(S, slot, nq, false);
+  Error er = EnqueueBatch(S, slot, nq, false, host_io);
   hipGraph_t graph = nullptr;
   const hipError_t ce = hipStreamEndCapture(slot.stream, &graph);
   if (!er.ok()) {
@@ -857,22 +994,22 @@ Error PrepareGraph(pirg_server* S, Slot& slot, int nq) {
   const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   PIRG_HIP(ie);
-  slot.graphs.emplace(nq, exec);
+  slot.graphs.emplace(GraphKey(nq, host_io), exec);
   return Ok();
 }
 
-Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph) {
+Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph, bool host_io) {
   if (nq <= 0 || nq > slot.ws.cap) return Invalid("batch size exceeds the slot's workspace.");
   if (!use_graph) {
-    Error er = EnqueueBatch(S, slot, nq, true);
+    Error er = EnqueueBatch(S, slot, nq, true, host_io);
     if (!er.ok()) return er;
     slot.timing_pending = true;
     slot.graph_run = false;
     return Ok();
   }
-  Error pe = PrepareGraph(S, slot, nq);
+  Error pe = PrepareGraph(S, slot, nq, host_io);
   if (!pe.ok()) return pe;
-  auto it = slot.graphs.find(nq);
+  auto it = slot.graphs.find(GraphKey(nq, host_io));
   PIRG_HIP(hipEventRecord(slot.ev[0], slot.stream));
   PIRG_HIP(hipGraphLaunch(it->second, slot.stream));
   PIRG_HIP(hipEventRecord(slot.ev[3], slot.stream));
@@ -935,10 +1072,15 @@ Error EnqueueFetch(pirg_server* S, Slot& slot, int nq) {
 
 namespace {
 
+// Waits for the batch and copies its responses out of the pinned output
+// buffer; `copy` first enqueues the device-to-host copy (a host_io batch
+// has written the buffer itself).
 Error FetchOutputs(pirg_server* S, Slot& slot, int nq, uint64_t* const* outs,
-                   const std::vector<int>* idx) {
-  Error er = EnqueueFetch(S, slot, nq);
-  if (!er.ok()) return er;
+                   const std::vector<int>* idx, bool copy = true) {
+  if (copy) {
+    Error er = EnqueueFetch(S, slot, nq);
+    if (!er.ok()) return er;
+  }
   PIRG_HIP(hipStreamSynchronize(slot.stream));
   CollectTiming(S, slot);
   const size_t per = ResponseWords(S);
@@ -1396,11 +1538,18 @@ int pirg_process_batch(pirg_server* S, int nreq, const pirg_request_view* reqs,
   const int cap = static_cast<int>(idx.size());
   Error er = EnsureWorkspace(S, S->main, cap);
   if (!er.ok()) return Report(er, err, errlen);
-  er = StageRequests(S, S->main, reqs, idx);
+  // A small batch moves its records in and its responses out with kernels
+  // in line with its own: no copy-engine transfers to wait on either side.
+  const bool host_io = cap <= kHostIoMax;
+  if (host_io) {
+    WriteRecords(S, S->main, reqs, idx);
+  } else {
+    er = StageRequests(S, S->main, reqs, idx);
+    if (!er.ok()) return Report(er, err, errlen);
+  }
+  er = RunBatch(S, S->main, cap, S->use_graphs, host_io);
   if (!er.ok()) return Report(er, err, errlen);
-  er = RunBatch(S, S->main, cap, S->use_graphs);
-  if (!er.ok()) return Report(er, err, errlen);
-  er = FetchOutputs(S, S->main, cap, outs, &idx);
+  er = FetchOutputs(S, S->main, cap, outs, &idx, !host_io);
   if (!er.ok()) return Report(er, err, errlen);
   return 0;
 }

@@ -220,17 +220,25 @@ Error ValidateRequest(const pirg_server* S, size_t fd_len, size_t sd_polys, size
 // Bytes of device workspace a slot needs for `cap` requests.
 size_t WorkspaceBytes(const pirg_server* S, int cap);
 Error EnsureWorkspace(pirg_server* S, Slot& slot, int cap);
-// Copies requests reqs[idx[i]] into the slot's pinned buffers and enqueues
-// the host-to-device copies on its stream.
+// Writes requests reqs[idx[i]] as records into the slot's pinned request
+// buffer (several threads for large batches).
+void WriteRecords(pirg_server* S, Slot& slot, const pirg_request_view* reqs,
+                  const std::vector<int>& idx);
+// WriteRecords, then the host-to-device copy on the slot's stream.
 Error StageRequests(pirg_server* S, Slot& slot, const pirg_request_view* reqs,
                     const std::vector<int>& idx);
+// Batches of at most this many requests on the direct path move their
+// records in and responses out with kernels inside the batch (host_io).
+constexpr int kHostIoMax = 16;
 // Enqueues ProcessResponse for the nq staged requests on the slot's stream,
 // eagerly (with per-stage timing) or as a replay of the slot's hipGraph for
-// nq (captured on first use).
-Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph);
-// Captures and instantiates the slot's hipGraph for nq requests, if it has
-// none, without running it.
-Error PrepareGraph(pirg_server* S, Slot& slot, int nq);
+// nq (captured on first use). host_io: the batch itself copies the records
+// from the slot's pinned request buffer first and its responses into the
+// pinned output buffer last (WriteRecords before, no EnqueueFetch after).
+Error RunBatch(pirg_server* S, Slot& slot, int nq, bool use_graph, bool host_io = false);
+// Captures and instantiates the slot's hipGraph for nq requests (and
+// host_io), if it has none, without running it.
+Error PrepareGraph(pirg_server* S, Slot& slot, int nq, bool host_io = false);
 // Enqueues the device-to-host copy of nq responses into the slot's pinned
 // output buffer.
 Error EnqueueFetch(pirg_server* S, Slot& slot, int nq);
