@@ -16,9 +16,11 @@
  *
  * Threading: a pirg_server is externally synchronised (like the reference,
  * whose ProcessResponse is not safe to call concurrently), except for the
- * serving-loop calls pirg_serve_submit / pirg_serve_poll / pirg_serve_stats /
- * pirg_serve_stop, which are thread-safe and may run beside each other and
- * beside the other calls (pirg_server_destroy excepted).
+ * serving-loop calls pirg_serve_submit(_batch, _records) / pirg_serve_poll
+ * (_batch) / pirg_serve_stats / pirg_serve_stop, which are thread-safe and
+ * may run beside each other and beside the other calls (pirg_server_destroy
+ * excepted), and pirg_host_alloc / pirg_host_free / pirg_request_record,
+ * which are thread-safe.
  */
 #ifndef PIRGPU_H_
 #define PIRGPU_H_
@@ -184,11 +186,21 @@ int pirg_server_time_graph_scans(pirg_server* server, int enable);
  * request per call). The DB and preprocessed material stay resident; a
  * worker thread keeps `slots` execution slots busy, each answering up to
  * `max_batch` requests per graph launch, and every request waiting when a
- * slot frees joins that slot's next batch (continuous batching; the slots'
- * batches overlap on the GPU). Slots and batch capacity are reduced at start
- * to fit 80% of the free HBM; a submit is refused with
- * PIRG_RESOURCE_EXHAUSTED while max_outstanding requests are submitted but
- * not yet polled. */
+ * slot frees joins that slot's next batch (continuous batching). Batches run
+ * at fixed sizes (powers of two up to 16, then multiples of 16, padded),
+ * whose graphs are captured at start; a batch with an MFMA-bound DB scan
+ * runs its kernels after the previous batch's, its copies overlapping them,
+ * and smaller batches overlap freely. Slots and batch capacity are reduced
+ * at start to fit 80% of the free HBM, or the limit set with
+ * pirg_serve_set_memory_limit when lower; start fails with
+ * PIRG_RESOURCE_EXHAUSTED when not even one batch fits. A submit is refused
+ * with PIRG_RESOURCE_EXHAUSTED while max_outstanding requests are submitted
+ * but not yet polled. */
+/* The device memory a later pirg_serve_start may give its slots (0, the
+ * default: 80% of the free memory at start), and the bytes one slot takes
+ * for batches of `batch` requests. */
+int pirg_serve_set_memory_limit(pirg_server* server, size_t bytes);
+size_t pirg_serve_workspace_bytes(const pirg_server* server, int batch);
 int pirg_serve_start(pirg_server* server, int max_batch, int slots,
                      int max_outstanding, char* err, size_t errlen);
 /* Validates like ProcessResponse (same messages) and queues the request; the

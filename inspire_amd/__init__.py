@@ -69,6 +69,9 @@ def lib() -> ctypes.CDLL:
         _lib.pirg_host_free.restype = None
         _lib.pirg_request_record_words.argtypes = [ctypes.c_void_p]
         _lib.pirg_request_record_words.restype = ctypes.c_size_t
+        _lib.pirg_serve_set_memory_limit.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+        _lib.pirg_serve_workspace_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _lib.pirg_serve_workspace_bytes.restype = ctypes.c_size_t
     return _lib
 
 
@@ -558,6 +561,15 @@ class PirServer:
         rc = lib().pirg_serve_start(self._h, max_batch, slots, max_outstanding, err,
                                     ctypes.c_size_t(len(err)))
         _check(rc, err)
+
+    def serve_set_memory_limit(self, nbytes: int) -> None:
+        """Device memory serve_start may give its slots (0: 80% of the free
+        memory at start)."""
+        _check(lib().pirg_serve_set_memory_limit(self._h, int(nbytes)), _err())
+
+    def serve_workspace_bytes(self, batch: int) -> int:
+        """Device bytes one serving slot takes for batches of `batch`."""
+        return int(lib().pirg_serve_workspace_bytes(self._h, int(batch)))
 
     def submit(self, request: PirRequest, tag: int) -> None:
         """Queues a request (validated like ProcessResponse); raises PirError

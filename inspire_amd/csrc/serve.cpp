@@ -111,10 +111,14 @@ class ServeLoop {
 
   Error Start() {
     // Admission by free HBM: as many slots of max_batch requests as fit in
-    // 80% of the free device memory (fewer slots, then smaller batches).
+    // 80% of the free device memory, or in the server's serving memory
+    // limit when that is lower (fewer slots, then smaller batches).
     size_t free_b = 0, total_b = 0;
     PIRG_HIP(hipMemGetInfo(&free_b, &total_b));
-    const double budget = 0.8 * static_cast<double>(free_b);
+    double budget = 0.8 * static_cast<double>(free_b);
+    if (S_->serve_mem_limit > 0) {
+      budget = std::min(budget, static_cast<double>(S_->serve_mem_limit));
+    }
     while (max_batch_ > 1 &&
            static_cast<double>(WorkspaceBytes(S_, max_batch_)) * nslots_ > budget) {
       if (nslots_ > 2) {
@@ -124,7 +128,11 @@ class ServeLoop {
       }
     }
     if (static_cast<double>(WorkspaceBytes(S_, max_batch_)) * nslots_ > budget) {
-      return Error{PIRG_RESOURCE_EXHAUSTED, "not enough free device memory for a serving slot"};
+      return Error{PIRG_RESOURCE_EXHAUSTED,
+                   "not enough free device memory for a serving slot (" +
+                       std::to_string(nslots_) + " x " +
+                       std::to_string(WorkspaceBytes(S_, max_batch_)) + " bytes, budget " +
+                       std::to_string(static_cast<unsigned long long>(budget)) + ")"};
     }
     // The pinned record pool: submits write requests here and batches copy
     // them to HBM from here, so a request crosses host memory once. A record
@@ -644,6 +652,18 @@ int pirg_serve_submit_batch(pirg_server* S, int n, const pirg_request_view* reqs
     }
   }
   return 0;
+}
+
+int pirg_serve_set_memory_limit(pirg_server* S, size_t bytes) {
+  if (S == nullptr) return PIRG_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(S->serve_mu);
+  S->serve_mem_limit = bytes;
+  return 0;
+}
+
+size_t pirg_serve_workspace_bytes(const pirg_server* S, int batch) {
+  if (S == nullptr || batch < 1) return 0;
+  return WorkspaceBytes(S, batch);
 }
 
 int pirg_host_alloc(size_t bytes, void** out, char* err, size_t errlen) {

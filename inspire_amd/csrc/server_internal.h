@@ -164,6 +164,9 @@ struct pirg_server {
   pirg::LimbLayout dbL{}, pkL{};
   // Batches of at least this many requests use the MFMA-bound WideScan.
   int wide_min = 10;
+  // Device memory the serving loop's slots may take (0: 80% of the free
+  // memory at start); under serve_mu.
+  size_t serve_mem_limit = 0;
   // Small batches run their pack branch beside the DB scan.
   bool fork_branches = true;
   // Default plans may use the digit-major WideScan layout, and for two-plane

@@ -361,3 +361,47 @@ def test_zero_copy_records(material, requests):
         np.testing.assert_array_equal(got[100 + i], alone[i])
     want, _, _ = O.process_response(P, material, requests[5])
     np.testing.assert_array_equal(got[5], want)
+
+
+def test_admission_by_device_memory_limit(material, requests):
+    # Admission by device memory, forced with a serving memory limit: slots
+    # are dropped first (down to two), then the batch capacity halves; a
+    # limit below two single-request slots refuses the start; the loop then
+    # starts and answers under the default budget again.
+    reqs = [gpu_request(r) for r in requests[:6]]
+    with server_from_material(P, material) as srv:
+        want = [srv.process_response(r).ciphertexts for r in reqs]
+        w8, w4, w1 = (srv.serve_workspace_bytes(b) for b in (8, 4, 1))
+        assert w1 < w4 < w8
+        srv.serve_set_memory_limit(2 * w4 + w4 // 2)   # two slots of 4 fit, not of 8
+        srv.serve_start(max_batch=8, slots=3, max_outstanding=64)
+        try:
+            st = srv.serve_stats()
+            assert (st["slots"], st["max_batch"]) == (2, 4)
+            assert srv.submit_batch(reqs, list(range(6))) == [0] * 6
+            got = {}
+            while len(got) < 6:
+                r = srv.poll_batch(8, 30000)
+                assert r is not None
+                for tg, o, s_ in zip(r[0], r[1], r[2]):
+                    assert s_ == 0
+                    got[tg] = o.copy()
+        finally:
+            srv.serve_stop()
+        for i in range(6):
+            np.testing.assert_array_equal(got[i], want[i])
+        srv.serve_set_memory_limit(2 * w1 - 1)
+        with pytest.raises(ia.PirError) as ei:
+            srv.serve_start(max_batch=8, slots=2, max_outstanding=64)
+        assert ei.value.code == ia.RESOURCE_EXHAUSTED
+        assert "not enough free device memory" in ei.value.message
+        srv.serve_set_memory_limit(0)
+        srv.serve_start(max_batch=8, slots=2, max_outstanding=64)
+        try:
+            assert srv.serve_stats()["max_batch"] == 8
+            srv.submit(reqs[0], 99)
+            r = srv.poll(30000)
+            assert r is not None and r[0] == 99
+            np.testing.assert_array_equal(r[1].ciphertexts, want[0])
+        finally:
+            srv.serve_stop()
