@@ -60,9 +60,19 @@ struct IpJob {
   u64* const* optr;            // [n_out] base of the pair partials (npairs x d)
 };
 
+// Raises *margin to bits unless it already holds as much. The margin settles
+// within the first workgroups of a launch, so a plain load turns nearly every
+// later atomic into none: tens of thousands of same-address atomics per
+// 256-query batch took 0.6 ms of its 15.5.
+__device__ __forceinline__ void RaiseMargin(unsigned long long* margin, unsigned long long bits) {
+  const unsigned long long cur =
+      __hip_atomic_load(margin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (bits > cur) atomicMax(margin, bits);
+}
+
 // Maximum of non-negative doubles (their bit patterns order the same way as
-// the values): a wave-level reduction and one atomic per wave when the
-// workgroup is made of full waves, one atomic per thread otherwise.
+// the values): a wave-level reduction and one update per wave when the
+// workgroup is made of full waves, one per thread otherwise.
 template <int NT>
 __device__ __forceinline__ void MarginMax(double local, unsigned long long* margin) {
   unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(local));
@@ -72,9 +82,9 @@ __device__ __forceinline__ void MarginMax(double local, unsigned long long* marg
       const unsigned long long o = __shfl_xor(bits, off);
       bits = o > bits ? o : bits;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(margin, bits);
+    if ((threadIdx.x & 63) == 0) RaiseMargin(margin, bits);
   } else {
-    atomicMax(margin, bits);
+    RaiseMargin(margin, bits);
   }
 }
 

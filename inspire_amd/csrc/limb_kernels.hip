@@ -807,9 +807,7 @@ __global__ __launch_bounds__(512) void WideScanPipeKernel(
 // per 52. The 168 accumulators leave no room for a second set of A
 // fragments, so A is streamed one MFMA block ahead (one fragment per block)
 // while B is read a whole step ahead. The A ring leads by 2 steps and the B
-// ring by 3; besides the barrier at the top of a step (A(i) and B(i + 1) have
-// landed everywhere), one before the last block guards the read of step
-// i + 1's first A fragment.
+// ring by 3, with one barrier per step, before its last block.
 template <int CT>
 struct TallShape {
   static constexpr int NP = 2, RS = 12, RW = 6, AF = RS * NP / 8;
@@ -898,8 +896,11 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
   // Step i: B fragments of step i in cb, its first A fragment in a_in. Issues
   // A(i + 2) and B(i + 3) (A first), reads step i + 1's B into nb and leaves
   // its first A fragment in a_in. Issue order: prologue A(0) B(0) A(1) B(1)
-  // B(2), then per step A(i + 2) B(i + 3). Before the last block, A(i + 1)
-  // must have landed: issued after it are B(i + 2) and all of step i's.
+  // B(2), then per step A(i + 2) B(i + 3). The step's one barrier, before its
+  // last block, needs A(i + 1) and B(i + 2) landed everywhere (in flight only
+  // step i's own issues), and orders every wave's reads of A(i) and B(i + 1)
+  // before the next step's DMAs that refill those slots (A(i + 3) into A(i)'s,
+  // B(i + 4) into B(i + 1)'s).
   auto step = [&](int i, v4i (&cb)[CT], v4i (&nb)[CT], v4i& a_in) {
     v4i ab[2];
     ab[0] = a_in;
@@ -922,7 +923,7 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
       if (blk + 1 < kBlk) {
         ab[(blk + 1) & 1] = read_a(i, blk + 1);
       } else {
-        wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{});
+        wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
         LdsBarrier();
         a_in = read_a(i + 1, 0);
       }
@@ -946,16 +947,12 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
 #pragma unroll
   for (int c = 0; c < CT; ++c) cb0[c] = read_b(0, c);
   a_in = read_a(0, 0);
+  // Every wave has read B(0) before step 0's B(3) DMA refills its slot.
+  LdsBarrier();
   const int nkb2 = (nkb + 1) & ~1;
   for (int i = 0; i < nkb2; i += 2) {
     step(i, cb0, cb1, a_in);
-    // A(i + 1) and B(i + 2) have landed everywhere and step i is consumed:
-    // in flight at most step i's issues.
-    wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
-    LdsBarrier();
     step(i + 1, cb1, cb0, a_in);
-    wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
-    LdsBarrier();
   }
   WaitVmcnt<0>();
   LdsBarrier();
