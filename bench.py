@@ -391,8 +391,59 @@ def run_serving(rates, seconds: float, seed: int = 5):
                     f"mean batch {st['mean_batch']:.1f}")
             st = srv.serve_stats()
             out["slots"], out["max_batch"] = st["slots"], st["max_batch"]
+            out["saturated"] = run_saturated(srv, pool, seconds)
         finally:
             srv.serve_stop()
+    return out
+
+
+def run_saturated(srv: ia.PirServer, pool, seconds: float, windows=(192, 768)):
+    """The serving loop past its capacity: a closed loop keeps `window`
+    requests outstanding (every answer polled is replaced at once), from
+    pinned request records (pirg_serve_submit_records, no host copy), so the
+    GPU and not the submitter sets the rate. Reports the sustained rate and
+    the loop's p50 / p99 at that load."""
+    out = []
+    with ia.HostRecords(srv, len(pool)) as hr:
+        for i, r in enumerate(pool):
+            srv.write_record(r, hr.array[i])
+        rows = [hr.array[i] for i in range(len(pool))]
+        for window in windows:
+            srv.serve_stats(reset=True)
+            submitted = answered = failed = 0
+
+            def submit(k):
+                nonlocal submitted
+                while k > 0:
+                    c = min(k, 64)
+                    srv.submit_records([rows[(submitted + j) % len(rows)] for j in range(c)],
+                                       list(range(submitted, submitted + c)))
+                    submitted += c
+                    k -= c
+
+            t0 = time.perf_counter()
+            submit(window)
+            while time.perf_counter() - t0 < seconds:
+                r = srv.poll_batch(256, 30000)
+                if r is None:
+                    break
+                answered += len(r[0])
+                failed += sum(1 for x in r[2] if x != 0)
+                submit(len(r[0]))
+            while answered < submitted:
+                r = srv.poll_batch(256, 30000)
+                if r is None:
+                    break
+                answered += len(r[0])
+                failed += sum(1 for x in r[2] if x != 0)
+            wall = time.perf_counter() - t0
+            st = srv.serve_stats()
+            out.append({"outstanding": window, "answered": answered, "failed": failed,
+                        "sustained_qps": answered / wall, "p50_ms": st["p50_ms"],
+                        "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"]})
+            log(f"serving saturated, {window} outstanding: {answered / wall:.0f} q/s, "
+                f"p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms mean batch "
+                f"{st['mean_batch']:.1f}")
     return out
 
 
