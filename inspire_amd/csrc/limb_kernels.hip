@@ -896,11 +896,20 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
   // Step i: B fragments of step i in cb, its first A fragment in a_in. Issues
   // A(i + 2) and B(i + 3) (A first), reads step i + 1's B into nb and leaves
   // its first A fragment in a_in. Issue order: prologue A(0) B(0) A(1) B(1)
-  // B(2), then per step A(i + 2) B(i + 3). The step's one barrier, before its
-  // last block, needs A(i + 1) and B(i + 2) landed everywhere (in flight only
-  // step i's own issues), and orders every wave's reads of A(i) and B(i + 1)
-  // before the next step's DMAs that refill those slots (A(i + 3) into A(i)'s,
-  // B(i + 4) into B(i + 1)'s).
+  // B(2), then per step A(i + 2) B(i + 3). Each block's MFMAs are issued after
+  // the read of the next block's A fragment, so that read lands while they
+  // run. The step's one barrier, after the tail of its next-to-last block
+  // (whose read of A(i)'s last fragment it waits for), needs A(i + 1) and
+  // B(i + 2) landed everywhere (in flight only step i's own issues), and
+  // orders every wave's reads of A(i) and B(i + 1) before the next step's
+  // DMAs that refill those slots (A(i + 3) into A(i)'s, B(i + 4) into
+  // B(i + 1)'s). The read of A(i + 1)'s first fragment after it then lands
+  // under the last block's MFMAs. Measured at 256 queries: 2% faster than
+  // reading each fragment after the previous block's MFMAs with the barrier
+  // after the last block, with identical partial sums.
+  constexpr int kBar = kBlk - 2;
+  static_assert(kBar >= kDma - 1 && kBar >= CT - 1,
+                "the barrier follows the step's DMA issues and B reads");
   auto step = [&](int i, v4i (&cb)[CT], v4i (&nb)[CT], v4i& a_in) {
     v4i ab[2];
     ab[0] = a_in;
@@ -908,6 +917,8 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
     for (int blk = 0; blk < kBlk; ++blk) {
       const int r = blk / NP, p = blk % NP;
       const v4i acur = ab[blk & 1];
+      if (blk + 1 < kBlk) ab[(blk + 1) & 1] = read_a(i, blk + 1);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int c = p; c < CT; ++c)
         acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur, cb[c - p], acc[r][c], 0, 0, 0);
@@ -920,9 +931,7 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
         }
       }
       if (blk < CT) nb[blk] = read_b(i + 1, blk);
-      if (blk + 1 < kBlk) {
-        ab[(blk + 1) & 1] = read_a(i, blk + 1);
-      } else {
+      if (blk == kBar) {
         wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
         LdsBarrier();
         a_in = read_a(i + 1, 0);
