@@ -874,6 +874,9 @@ def main():
     best = per_width[best_B]
     read_ms = probes.read_probe(4 << 30, repeats=4)
     mfma_ms, mfma_tops = probes.mfma_probe(2048, 4096)
+    # The achievable rate on random operands (two waves per SIMD, like the
+    # scans): the practical ceiling of a product over random data.
+    _, mfma_random_tops = probes.mfma_probe_random(512, 16384)
 
     db_bytes = M * K * 2
     pairs = 13  # (plane, digit) pairs with 8 (p + c) < 52
@@ -886,6 +889,8 @@ def main():
                 "traffic": traffic.get(f"db_scan_batch{B}")}
 
     roofline = mfma_line(best_B, best["db_scan_ms"])
+    roofline["achievable_random_operands"] = mfma_random_tops
+    roofline["frac_of_achievable"] = roofline["achieved"] / mfma_random_tops
     gbs = db_bytes / (single["db_scan_ms"] * 1e-3) / 1e9
     roofline_single = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": gbs / HBM_PEAK_GBS, "traffic": traffic.get("db_scan_single"),
@@ -1012,6 +1017,7 @@ def main():
         "roofline": roofline,
         "roofline_single_scan": roofline_single,
         "mfma_probe_tops": mfma_tops,
+        "mfma_probe_random_tops": mfma_random_tops,
         "widths": widths_out,
         "autotune": tune_report.strip().split("\n"),
         "serving": serving,

@@ -31,6 +31,12 @@ int pirg_read_probe(size_t bytes, int repeats, double* ms);
  * time and the int8 rate in TOP/s: the achievable MFMA rate. */
 int pirg_mfma_probe(int blocks, int iters, double* ms, double* tops);
 
+/* The same stream with random int8 operands (eight hashed operand pairs per
+ * lane, rotated across the accumulators): the rate that products over random
+ * data can approach, since the matrix cores' clock falls with the operands'
+ * bit activity. */
+int pirg_mfma_probe_random(int blocks, int iters, double* ms, double* tops);
+
 #ifdef __cplusplus
 }
 #endif

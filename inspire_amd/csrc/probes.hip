@@ -68,6 +68,47 @@ __global__ __launch_bounds__(256) void MfmaProbeKernel(int iters, int* __restric
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+__device__ __forceinline__ unsigned ProbeHash(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// The same stream with random int8 operands: eight operand pairs per lane,
+// hashed from the thread index and rotated across the accumulators, so that
+// consecutive MFMAs see different random bytes as the scans' do. The matrix
+// cores' clock drops with the operands' bit activity, so this rate, not the
+// near-constant one above, is what a scan over random data can approach.
+__global__ __launch_bounds__(256) void MfmaRandomProbeKernel(int iters, int* __restrict__ out) {
+  constexpr int kSets = 8;
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  v4i a[kSets], b[kSets];
+#pragma unroll
+  for (int s = 0; s < kSets; ++s) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      a[s][e] = static_cast<int>(ProbeHash(t * 64u + s * 8u + e));
+      b[s][e] = static_cast<int>(ProbeHash(t * 64u + s * 8u + 4u + e) ^ 0x5bd1e995u);
+    }
+  }
+  v4i acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = v4i{0, 0, 0, 0};
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[j % kSets], b[(j / 2) % kSets], acc[j], 0,
+                                                     0, 0);
+  }
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  out[t] = s;
+}
+
 constexpr int kReadBlocks = 2048;
 
 // Device buffers, a non-blocking stream and two events, released on return.
@@ -100,6 +141,25 @@ struct Ctx {
     return t;
   }
 };
+
+int MfmaProbe(bool random, int blocks, int iters, double* ms, double* tops) {
+  if (blocks <= 0 || iters <= 0) return 3;
+  Ctx c;
+  if (!c.Init(static_cast<size_t>(blocks) * 256 * 4, 16)) return 14;
+  int* out = static_cast<int*>(c.a);
+  auto kernel = random ? MfmaRandomProbeKernel : MfmaProbeKernel;
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, c.s, iters, out);
+  (void)hipEventRecord(c.e0, c.s);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, c.s, iters, out);
+  (void)hipEventRecord(c.e1, c.s);
+  const float t = c.Elapsed();
+  if (t < 0 || hipGetLastError() != hipSuccess) return 13;
+  if (ms) *ms = t;
+  // 4 waves per block, 16 MFMAs per round, 16 x 16 x 64 x 2 ops each.
+  const double ops = static_cast<double>(blocks) * 4 * iters * 16 * 32768.0;
+  if (tops) *tops = ops / (t * 1e-3) / 1e12;
+  return 0;
+}
 
 }  // namespace
 
@@ -145,22 +205,13 @@ int pirg_read_probe(size_t bytes, int repeats, double* ms) {
   return 0;
 }
 
+
 int pirg_mfma_probe(int blocks, int iters, double* ms, double* tops) {
-  if (blocks <= 0 || iters <= 0) return 3;
-  Ctx c;
-  if (!c.Init(static_cast<size_t>(blocks) * 256 * 4, 16)) return 14;
-  int* out = static_cast<int*>(c.a);
-  hipLaunchKernelGGL(MfmaProbeKernel, dim3(blocks), dim3(256), 0, c.s, iters, out);
-  (void)hipEventRecord(c.e0, c.s);
-  hipLaunchKernelGGL(MfmaProbeKernel, dim3(blocks), dim3(256), 0, c.s, iters, out);
-  (void)hipEventRecord(c.e1, c.s);
-  const float t = c.Elapsed();
-  if (t < 0 || hipGetLastError() != hipSuccess) return 13;
-  if (ms) *ms = t;
-  // 4 waves per block, 16 MFMAs per round, 16 x 16 x 64 x 2 ops each.
-  const double ops = static_cast<double>(blocks) * 4 * iters * 16 * 32768.0;
-  if (tops) *tops = ops / (t * 1e-3) / 1e12;
-  return 0;
+  return MfmaProbe(false, blocks, iters, ms, tops);
+}
+
+int pirg_mfma_probe_random(int blocks, int iters, double* ms, double* tops) {
+  return MfmaProbe(true, blocks, iters, ms, tops);
 }
 
 }  // extern "C"

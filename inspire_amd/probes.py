@@ -50,3 +50,14 @@ def mfma_probe(blocks: int = 1024, iters: int = 4096):
     if rc != 0:
         raise PirError(rc, "mfma probe failed")
     return ms.value, tops.value
+
+
+def mfma_probe_random(blocks: int = 512, iters: int = 16384):
+    """(ms, TOP/s) of the register-only int8 MFMA stream on random operands
+    (the rate a product over random data can approach)."""
+    ms = ctypes.c_double()
+    tops = ctypes.c_double()
+    rc = lib().pirg_mfma_probe_random(blocks, iters, ctypes.byref(ms), ctypes.byref(tops))
+    if rc != 0:
+        raise PirError(rc, "mfma probe failed")
+    return ms.value, tops.value

@@ -34,7 +34,8 @@ def test_probes_library_exports_its_header_and_the_product_does_not():
     # path's ABI.
     from inspire_amd import probes
     names = _declared(os.path.join(ROOT, "include", "pirgpu_probes.h"), "pirg_")
-    assert names == ["pirg_copy_probe", "pirg_mfma_probe", "pirg_read_probe"]
+    assert names == ["pirg_copy_probe", "pirg_mfma_probe", "pirg_mfma_probe_random",
+                     "pirg_read_probe"]
     plib = probes.lib()
     assert all(hasattr(plib, n) for n in names)
     product = set(_declared(os.path.join(ROOT, "include", "pirgpu.h"), "pirg_"))
