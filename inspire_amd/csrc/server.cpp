@@ -88,6 +88,8 @@ Slot::~Slot() {
   if (side != nullptr) (void)hipStreamSynchronize(side);
   if (fork_ev != nullptr) (void)hipEventDestroy(fork_ev);
   if (join_ev != nullptr) (void)hipEventDestroy(join_ev);
+  if (hfork_ev != nullptr) (void)hipEventDestroy(hfork_ev);
+  if (hjoin_ev != nullptr) (void)hipEventDestroy(hjoin_ev);
   if (side != nullptr) (void)hipStreamDestroy(side);
   if (stream != nullptr) (void)hipStreamDestroy(stream);
 }
@@ -102,6 +104,8 @@ Error Slot::Init() {
   PIRG_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   PIRG_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
   PIRG_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+  PIRG_HIP(hipEventCreateWithFlags(&hfork_ev, hipEventDisableTiming));
+  PIRG_HIP(hipEventCreateWithFlags(&hjoin_ev, hipEventDisableTiming));
   return Ok();
 }
 
@@ -917,52 +921,82 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
     }
     const size_t n_ext_cap = static_cast<size_t>(W.cap) * S->ell * 2;
     void* const* et = W.ext_tab.as<void* const>();
-    for (int i = S->t - 2; i >= 0; --i) {
-      DigitsArgs DA{};
-      DA.st_a = W.st_a.as<u64>();
-      DA.st_b = W.st_b.as<u64>();
-      DA.nq = nq;
-      DA.ell = S->ell;
-      DA.d = d;
-      DA.e = S->e;
-      DA.log_q = S->log_q;
-      DA.log_digit = S->P.eval_log_digit;
-      DA.rule = S->ext_u;
-      DA.dspec = W.d_spec.as<double2>();
-      // After the first step the digits kernel forms the state itself from
-      // the previous step's partials; only the last step runs ExtCombine
-      // (which also switches the modulus).
-      DA.prev_step = i == S->t - 2 ? -1 : i + 1;
-      DA.ext_partial = W.ext_partial.as<u64>();
-      DA.npairs = S->ext_pairs.count;
-      DA.t = S->t;
-      DA.a_tilde = S->a_tilde.as<u64>();
-      DA.ct_b = W.ct_b.as<u64>();
-      PIRG_HIP(LaunchDigitsSpectra(DA, S->F, S->tw_fwd.as<double2>(), st));
-      PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(et),
-                        reinterpret_cast<const double2* const*>(et + n_ext_cap),
-                        reinterpret_cast<u64* const*>(et + 2 * n_ext_cap),
-                        nq * S->ell * 2, S->ell * 2, 2 * S->e, S->ext_u.num_chunks,
-                        S->ext_v.num_chunks, S->ext_pairs, S->F, S->tw_bwd.as<double2>(),
-                        margin, st, W.mac_acc.as<double2>()));
-      ExtCombineArgs EA{};
-      EA.ext_partial = W.ext_partial.as<u64>();
-      EA.npairs = S->ext_pairs.count;
-      EA.a_tilde = S->a_tilde.as<u64>();
-      EA.ct_b = W.ct_b.as<u64>();
-      EA.st_a = W.st_a.as<u64>();
-      EA.st_b = W.st_b.as<u64>();
-      EA.out = W.out.as<u64>();
-      EA.nq = nq;
-      EA.ell = S->ell;
-      EA.t = S->t;
-      EA.d = d;
-      EA.step_i = i;
-      EA.log_q = S->log_q;
-      EA.log_q1 = S->P.log_q1;
-      EA.log_q2 = S->P.log_q2;
-      EA.cmask = S->cmask;
-      if (i == 0) PIRG_HIP(LaunchExtCombine(EA, st));
+    const int d_n = S->N;
+    const int nu_e = S->ext_u.num_chunks;
+    // Requests are independent, so the chain of half the batch's requests
+    // runs beside the other half's: one half's digit transforms (FFT-bound)
+    // then overlap the other's inner products (bound by their loads), where
+    // in line every kernel waits for the one before. Each half must keep its
+    // inner products off the shared accumulation workspace of small
+    // launches (more than kIpSplitMaxJobs jobs).
+    const int half = nq / 2;
+    const bool split = S->horner_split &&
+                       static_cast<long long>(half) * S->ell * 2 * S->ext_pairs.count >
+                           kIpSplitMaxJobs;
+    const int nparts = split ? 2 : 1;
+    if (split) {
+      PIRG_HIP(hipEventRecord(slot.hfork_ev, st));
+      PIRG_HIP(hipStreamWaitEvent(slot.side, slot.hfork_ev, 0));
+    }
+    for (int part = 0; part < nparts; ++part) {
+      hipStream_t ps = part == 0 ? st : slot.side;
+      const int q0 = part == 0 ? 0 : half;
+      const int pq = split ? (part == 0 ? half : nq - half) : nq;
+      const size_t qo = static_cast<size_t>(q0) * S->ell;  // first (request, shard)
+      const size_t to = qo * 2;                            // its first output
+      for (int i = S->t - 2; i >= 0; --i) {
+        DigitsArgs DA{};
+        DA.st_a = W.st_a.as<u64>() + qo * d;
+        DA.st_b = W.st_b.as<u64>() + qo * d;
+        DA.nq = pq;
+        DA.ell = S->ell;
+        DA.d = d;
+        DA.e = S->e;
+        DA.log_q = S->log_q;
+        DA.log_digit = S->P.eval_log_digit;
+        DA.rule = S->ext_u;
+        DA.dspec = W.d_spec.as<double2>() + qo * 2 * S->e * nu_e * static_cast<size_t>(d_n);
+        // After the first step the digits kernel forms the state itself from
+        // the previous step's partials; only the last step runs ExtCombine
+        // (which also switches the modulus).
+        DA.prev_step = i == S->t - 2 ? -1 : i + 1;
+        DA.ext_partial = W.ext_partial.as<u64>() + to * S->ext_pairs.count * d;
+        DA.npairs = S->ext_pairs.count;
+        DA.t = S->t;
+        DA.a_tilde = S->a_tilde.as<u64>();
+        DA.ct_b = W.ct_b.as<u64>() + qo * S->t * d;
+        PIRG_HIP(LaunchDigitsSpectra(DA, S->F, S->tw_fwd.as<double2>(), ps));
+        PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(et + to),
+                          reinterpret_cast<const double2* const*>(et + n_ext_cap + to),
+                          reinterpret_cast<u64* const*>(et + 2 * n_ext_cap + to),
+                          pq * S->ell * 2, S->ell * 2, 2 * S->e, S->ext_u.num_chunks,
+                          S->ext_v.num_chunks, S->ext_pairs, S->F, S->tw_bwd.as<double2>(),
+                          margin, ps, W.mac_acc.as<double2>()));
+        if (i == 0) {
+          ExtCombineArgs EA{};
+          EA.ext_partial = W.ext_partial.as<u64>() + to * S->ext_pairs.count * d;
+          EA.npairs = S->ext_pairs.count;
+          EA.a_tilde = S->a_tilde.as<u64>();
+          EA.ct_b = W.ct_b.as<u64>() + qo * S->t * d;
+          EA.st_a = W.st_a.as<u64>() + qo * d;
+          EA.st_b = W.st_b.as<u64>() + qo * d;
+          EA.out = W.out.as<u64>() + to * d;
+          EA.nq = pq;
+          EA.ell = S->ell;
+          EA.t = S->t;
+          EA.d = d;
+          EA.step_i = i;
+          EA.log_q = S->log_q;
+          EA.log_q1 = S->P.log_q1;
+          EA.log_q2 = S->P.log_q2;
+          EA.cmask = S->cmask;
+          PIRG_HIP(LaunchExtCombine(EA, ps));
+        }
+      }
+    }
+    if (split) {
+      PIRG_HIP(hipEventRecord(slot.hjoin_ev, slot.side));
+      PIRG_HIP(hipStreamWaitEvent(st, slot.hjoin_ev, 0));
     }
   }
   if (host_io) {
@@ -1122,7 +1156,8 @@ Error ValidateView(const pirg_server* S, const pirg_preprocessed_view* v) {
 // sets the batch size from which the DB and pack products use the MFMA-bound
 // WideScan, PIRG_DIGIT_MAJOR=0 keeps WideScan on the query-major B layout and
 // PIRG_TALL=0 keeps digit-major scans on the 8-strip kernel (tests cover every
-// kernel at the same batch sizes with them).
+// kernel at the same batch sizes with them); PIRG_HORNER_SPLIT=0 runs
+// EvalPoly's Horner chain for the whole batch in line.
 void ApplyPlanOverrides(pirg_server* S) {
   if (const char* env = std::getenv("PIRG_WIDE_MIN")) {
     const int v = std::atoi(env);
@@ -1130,6 +1165,7 @@ void ApplyPlanOverrides(pirg_server* S) {
   }
   if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
   if (const char* env = std::getenv("PIRG_TALL")) S->tall = std::atoi(env) != 0;
+  if (const char* env = std::getenv("PIRG_HORNER_SPLIT")) S->horner_split = std::atoi(env) != 0;
 }
 
 // Matrix<uint64_t> pack blocks (PirServer<Db, uint64_t, uint64_t>,

@@ -130,6 +130,9 @@ struct Slot {
   // beside the DB scan, forked from and joined back into `stream`.
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // Larger batches run EvalPoly's Horner chain as two independent halves of
+  // their requests, the second half on `side`.
+  hipEvent_t hfork_ev = nullptr, hjoin_ev = nullptr;
   bool timing_pending = false;
   bool graph_run = false;  // the pending timing came from a graph launch
   // Graphs captured on this slot bracket the two scans with event-record
@@ -173,6 +176,9 @@ struct pirg_server {
   // matrices its tall kernel.
   bool digit_major = true;
   bool tall = true;
+  // Batches with enough requests split EvalPoly's Horner chain into two
+  // concurrent halves.
+  bool horner_split = true;
   // Scan configurations chosen by pirg_server_autotune, per batch size.
   std::map<int, pirg::ScanPlan> tuned_db, tuned_pk;
   pirg::DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
