@@ -119,6 +119,7 @@ def time_width(srv: ia.PirServer, reqs, B: int, steps: int, warmup: int):
     db_ms, pk_ms, n = srv.scan_times_ms(reset=True)
     srv.time_graph_scans(False)
     srv.use_graphs(False)
+    fused = srv.scans_fused(B)
     # The last replay's answers, sampled, against the single-request path
     # (whose first answer the CPU baseline compares with the oracle).
     outs = srv.batch_fetch(B)
@@ -127,7 +128,17 @@ def time_width(srv: ia.PirServer, reqs, B: int, steps: int, warmup: int):
     return {"batch": B, "qps": B * steps / total, "ms_per_batch": float(np.mean(lat)),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "db_scan_ms": db_ms / max(n, 1), "pack_scan_ms": pk_ms / max(n, 1),
-            "matches_single": bool(same)}
+            "scans_fused": fused, "matches_single": bool(same)}
+
+
+def scan_mfma_tops(M, K, d, B, res):
+    """Int8 rate of the batch's DB scan: 13 (plane, digit) products per
+    element of the DB product; when the DB and pack scans are one launch
+    (db_scan_ms is then that launch's time), plus the pack product's 22."""
+    ops = 2.0 * M * K * B * 13
+    if res.get("scans_fused"):
+        ops += 2.0 * M * 2 * d * B * 22
+    return ops / (res["db_scan_ms"] * 1e-3) / 1e12
 
 
 def cpu_baseline(P, mat, req, max_queries=3, budget_s=30.0, threads=1):
@@ -314,9 +325,10 @@ def run_small_config(name: str, n: int, log_d: int, t: int, B: int, steps: int, 
         out["db_scan_hbm_frac"] = db_bytes / (res["db_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
         out["pack_scan_hbm_frac"] = pk_bytes / (res["pack_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     else:
-        tops = 2.0 * M * K * B * 13 / (res["db_scan_ms"] * 1e-3) / 1e12
-        out["db_scan_mfma_frac"] = tops / INT8_PEAK_TOPS
-        out["db_scan_hbm_frac"] = db_bytes / (res["db_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        out["scans_fused"] = res["scans_fused"]
+        out["db_scan_mfma_frac"] = scan_mfma_tops(M, K, d, B, res) / INT8_PEAK_TOPS
+        if not res["scans_fused"]:
+            out["db_scan_hbm_frac"] = db_bytes / (res["db_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     if cpu:
         qps, nq, cpu_out, _ = cpu_baseline(P, mat, reqs[0], max_queries=5, budget_s=10.0)
         out["cpu_baseline"] = {"value": qps, "unit": "queries/s", "cores": 1, "kind": "port",
@@ -879,16 +891,17 @@ def main():
     _, mfma_random_tops = probes.mfma_probe_random(512, 16384)
 
     db_bytes = M * K * 2
-    pairs = 13  # (plane, digit) pairs with 8 (p + c) < 52
     traffic = load_traffic()
 
-    def mfma_line(B, scan_ms):
-        tops = 2.0 * M * K * B * pairs / (scan_ms * 1e-3) / 1e12
+    def mfma_line(B, res):
+        tops = scan_mfma_tops(M, K, d, B, res)
         return {"bound": "mfma", "achieved": tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                 "frac": tops / INT8_PEAK_TOPS,
-                "traffic": traffic.get(f"db_scan_batch{B}")}
+                "kernel": ("DB and pack scans, one launch" if res["scans_fused"]
+                           else "DB scan"),
+                "traffic": traffic.get(f"db_scan_batch{B}") if not res["scans_fused"] else None}
 
-    roofline = mfma_line(best_B, best["db_scan_ms"])
+    roofline = mfma_line(best_B, best)
     roofline["achievable_random_operands"] = mfma_random_tops
     roofline["frac_of_achievable"] = roofline["achieved"] / mfma_random_tops
     gbs = db_bytes / (single["db_scan_ms"] * 1e-3) / 1e9
@@ -899,8 +912,10 @@ def main():
     for B, res in sorted(per_width.items()):
         line = dict(res)
         if B > 1:
-            line["db_scan_mfma_frac"] = mfma_line(B, res["db_scan_ms"])["frac"]
-            line["db_scan_hbm_frac"] = db_bytes / (res["db_scan_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+            line["db_scan_mfma_frac"] = mfma_line(B, res)["frac"]
+            if not res["scans_fused"]:
+                line["db_scan_hbm_frac"] = (db_bytes / (res["db_scan_ms"] * 1e-3) / 1e9 /
+                                            HBM_PEAK_GBS)
         else:
             line["db_scan_hbm_frac"] = roofline_single["frac"]
         widths_out.append(line)

@@ -313,6 +313,14 @@ int pirg_server_scan_times(pirg_server* server, double* db_scan_ms,
                            double* pack_scan_ms, long long* launches,
                            int reset);
 
+/* *fused = 1 when a batch of `batch` requests runs the DB and pack scans as
+ * one kernel launch (batches whose DB scan takes a few rounds of workgroups:
+ * the pack product's workgroups fill the CUs the DB product's last round
+ * leaves idle); pirg_server_scan_times then reports that launch's time as the
+ * DB scan's and none for the pack scan. 0 when they are two launches. No
+ * reference counterpart. */
+int pirg_server_scans_fused(pirg_server* server, int batch, int* fused);
+
 /* Largest |x - round(x)| over every value the FFT products rounded since the
  * last reset, the precision margin of the double-precision products. */
 double pirg_server_fft_margin(pirg_server* server, int reset);

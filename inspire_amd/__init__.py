@@ -532,6 +532,13 @@ class PirServer:
                                      int(reset))
         return a.value, b.value, n.value
 
+    def scans_fused(self, batch: int) -> bool:
+        """Whether batches of this size run the DB and pack scans as one
+        launch (scan_times_ms then reports it as the DB scan's time)."""
+        f = ctypes.c_int()
+        _check(lib().pirg_server_scans_fused(self._h, int(batch), ctypes.byref(f)), _err())
+        return bool(f.value)
+
     def fft_margin(self, reset: bool = False) -> float:
         """Largest |x - round(x)| the FFT products have rounded so far."""
         return lib().pirg_server_fft_margin(self._h, int(reset))

@@ -555,17 +555,18 @@ __device__ __forceinline__ bool DecodeWideTile(int w, int nx, int ntiles, int nz
   return true;
 }
 
+// w: the workgroup's position in the launch order (blockIdx.x of its own
+// launch); lds: the launch's dynamic LDS (WideShape<NP, CT>::kLdsBytes).
 template <int NP, int CT, bool DM>
-__global__ __launch_bounds__(512) void WideScanKernel(
-    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
-    const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
-    u64* __restrict__ partial) {
+__device__ __forceinline__ void WideScanBody(
+    int w, const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps,
+    int ksplit, const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
+    u64* __restrict__ partial, v4i* lds) {
   using W = WideShape<NP, CT>;
   constexpr int RS = W::RS, RW = W::RW, AF = W::AF, S = W::S;
-  extern __shared__ v4i lds[];
   const int nx = (strips + RS - 1) / RS;
   WideTile wt;
-  if (!DecodeWideTile(blockIdx.x, nx, nx * ksplit, ngp / 4, &wt)) return;
+  if (!DecodeWideTile(w, nx, nx * ksplit, ngp / 4, &wt)) return;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int h = wave >> 2;       // strip half
@@ -676,6 +677,16 @@ __global__ __launch_bounds__(512) void WideScanKernel(
     WaveEpilogue<RW, CT>(acc, red, lane, ns, qpg, grp, nq, rows,
                          (long long)(strip0 + h * RW) * 16, ks, partial);
   }
+}
+
+template <int NP, int CT, bool DM>
+__global__ __launch_bounds__(512) void WideScanKernel(
+    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
+    const int8_t* __restrict__ btiles, int ngp, int nq, int ns, int qpg,
+    u64* __restrict__ partial) {
+  extern __shared__ v4i lds[];
+  WideScanBody<NP, CT, DM>(blockIdx.x, tiles, strips, rows, kblocks, kbps, ksplit, btiles, ngp,
+                           nq, ns, qpg, partial, lds);
 }
 
 // Digit-major WideScan with software-pipelined fragments (NP <= 2): the
@@ -821,15 +832,15 @@ struct TallShape {
 };
 
 template <int CT>
-__global__ __launch_bounds__(512) void WideScanTallKernel(
-    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
-    const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial) {
+__device__ __forceinline__ void WideScanTallBody(
+    int w, const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps,
+    int ksplit, const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial,
+    v4i* lds) {
   using W = TallShape<CT>;
   constexpr int NP = W::NP, RS = W::RS, RW = W::RW, AF = W::AF, SA = W::SA, SB = W::SB;
-  extern __shared__ v4i lds[];
   const int nx = (strips + RS - 1) / RS;
   WideTile wt;
-  if (!DecodeWideTile(blockIdx.x, nx, nx * ksplit, ngp / 4, &wt)) return;
+  if (!DecodeWideTile(w, nx, nx * ksplit, ngp / 4, &wt)) return;
   const int lane = threadIdx.x & 63;
   // Wave-uniform, so that addresses derived from it stay in scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -969,6 +980,48 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
   WaveEpilogueDigitMajor<RW, CT>(acc, red, lane, grp, nq, rows,
                                  (long long)(strip0 + h * RW) * 16, ks, partial);
 }
+
+template <int CT>
+__global__ __launch_bounds__(512) void WideScanTallKernel(
+    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
+    const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial) {
+  extern __shared__ v4i lds[];
+  WideScanTallBody<CT>(blockIdx.x, tiles, strips, rows, kblocks, kbps, ksplit, btiles, ngp, nq,
+                       partial, lds);
+}
+
+// One operand of a fused scan launch.
+struct ScanOperand {
+  const int8_t* tiles;
+  int strips, rows, kblocks, kbps, ksplit;
+  const int8_t* btiles;
+  int ngp, nq, ns, qpg;
+  u64* partial;
+};
+
+// The DB product's tall scan and the pack product's digit-major WideScan
+// (four planes) as one launch: workgroups [0, db_blocks) are the DB scan's,
+// in its own order, the rest the pack scan's. Workgroups start in launch
+// order, so the pack product's take the CUs the DB product's last, partial
+// round leaves idle instead of waiting for the whole DB product. db_blocks
+// is a multiple of 8, so both keep their XCD placement (workgroup w on XCD
+// w % 8).
+__global__ __launch_bounds__(512) void DbPackScanKernel(ScanOperand db, ScanOperand pk,
+                                                        int db_blocks) {
+  extern __shared__ v4i lds[];
+  const int w = static_cast<int>(blockIdx.x);
+  if (w < db_blocks) {
+    WideScanTallBody<7>(w, db.tiles, db.strips, db.rows, db.kblocks, db.kbps, db.ksplit,
+                        db.btiles, db.ngp, db.nq, db.partial, lds);
+  } else {
+    WideScanBody<4, 7, true>(w - db_blocks, pk.tiles, pk.strips, pk.rows, pk.kblocks, pk.kbps,
+                             pk.ksplit, pk.btiles, pk.ngp, pk.nq, pk.ns, pk.qpg, pk.partial, lds);
+  }
+}
+
+constexpr size_t kFusedScanLds = TallShape<7>::kLdsBytes > WideShape<4, 7>::kLdsBytes
+                                     ? TallShape<7>::kLdsBytes
+                                     : WideShape<4, 7>::kLdsBytes;
 
 int GridBlocks(long long total, int threads) {
   long long b = (total + threads - 1) / threads;
@@ -1163,7 +1216,10 @@ hipError_t ConfigureScanKernels() {
                            ConfigureWide<4, 7, true>(), ConfigureWidePipe<1, 7>(),
                            ConfigureWidePipe<2, 7>(), ConfigureWide<4, 4, true>(),
                            ConfigureWidePipe<1, 4>(), ConfigureWidePipe<2, 4>(),
-                           ConfigureWideTall<7>()};
+                           ConfigureWideTall<7>(),
+                           hipFuncSetAttribute(reinterpret_cast<const void*>(DbPackScanKernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(kFusedScanLds))};
   for (hipError_t e : es) {
     if (e != hipSuccess) return e;
   }
@@ -1353,6 +1409,40 @@ hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& 
     case 4: return LaunchScanNP<4>(tiles, L, P, btiles, partial, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+bool ScanPairFusable(const LimbLayout& Ld, const ScanPlan& Pd, const LimbLayout& Lp,
+                     const ScanPlan& Pp) {
+  return Pd.nq > 0 && Pp.nq > 0 && Pd.wide && Pd.digit_major && Pd.tall && Pd.ctg == 7 &&
+         Pd.ns == 7 && Pd.qpg == 16 && Pd.ngp % 4 == 0 && Pd.group_rows == 0 &&
+         Ld.planes == 2 && Pp.wide && Pp.digit_major && Pp.ctg == 7 && Pp.ns == 7 &&
+         Pp.qpg == 16 && Pp.ngp % 4 == 0 && Pp.group_rows == 0 && Lp.planes == 4;
+}
+
+long long ScanWorkgroups(const LimbLayout& L, const ScanPlan& P) {
+  const long long tiles =
+      static_cast<long long>((L.strips + TallShape<7>::RS - 1) / TallShape<7>::RS) * P.ksplit;
+  return (tiles + 7) / 8 * 8 * (P.ngp / 4);
+}
+
+hipError_t LaunchScanPair(const int8_t* db_tiles, const LimbLayout& Ld, const ScanPlan& Pd,
+                          const int8_t* db_btiles, u64* db_partial, const int8_t* pk_tiles,
+                          const LimbLayout& Lp, const ScanPlan& Pp, const int8_t* pk_btiles,
+                          u64* pk_partial, hipStream_t s) {
+  if (!ScanPairFusable(Ld, Pd, Lp, Pp)) return hipErrorInvalidValue;
+  const long long db_blocks = ScanWorkgroups(Ld, Pd);
+  const long long pk_tiles_n =
+      static_cast<long long>((Lp.strips + WideShape<4, 7>::RS - 1) / WideShape<4, 7>::RS) *
+      Pp.ksplit;
+  const long long pk_blocks = (pk_tiles_n + 7) / 8 * 8 * (Pp.ngp / 4);
+  if (db_blocks + pk_blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  const ScanOperand db{db_tiles, Ld.strips, Ld.rows, Ld.kblocks, Pd.kb_per_split, Pd.ksplit,
+                       db_btiles, Pd.ngp, Pd.nq, Pd.ns, Pd.qpg, db_partial};
+  const ScanOperand pk{pk_tiles, Lp.strips, Lp.rows, Lp.kblocks, Pp.kb_per_split, Pp.ksplit,
+                       pk_btiles, Pp.ngp, Pp.nq, Pp.ns, Pp.qpg, pk_partial};
+  hipLaunchKernelGGL(DbPackScanKernel, dim3(static_cast<unsigned>(db_blocks + pk_blocks)),
+                     dim3(512), kFusedScanLds, s, db, pk, static_cast<int>(db_blocks));
+  return hipGetLastError();
 }
 
 }  // namespace pirg

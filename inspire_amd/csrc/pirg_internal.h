@@ -179,6 +179,19 @@ hipError_t LaunchVectorSums(const u64* vec, int nq, int stride, int len, u64 c_o
 // combined over (plane, digit) pairs with shift < 8 * ns.
 hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                       const int8_t* btiles, u64* partial, hipStream_t s);
+// The DB product's tall scan and the pack product's digit-major four-plane
+// WideScan as one launch, when both plans are of those kinds
+// (ScanPairFusable): the pack product's workgroups follow the DB product's
+// in launch order and fill the CUs its last round leaves idle. Same results
+// as the two LaunchScan calls.
+bool ScanPairFusable(const LimbLayout& Ld, const ScanPlan& Pd, const LimbLayout& Lp,
+                     const ScanPlan& Pp);
+// Workgroups of the DB product's scan under ScanPairFusable's plans.
+long long ScanWorkgroups(const LimbLayout& L, const ScanPlan& P);
+hipError_t LaunchScanPair(const int8_t* db_tiles, const LimbLayout& Ld, const ScanPlan& Pd,
+                          const int8_t* db_btiles, u64* db_partial, const int8_t* pk_tiles,
+                          const LimbLayout& Lp, const ScanPlan& Pp, const int8_t* pk_btiles,
+                          u64* pk_partial, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // FFT-side launchers (fft_kernels.hip). N = 2d, twiddles = N/2 complex.

@@ -337,6 +337,7 @@ Error BuildResident(pirg_server* S, const std::vector<const uint64_t*>& db_parts
   const int d = S->d;
   S->K = db_cols;
   PIRG_HIP(hipGetDevice(&S->device));
+  PIRG_HIP(hipDeviceGetAttribute(&S->cus, hipDeviceAttributeMultiprocessorCount, S->device));
   {
     Error ie = S->main.Init();
     if (!ie.ok()) return ie;
@@ -779,6 +780,19 @@ namespace {
 // ProcessResponse for nq staged requests (pir_server.cc:207-305), enqueued on
 // the slot's stream. `timed` records the stage and scan-kernel events (it is
 // off while the sequence is captured into a graph).
+// Whether a batch with these (MFMA-bound) plans runs its DB and pack scans as
+// one launch: the kinds LaunchScanPair fuses, and a DB scan of at most four
+// rounds of workgroups (see EnqueueBatch).
+bool FuseScans(const pirg_server* S, int nq, const ScanPlan& pd, const ScanPlan& pp) {
+  if (!pd.wide || !S->fuse_scans || !ScanPairFusable(S->dbL, pd, S->pkL, pp) ||
+      ScanWorkgroups(S->dbL, pd) > 4LL * S->cus) {
+    return false;
+  }
+  // A tuned width keeps the autotuner's measured choice.
+  const auto it = S->tuned_fuse.find(nq);
+  return it == S->tuned_fuse.end() || it->second;
+}
+
 Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io) {
   Workspace& W = slot.ws;
   hipStream_t st = slot.stream;
@@ -871,14 +885,37 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
   PIRG_HIP(LaunchQueryDigits(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
   PIRG_HIP(LaunchVectorSums(rec_fd, nq, rw, S->K, S->dbL.c_off,
                             W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
-  PIRG_HIP(kmark(slot.kev[0], st));
-  PIRG_HIP(LaunchScan(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
-                      W.db_partial.as<u64>(), st));
-  PIRG_HIP(kmark(slot.kev[1], st));
+  // Two MFMA-bound scans of the kinds LaunchScanPair fuses run as one launch
+  // (the pack product's query digits first) when the DB product takes few
+  // rounds of workgroups: the pack product's then fill the CUs that its last,
+  // partial round leaves idle (64 queries at n = 2^20, d = 4096: 2.7 rounds,
+  // 4.21 -> 4.11 ms per batch). Over many rounds the idle CUs only let the
+  // others clock higher (256 queries: 10.7 rounds, 14.57 against 14.62 ms),
+  // so those batches keep the two launches. The scan brackets of a fused
+  // batch both cover the one launch.
+  const bool fused = !fork && FuseScans(S, nq, pd, pp);
+  if (fused) {
+    Error er = pack_digits(st);
+    if (!er.ok()) return er;
+    PIRG_HIP(kmark(slot.kev[0], st));
+    PIRG_HIP(LaunchScanPair(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
+                            W.db_partial.as<u64>(), S->pk_tiles.as<int8_t>(), S->pkL, pp,
+                            W.bt_pk.as<int8_t>(), W.pk_partial.as<u64>(), st));
+    PIRG_HIP(kmark(slot.kev[1], st));
+    PIRG_HIP(kmark(slot.kev[2], st));
+    PIRG_HIP(kmark(slot.kev[3], st));
+  } else {
+    PIRG_HIP(kmark(slot.kev[0], st));
+    PIRG_HIP(LaunchScan(S->db_tiles.as<int8_t>(), S->dbL, pd, W.bt_db.as<int8_t>(),
+                        W.db_partial.as<u64>(), st));
+    PIRG_HIP(kmark(slot.kev[1], st));
+  }
   PIRG_HIP(mark(slot.ev[1]));
   {
     Error er;
-    if (fork) {
+    if (fused) {
+      er = finalize_ip(st);
+    } else if (fork) {
       PIRG_HIP(hipStreamWaitEvent(st, slot.join_ev, 0));
       er = pack_scan();
     } else {
@@ -1166,6 +1203,7 @@ void ApplyPlanOverrides(pirg_server* S) {
   if (const char* env = std::getenv("PIRG_DIGIT_MAJOR")) S->digit_major = std::atoi(env) != 0;
   if (const char* env = std::getenv("PIRG_TALL")) S->tall = std::atoi(env) != 0;
   if (const char* env = std::getenv("PIRG_HORNER_SPLIT")) S->horner_split = std::atoi(env) != 0;
+  if (const char* env = std::getenv("PIRG_FUSE_SCANS")) S->fuse_scans = std::atoi(env) != 0;
 }
 
 // Matrix<uint64_t> pack blocks (PirServer<Db, uint64_t, uint64_t>,
@@ -1400,7 +1438,28 @@ std::string DescribePlan(const ScanPlan& P) {
 
 // Times every candidate scan configuration for nq queries on the resident
 // matrix and keeps the fastest.
-Error TuneOne(pirg_server* S, bool db, int nq, std::string* report) {
+struct TimedPlan {
+  ScanPlan plan;
+  double ms;
+};
+
+// Scan operands for timing: B tiles with the bit patterns of real digits
+// (the MFMA clock depends on the data, and all-zero operands run faster than
+// query digits), so they are filled with bytes of the resident matrix.
+Error TuningOperands(pirg_server* S, const DevBuf& tiles, size_t bt_bytes, DevBuf* bt) {
+  Error er = bt->alloc(bt_bytes);
+  if (!er.ok()) return er;
+  hipStream_t st = S->main.stream;
+  const size_t fill = std::min(bt_bytes, tiles.bytes);
+  PIRG_HIP(hipMemcpyAsync(bt->p, tiles.p, fill, hipMemcpyDeviceToDevice, st));
+  if (fill < bt_bytes) {
+    PIRG_HIP(hipMemsetAsync(static_cast<char*>(bt->p) + fill, 0x5b, bt_bytes - fill, st));
+  }
+  return Ok();
+}
+
+Error TuneOne(pirg_server* S, bool db, int nq, std::string* report,
+              std::vector<TimedPlan>* timed) {
   const LimbLayout& L = db ? S->dbL : S->pkL;
   const DevBuf& tiles = db ? S->db_tiles : S->pk_tiles;
   const std::vector<ScanPlan> cands = ScanPlanCandidates(L, nq, S->log_q);
@@ -1410,19 +1469,11 @@ Error TuneOne(pirg_server* S, bool db, int nq, std::string* report) {
     part_words = std::max(part_words, static_cast<size_t>(c.ksplit) * nq * L.rows);
   }
   DevBuf bt, part;
-  Error er = bt.alloc(bt_bytes);
+  Error er = TuningOperands(S, tiles, bt_bytes, &bt);
   if (!er.ok()) return er;
   er = part.alloc(part_words * 8);
   if (!er.ok()) return er;
-  // Operands with realistic bit patterns: the MFMA clock depends on the data
-  // (all-zero operands run faster than query digits), so the B tiles are
-  // filled with bytes of the resident matrix.
   hipStream_t st = S->main.stream;
-  const size_t fill = std::min(bt_bytes, tiles.bytes);
-  PIRG_HIP(hipMemcpyAsync(bt.p, tiles.p, fill, hipMemcpyDeviceToDevice, st));
-  if (fill < bt_bytes) {
-    PIRG_HIP(hipMemsetAsync(static_cast<char*>(bt.p) + fill, 0x5b, bt_bytes - fill, st));
-  }
   double best = 1e30;
   ScanPlan best_plan = cands.front();
   char line[160];
@@ -1442,6 +1493,7 @@ Error TuneOne(pirg_server* S, bool db, int nq, std::string* report) {
     ms /= reps;
     std::snprintf(line, sizeof(line), " [%s %.3f ms]", DescribePlan(c).c_str(), ms);
     *report += line;
+    timed->push_back({c, ms});
     if (ms < best) {
       best = ms;
       best_plan = c;
@@ -1450,6 +1502,67 @@ Error TuneOne(pirg_server* S, bool db, int nq, std::string* report) {
   std::snprintf(line, sizeof(line), " -> %s\n", DescribePlan(best_plan).c_str());
   *report += line;
   (db ? S->tuned_db : S->tuned_pk)[nq] = best_plan;
+  return Ok();
+}
+
+// After both products are tuned: times the fastest pair of candidates that
+// LaunchScanPair fuses as one launch, and keeps the pair and fusion when that
+// launch beats the fastest two separate ones (the pack product then fills
+// the CUs the DB product's last round leaves idle).
+Error TunePair(pirg_server* S, int nq, const std::vector<TimedPlan>& tdb,
+               const std::vector<TimedPlan>& tpk, std::string* report) {
+  S->tuned_fuse[nq] = false;
+  double best_db = 1e30, best_pk = 1e30;
+  for (const TimedPlan& x : tdb) best_db = std::min(best_db, x.ms);
+  for (const TimedPlan& x : tpk) best_pk = std::min(best_pk, x.ms);
+  const TimedPlan* fd = nullptr;
+  const TimedPlan* fp = nullptr;
+  for (const TimedPlan& x : tdb) {
+    for (const TimedPlan& y : tpk) {
+      if (!ScanPairFusable(S->dbL, x.plan, S->pkL, y.plan) ||
+          ScanWorkgroups(S->dbL, x.plan) > 4LL * S->cus) {
+        continue;
+      }
+      if (fd == nullptr || x.ms + y.ms < fd->ms + fp->ms) {
+        fd = &x;
+        fp = &y;
+      }
+    }
+  }
+  if (!S->fuse_scans || fd == nullptr) return Ok();
+  DevBuf bt_db, bt_pk, part_db, part_pk;
+  Error er = TuningOperands(S, S->db_tiles, fd->plan.btile_bytes, &bt_db);
+  if (er.ok()) er = TuningOperands(S, S->pk_tiles, fp->plan.btile_bytes, &bt_pk);
+  if (er.ok()) er = part_db.alloc(static_cast<size_t>(fd->plan.ksplit) * nq * S->dbL.rows * 8);
+  if (er.ok()) er = part_pk.alloc(static_cast<size_t>(fp->plan.ksplit) * nq * S->pkL.rows * 8);
+  if (!er.ok()) return er;
+  hipStream_t st = S->main.stream;
+  auto launch = [&]() {
+    return LaunchScanPair(S->db_tiles.as<int8_t>(), S->dbL, fd->plan, bt_db.as<int8_t>(),
+                          part_db.as<u64>(), S->pk_tiles.as<int8_t>(), S->pkL, fp->plan,
+                          bt_pk.as<int8_t>(), part_pk.as<u64>(), st);
+  };
+  PIRG_HIP(launch());
+  PIRG_HIP(hipEventRecord(S->main.kev[0], st));
+  const int reps = 3;
+  for (int r = 0; r < reps; ++r) PIRG_HIP(launch());
+  PIRG_HIP(hipEventRecord(S->main.kev[1], st));
+  PIRG_HIP(hipEventSynchronize(S->main.kev[1]));
+  float ms = 0;
+  PIRG_HIP(hipEventElapsedTime(&ms, S->main.kev[0], S->main.kev[1]));
+  ms /= reps;
+  const bool fuse = ms < best_db + best_pk;
+  char line[200];
+  std::snprintf(line, sizeof(line),
+                "pair nq=%d: [one launch %s + %s %.3f ms] [two launches %.3f ms] -> %s\n", nq,
+                DescribePlan(fd->plan).c_str(), DescribePlan(fp->plan).c_str(), ms,
+                best_db + best_pk, fuse ? "one launch" : "two launches");
+  *report += line;
+  if (fuse) {
+    S->tuned_db[nq] = fd->plan;
+    S->tuned_pk[nq] = fp->plan;
+    S->tuned_fuse[nq] = true;
+  }
   return Ok();
 }
 
@@ -1468,10 +1581,11 @@ int pirg_server_autotune(pirg_server* S, const int* widths, int nwidths, char* r
   std::string rep;
   for (int i = 0; i < nwidths; ++i) {
     if (widths[i] < 1) return PIRG_INVALID_ARGUMENT;
-    for (bool db : {true, false}) {
-      Error er = TuneOne(S, db, widths[i], &rep);
-      if (!er.ok()) return Report(er, report, report_len);
-    }
+    std::vector<TimedPlan> tdb, tpk;
+    Error er = TuneOne(S, true, widths[i], &rep, &tdb);
+    if (er.ok()) er = TuneOne(S, false, widths[i], &rep, &tpk);
+    if (er.ok()) er = TunePair(S, widths[i], tdb, tpk, &rep);
+    if (!er.ok()) return Report(er, report, report_len);
   }
   // Plans changed: graphs and workspace sizes follow on the next batch.
   (void)hipStreamSynchronize(S->main.stream);
@@ -1678,6 +1792,12 @@ int pirg_server_scan_times(pirg_server* S, double* db_scan_ms, double* pack_scan
     S->db_scan_ms = S->pk_scan_ms = 0;
     S->scans = 0;
   }
+  return 0;
+}
+
+int pirg_server_scans_fused(pirg_server* S, int batch, int* fused) {
+  if (S == nullptr || fused == nullptr || batch < 1) return PIRG_INVALID_ARGUMENT;
+  *fused = FuseScans(S, batch, PlanFor(S, true, batch), PlanFor(S, false, batch)) ? 1 : 0;
   return 0;
 }
 

@@ -154,6 +154,7 @@ class ServeLoop;
 struct pirg_server {
   pirg_params P{};
   int device = 0;
+  int cus = 0;  // compute units of the device
   int d = 0, log_d = 0, N = 0, log_N = 0, t = 0, ell = 0, k = 0, e = 0;
   int r = 0, K = 0, M = 0, log_q = 0;
   pirg::u64 qmask = 0;
@@ -179,8 +180,13 @@ struct pirg_server {
   // Batches with enough requests split EvalPoly's Horner chain into two
   // concurrent halves.
   bool horner_split = true;
+  // MFMA-bound batches run the DB and pack scans as one launch when their
+  // plans allow (LaunchScanPair).
+  bool fuse_scans = true;
   // Scan configurations chosen by pirg_server_autotune, per batch size.
   std::map<int, pirg::ScanPlan> tuned_db, tuned_pk;
+  // Per tuned batch size: whether the DB and pack scans run as one launch.
+  std::map<int, bool> tuned_fuse;
   pirg::DevBuf db_tiles, pk_tiles, a_tilde, t_spec, sda_spec, tw_fwd, tw_bwd, margin;
   // t_vec_h ((ell*t) x k x d) and second_dim_a (2e x d, when t > 1) as given,
   // for SaveToFile (the kernels read their spectra).

@@ -170,26 +170,32 @@ def test_batch_equals_sequential_with_invalid_request():
                 np.testing.assert_array_equal(single, want)
 
 
-@pytest.mark.parametrize("nq,wide_min,digit_major,tall", [
-    (1, 10, 1, 1), (2, 10, 1, 1), (3, 10, 1, 1), (9, 10, 1, 1), (1, 1, 1, 1), (3, 1, 1, 1),
-    (16, 10, 1, 1), (33, 10, 0, 1), (33, 10, 1, 1), (33, 10, 1, 0), (64, 10, 0, 1),
-    (64, 10, 1, 1), (64, 10, 1, 0), (70, 10, 0, 1), (70, 10, 1, 1), (70, 10, 1, 0),
-    (129, 10, 1, 1), (129, 10, 1, 0)])
-def test_wide_batches_match_oracle(nq, wide_min, digit_major, tall, monkeypatch):
+@pytest.mark.parametrize("nq,wide_min,digit_major,tall,fuse", [
+    (1, 10, 1, 1, 1), (2, 10, 1, 1, 1), (3, 10, 1, 1, 1), (9, 10, 1, 1, 1), (1, 1, 1, 1, 1),
+    (3, 1, 1, 1, 1), (16, 10, 1, 1, 1), (33, 10, 0, 1, 1), (33, 10, 1, 1, 1), (33, 10, 1, 0, 1),
+    (64, 10, 0, 1, 1), (64, 10, 1, 1, 1), (64, 10, 1, 1, 0), (64, 10, 1, 0, 1), (70, 10, 0, 1, 1),
+    (70, 10, 1, 1, 1), (70, 10, 1, 0, 1), (129, 10, 1, 1, 1), (129, 10, 1, 1, 0),
+    (129, 10, 1, 0, 1)])
+def test_wide_batches_match_oracle(nq, wide_min, digit_major, tall, fuse, monkeypatch):
     # Every scan configuration: NarrowScan with 1, 2 and 4 tiles per group,
     # WideScan with 2, 4 and 7 tiles per wave in the query-major B layout and
     # with 7 digit tiles in the digit-major one (8 strips per workgroup, or
     # the tall kernel's 12, which here leaves the last row block a third
     # full: 1024 rows = 5 1/3 blocks), several workgroup column blocks (70 and
-    # 129 queries) and partly filled groups.
+    # 129 queries) and partly filled groups. With the tall kernel the DB and
+    # pack scans run as one launch (fuse = 1) or as two; from 22 requests on,
+    # EvalPoly runs in two halves.
     monkeypatch.setenv("PIRG_WIDE_MIN", str(wide_min))
     monkeypatch.setenv("PIRG_DIGIT_MAJOR", str(digit_major))
     monkeypatch.setenv("PIRG_TALL", str(tall))
+    monkeypatch.setenv("PIRG_FUSE_SCANS", str(fuse))
     P = O.metrics_tool_params(n=4096, log_d=9, t=2)
     mat = O.synthetic_material(P, seed=nq)
     reqs = [O.synthetic_request(P, seed=1000 * nq + i) for i in range(nq)]
     with server_from_material(P, mat) as srv:
         batch, statuses, _ = srv.process_batch([gpu_request(r) for r in reqs])
+        if nq >= 33:
+            assert srv.scans_fused(nq) == bool(fuse and digit_major and tall)
     assert statuses == [0] * nq
     for i in range(0, nq, max(1, nq // 5)):
         want, _, _ = O.process_response(P, mat, reqs[i])
@@ -298,7 +304,11 @@ def test_autotuned_plans_match_oracle():
     reqs = [O.synthetic_request(P, seed=4040 + i) for i in range(20)]
     with server_from_material(P, mat) as srv:
         report = srv.autotune([1, 5, 20])
-        assert report.count("->") == 6, report
+        lines = report.splitlines()
+        assert sum(1 for ln in lines if ln.startswith(("db ", "pack "))) == 6, report
+        # Widths whose plans could share one scan launch get a timed pair line.
+        assert all(ln.endswith(("-> one launch", "-> two launches"))
+                   for ln in lines if ln.startswith("pair ")), report
         for n in (1, 5, 7, 20):
             got, st, _ = srv.process_batch([gpu_request(r) for r in reqs[:n]])
             assert st == [0] * n
