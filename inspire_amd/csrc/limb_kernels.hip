@@ -250,6 +250,95 @@ __global__ void QueryDigitsDigitMajorKernel(const u64* __restrict__ vec, int nq,
   }
 }
 
+// Digit-major B tiles and the column sums of the same query values in one
+// pass. A workgroup owns one group of 16 queries over a chunk of k-blocks,
+// each wave one k-block at a time: the wave reads the 16 queries' 64 values
+// of the k-block row by row (512 contiguous bytes per load), transposes them
+// through its LDS region so that lane l holds query l % 16 at k's
+// 16 * (l / 16) .. +15 (the B operand layout), and emits the ns balanced
+// digit tiles as QueryDigitsDigitMajorKernel does. The lanes' sums of their
+// values reduce, per query, into part[q * nchunks + chunk] (raw sums mod
+// 2^64; VectorSumFinalKernel multiplies by C_off).
+constexpr int kDigitSumWaves = 4;
+constexpr int kDigitSumRow = 65;  // u64 per LDS row: 64 values + 1 of padding
+
+__global__ __launch_bounds__(256) void QueryDigitsSumsKernel(
+    const u64* __restrict__ vec, int nq, int stride, int len, int kblocks, int ns, int ngp,
+    int kb_per_chunk, int nchunks, int8_t* __restrict__ btiles, u64* __restrict__ part) {
+  __shared__ u64 rows[kDigitSumWaves][16 * kDigitSumRow];
+  __shared__ u64 qsum[kDigitSumWaves][16];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int chunk = blockIdx.x;
+  const int grp = blockIdx.y;
+  const int kb_lo = chunk * kb_per_chunk;
+  const int kb_hi = min(kblocks, kb_lo + kb_per_chunk);
+  const int qi = lane & 15;
+  const int kc = lane >> 4;
+  u64* const mine = rows[w];
+  u64 acc = 0;
+  for (int kb = kb_lo + w; kb < kb_hi; kb += kDigitSumWaves) {
+    const int k = kb * 64 + lane;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int q = grp * 16 + i;
+      mine[i * kDigitSumRow + lane] = (q < nq && k < len) ? vec[(long long)q * stride + k] : 0ULL;
+    }
+    // The wave's own LDS region: its stores are visible to its loads after
+    // the wave-wide fence.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    u64 v[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      v[b] = mine[qi * kDigitSumRow + kc * 16 + b];
+      acc += v[b];
+    }
+    // Every lane has read the region before the next k-block overwrites it.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int8_t* dst = btiles + ((((long long)kb * ngp + grp) * ns) * 64 + lane) * 16;
+    for (int sft = 0; sft < ns; ++sft) {
+      int8_t out[16];
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const unsigned byte = static_cast<unsigned>(v[b] & 255u);
+        const bool hi = byte >= 128u;
+        out[b] = static_cast<int8_t>(hi ? static_cast<int>(byte) - 256 : static_cast<int>(byte));
+        v[b] = (v[b] >> 8) + (hi ? 1 : 0);
+      }
+      *reinterpret_cast<int4*>(dst + (long long)sft * 64 * 16) =
+          *reinterpret_cast<const int4*>(out);
+    }
+  }
+  // Lanes qi, qi + 16, qi + 32, qi + 48 hold query qi's partial sums.
+  acc += __shfl_xor(acc, 16);
+  acc += __shfl_xor(acc, 32);
+  if (lane < 16) qsum[w][lane] = acc;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int q = grp * 16 + threadIdx.x;
+    u64 t = 0;
+#pragma unroll
+    for (int ww = 0; ww < kDigitSumWaves; ++ww) t += qsum[ww][threadIdx.x];
+    if (q < nq) part[(long long)q * nchunks + chunk] = t;
+  }
+}
+
+// corr[q] = c_off * sum of query q's nchunks partial sums: one wave per query.
+__global__ __launch_bounds__(64) void ChunkSumFinalKernel(const u64* __restrict__ part,
+                                                          int nchunks, u64 c_off,
+                                                          u64* __restrict__ corr) {
+  const int q = blockIdx.x;
+  u64 t = 0;
+  for (int c = threadIdx.x; c < nchunks; c += 64) t += part[(long long)q * nchunks + c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+  if (threadIdx.x == 0) corr[q] = c_off * t;
+}
+
 // corr[q] = c_off * sum_k vec[q][k] mod 2^64, in two passes: partial sums of
 // chunks, then one thread per query.
 __global__ void VectorPartialSumKernel(const u64* __restrict__ vec, int stride,
@@ -1377,6 +1466,32 @@ hipError_t LaunchQueryDigits(const u64* vec, int stride, int len, const LimbLayo
 }
 
 int VectorSumScratchWords(int nq) { return nq * kVectorSumChunks; }
+
+int DigitSumChunks(const LimbLayout& L) {
+  return (L.kblocks + 1 + kDigitSumWaves - 1) / kDigitSumWaves;
+}
+
+hipError_t LaunchQueryDigitsSums(const u64* vec, int stride, int len, const LimbLayout& L,
+                                 const ScanPlan& P, int8_t* btiles, u64 c_off, u64* scratch,
+                                 u64* corr, hipStream_t s) {
+  if (!P.digit_major) {
+    hipError_t e = LaunchQueryDigits(vec, stride, len, L, P, btiles, s);
+    if (e != hipSuccess) return e;
+    return LaunchVectorSums(vec, P.nq, stride, len, c_off, scratch, corr, s);
+  }
+  // kblocks + 1 blocks (the last one is all zero), one per wave: the
+  // scratch holds nq x DigitSumChunks(L) sums.
+  const int kbt = L.kblocks + 1;
+  const int nchunks = DigitSumChunks(L);
+  hipLaunchKernelGGL(QueryDigitsSumsKernel, dim3(nchunks, P.ngp), dim3(64 * kDigitSumWaves), 0, s,
+                     vec, P.nq, stride, len, kbt, P.ns, P.ngp, kDigitSumWaves, nchunks, btiles,
+                     scratch);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(ChunkSumFinalKernel, dim3(P.nq), dim3(64), 0, s, scratch, nchunks, c_off,
+                     corr);
+  return hipGetLastError();
+}
 
 hipError_t LaunchVectorSums(const u64* vec, int nq, int stride, int len, u64 c_off,
                             u64* scratch, u64* corr, hipStream_t s) {

@@ -175,6 +175,13 @@ constexpr int kVectorSumChunks = 64;
 int VectorSumScratchWords(int nq);
 hipError_t LaunchVectorSums(const u64* vec, int nq, int stride, int len, u64 c_off,
                             u64* scratch, u64* corr, hipStream_t s);
+// LaunchQueryDigits and LaunchVectorSums of the same vectors (vec rows of
+// P.nq queries); digit-major plans read the values once for both. scratch:
+// P.nq * max(kVectorSumChunks, DigitSumChunks(L)) words.
+int DigitSumChunks(const LimbLayout& L);
+hipError_t LaunchQueryDigitsSums(const u64* vec, int stride, int len, const LimbLayout& L,
+                                 const ScanPlan& P, int8_t* btiles, u64 c_off, u64* scratch,
+                                 u64* corr, hipStream_t s);
 // partial[ks][q][row] = sum over the ks-th k-range of the limb product,
 // combined over (plane, digit) pairs with shift < 8 * ns.
 hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,

@@ -479,7 +479,11 @@ std::vector<size_t> WorkspaceSizes(const pirg_server* S, int cap) {
     part_db = std::max(part_db, static_cast<size_t>(pd.ksplit) * n);
     part_pk = std::max(part_pk, static_cast<size_t>(pp.ksplit) * n);
   }
-  const size_t corr_words = nq + VectorSumScratchWords(cap);
+  // Column-sum scratch of LaunchQueryDigitsSums for each product's layout.
+  const size_t corr_db_words =
+      nq + static_cast<size_t>(cap) * std::max(kVectorSumChunks, DigitSumChunks(S->dbL));
+  const size_t corr_pk_words =
+      nq + static_cast<size_t>(cap) * std::max(kVectorSumChunks, DigitSumChunks(S->pkL));
   const int nv_f = S->fin_v.num_chunks;
   const int nu_e = S->t > 1 ? S->ext_u.num_chunks : 0;
   const int nv_e = S->t > 1 ? S->ext_v.num_chunks : 0;
@@ -487,8 +491,8 @@ std::vector<size_t> WorkspaceSizes(const pirg_server* S, int cap) {
       nq * RequestWords(S) * 8,
       bt_db,
       bt_pk,
-      corr_words * 8,
-      corr_words * 8,
+      corr_db_words * 8,
+      corr_pk_words * 8,
       part_db * S->M * 8,
       part_pk * S->M * 8,
       nq * S->k * nv_f * N * sizeof(double2),
@@ -830,9 +834,9 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
   // Pack product (pir_server.cc:237-248), y = concat(y_vec_g): its query
   // digits, then the scan.
   auto pack_digits = [&](hipStream_t s) -> Error {
-    PIRG_HIP(LaunchQueryDigits(rec_pk, rw, S->k * d, S->pkL, pp, W.bt_pk.as<int8_t>(), s));
-    PIRG_HIP(LaunchVectorSums(rec_pk, nq, rw, S->k * d, S->pkL.c_off,
-                              W.corr_pk.as<u64>() + nq, W.corr_pk.as<u64>(), s));
+    PIRG_HIP(LaunchQueryDigitsSums(rec_pk, rw, S->k * d, S->pkL, pp, W.bt_pk.as<int8_t>(),
+                                   S->pkL.c_off, W.corr_pk.as<u64>() + nq, W.corr_pk.as<u64>(),
+                                   s));
     return Ok();
   };
   auto pack_scan = [&]() -> Error {
@@ -882,9 +886,9 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
     if (!er.ok()) return er;
     PIRG_HIP(hipEventRecord(slot.join_ev, slot.side));
   }
-  PIRG_HIP(LaunchQueryDigits(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
-  PIRG_HIP(LaunchVectorSums(rec_fd, nq, rw, S->K, S->dbL.c_off,
-                            W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(), st));
+  PIRG_HIP(LaunchQueryDigitsSums(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(),
+                                 S->dbL.c_off, W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(),
+                                 st));
   // Two MFMA-bound scans of the kinds LaunchScanPair fuses run as one launch
   // (the pack product's query digits first) when the DB product takes few
   // rounds of workgroups: the pack product's then fill the CUs that its last,
