@@ -900,32 +900,64 @@ __global__ __launch_bounds__(512) void WideScanPipeKernel(
                                  (long long)(strip0 + h * RW) * 16, ks, partial);
 }
 
-// The tall digit-major scan (NP = 2, 7 digit tiles): each wave owns 6 strips (a workgroup 12
-// strips, 192 rows) by one 16-query group, so a workgroup moves 52 KiB of
-// operands per k-block for 624 MFMAs where WideScanPipeKernel moves 44 KiB for
-// 416, and each wave reads 19 fragments from LDS per 78 MFMAs instead of 15
-// per 52. The 168 accumulators leave no room for a second set of A
-// fragments, so A is streamed one MFMA block ahead (one fragment per block)
-// while B is read a whole step ahead. The A ring leads by 2 steps and the B
-// ring by 3, with one barrier per step, before its last block.
-template <int CT>
-struct TallShape {
-  static constexpr int NP = 2, RS = 12, RW = 6, AF = RS * NP / 8;
+// The streaming digit-major scans. The tall one (NP = 2, 7 digit tiles): each
+// wave owns 6 strips (a workgroup 12 strips, 192 rows) by one 16-query group,
+// so a workgroup moves 52 KiB of operands per k-block for 624 MFMAs where
+// WideScanPipeKernel moves 44 KiB for 416, and each wave reads 19 fragments
+// from LDS per 78 MFMAs instead of 15 per 52. The 168 accumulators leave no
+// room for a second set of A fragments, so A is streamed one MFMA block ahead
+// (one fragment per block) while B is read a whole step ahead, with one
+// barrier per step, before its last block. Rings of SA and SB stages give
+// leads of SA - 1 and SB - 1 steps. The same schedule serves four-plane
+// matrices (NP = 4, 22 MFMAs per strip and group), whose 4 KiB A strips leave
+// LDS for two or three A stages.
+template <int NP_, int CT, int RW_, int SA_, int SB_>
+struct StreamShape {
+  static constexpr int NP = NP_, RW = RW_, RS = 2 * RW_, AF = RS * NP / 8;
+  static_assert((RS * NP) % 8 == 0, "A fragments split evenly over 8 waves");
   static constexpr int BH0 = (CT + 1) / 2, BH1 = CT - BH0;
   static constexpr int kA = RS * NP * 64;  // v4i of A per stage
   static constexpr int kB = 4 * CT * 64;   // v4i of B per stage
-  static constexpr int SA = 3, SB = 3;
+  static constexpr int SA = SA_, SB = SB_;
+  // Within a step the ring with the shorter lead issues first, so that the
+  // DMAs still allowed in flight at the step's barrier are the latest issued.
+  static constexpr bool kBFirst = SB_ < SA_;
+  static_assert((kBFirst && SB_ == 2) || (!kBFirst && SA_ == SB_), "in-order vmcnt waits");
   static constexpr size_t kRing = static_cast<size_t>(SA * kA + SB * kB) * 16;
   static constexpr size_t kRed = static_cast<size_t>(8) * RW * 16 * kQMax * 8;
   static constexpr size_t kLdsBytes = kRing > kRed ? kRing : kRed;
 };
 
 template <int CT>
+using TallShape = StreamShape<2, CT, 6, 3, 3>;
+
+// vmcnt of the prologue's wait: the DMAs issued after the last of A(0), B(0)
+// and B(1) (the operands step 0 reads) may stay in flight. Issue order: for
+// s = 0, 1, ...: A(s) (s < SA - 1) and B(s) (s < SB), B first when kBFirst.
+template <typename W>
+constexpr int PrologueVmcnt(int bh) {
+  int total = 0, needed_end = 0;
+  const int steps = (W::SA - 1 > W::SB) ? W::SA - 1 : W::SB;
+  for (int st = 0; st < steps; ++st) {
+    for (int part = 0; part < 2; ++part) {
+      const bool is_b = W::kBFirst ? part == 0 : part == 1;
+      if (is_b && st < W::SB) {
+        total += bh;
+        if (st <= 1) needed_end = total;
+      } else if (!is_b && st < W::SA - 1) {
+        total += W::AF;
+        if (st == 0) needed_end = total;
+      }
+    }
+  }
+  return total - needed_end;
+}
+
+template <int CT, typename W = TallShape<CT>>
 __device__ __forceinline__ void WideScanTallBody(
     int w, const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps,
     int ksplit, const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial,
     v4i* lds) {
-  using W = TallShape<CT>;
   constexpr int NP = W::NP, RS = W::RS, RW = W::RW, AF = W::AF, SA = W::SA, SB = W::SB;
   const int nx = (strips + RS - 1) / RS;
   WideTile wt;
@@ -994,19 +1026,20 @@ __device__ __forceinline__ void WideScanTallBody(
     }
   };
   // Step i: B fragments of step i in cb, its first A fragment in a_in. Issues
-  // A(i + 2) and B(i + 3) (A first), reads step i + 1's B into nb and leaves
-  // its first A fragment in a_in. Issue order: prologue A(0) B(0) A(1) B(1)
-  // B(2), then per step A(i + 2) B(i + 3). Each block's MFMAs are issued after
-  // the read of the next block's A fragment, so that read lands while they
-  // run. The step's one barrier, after the tail of its next-to-last block
-  // (whose read of A(i)'s last fragment it waits for), needs A(i + 1) and
-  // B(i + 2) landed everywhere (in flight only step i's own issues), and
-  // orders every wave's reads of A(i) and B(i + 1) before the next step's
-  // DMAs that refill those slots (A(i + 3) into A(i)'s, B(i + 4) into
-  // B(i + 1)'s). The read of A(i + 1)'s first fragment after it then lands
-  // under the last block's MFMAs. Measured at 256 queries: 2% faster than
-  // reading each fragment after the previous block's MFMAs with the barrier
-  // after the last block, with identical partial sums.
+  // A(i + SA - 1) and B(i + SB), reads step i + 1's B into nb and leaves its
+  // first A fragment in a_in (the tall shape, SA = SB = 3: prologue A(0) B(0)
+  // A(1) B(1) B(2), then per step A(i + 2) B(i + 3)). Each block's MFMAs are
+  // issued after the read of the next block's A fragment, so that read lands
+  // while they run. The step's one barrier, after the tail of its
+  // next-to-last block (whose read of A(i)'s last fragment it waits for),
+  // needs A(i + 1) and B(i + 2) landed everywhere (in flight only issues of
+  // later steps), and orders every wave's reads of A(i) and B(i + 1) before
+  // the next steps' DMAs that refill those slots. The read of A(i + 1)'s
+  // first fragment after it then lands under the last block's MFMAs.
+  // Measured at 256 queries: 2% faster than reading each fragment after the
+  // previous block's MFMAs with the barrier after the last block, with
+  // identical partial sums, and the query tiles' HBM traffic fell from about
+  // 20 to 15 GB per dispatch.
   constexpr int kBar = kBlk - 2;
   static_assert(kBar >= kDma - 1 && kBar >= CT - 1,
                 "the barrier follows the step's DMA issues and B reads");
@@ -1024,15 +1057,23 @@ __device__ __forceinline__ void WideScanTallBody(
         acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur, cb[c - p], acc[r][c], 0, 0, 0);
       __builtin_amdgcn_sched_barrier(0);
       if (blk < kDma) {
-        if (blk < AF) {
-          dma_a(i + 2, blk);
+        if constexpr (W::kBFirst) {
+          if (blk < W::BH0) {
+            dma_b(i + SB, blk);
+          } else {
+            dma_a(i + SA - 1, blk - W::BH0);
+          }
         } else {
-          dma_b(i + 3, blk - AF);
+          if (blk < AF) {
+            dma_a(i + SA - 1, blk);
+          } else {
+            dma_b(i + SB, blk - AF);
+          }
         }
       }
       if (blk < CT) nb[blk] = read_b(i + 1, blk);
       if (blk == kBar) {
-        wait_own(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+        wait_own(std::integral_constant<int, SA - 2>{}, std::integral_constant<int, SB - 2>{});
         LdsBarrier();
         a_in = read_a(i + 1, 0);
       }
@@ -1040,23 +1081,32 @@ __device__ __forceinline__ void WideScanTallBody(
     }
   };
   v4i cb0[CT], cb1[CT], a_in;
+  constexpr int kPro = (SA - 1 > SB) ? SA - 1 : SB;
 #pragma unroll
-  for (int j = 0; j < AF; ++j) dma_a(0, j);
+  for (int st = 0; st < kPro; ++st) {
 #pragma unroll
-  for (int cc = 0; cc < W::BH0; ++cc) dma_b(0, cc);
+    for (int part = 0; part < 2; ++part) {
+      const bool is_b = W::kBFirst ? part == 0 : part == 1;
+      if (is_b && st < SB) {
 #pragma unroll
-  for (int j = 0; j < AF; ++j) dma_a(1, j);
+        for (int cc = 0; cc < W::BH0; ++cc) dma_b(st, cc);
+      } else if (!is_b && st < SA - 1) {
 #pragma unroll
-  for (int cc = 0; cc < W::BH0; ++cc) dma_b(1, cc);
-#pragma unroll
-  for (int cc = 0; cc < W::BH0; ++cc) dma_b(2, cc);
-  // A(0), B(0) and B(1) have landed everywhere (only B(2) may be in flight).
-  wait_own(std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{});
+        for (int j = 0; j < AF; ++j) dma_a(st, j);
+      }
+    }
+  }
+  // A(0), B(0) and B(1) have landed everywhere.
+  if (h == 0) {
+    WaitVmcnt<PrologueVmcnt<W>(W::BH0)>();
+  } else {
+    WaitVmcnt<PrologueVmcnt<W>(W::BH1)>();
+  }
   LdsBarrier();
 #pragma unroll
   for (int c = 0; c < CT; ++c) cb0[c] = read_b(0, c);
   a_in = read_a(0, 0);
-  // Every wave has read B(0) before step 0's B(3) DMA refills its slot.
+  // Every wave has read B(0) before step 0's B(SB) DMA refills its slot.
   LdsBarrier();
   const int nkb2 = (nkb + 1) & ~1;
   for (int i = 0; i < nkb2; i += 2) {
@@ -1077,6 +1127,17 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
   extern __shared__ v4i lds[];
   WideScanTallBody<CT>(blockIdx.x, tiles, strips, rows, kblocks, kbps, ksplit, btiles, ngp, nq,
                        partial, lds);
+}
+
+// The streaming schedule on four-plane matrices (shape W).
+template <int NP, int CT, int RW, int SA, int SB>
+__global__ __launch_bounds__(512) void WideScanStreamKernel(
+    const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
+    const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial) {
+  extern __shared__ v4i lds[];
+  WideScanTallBody<CT, StreamShape<NP, CT, RW, SA, SB>>(blockIdx.x, tiles, strips, rows, kblocks,
+                                                        kbps, ksplit, btiles, ngp, nq, partial,
+                                                        lds);
 }
 
 // One operand of a fused scan launch.
@@ -1174,6 +1235,21 @@ hipError_t LaunchWideTallT(const int8_t* tiles, const LimbLayout& L, const ScanP
   return hipGetLastError();
 }
 
+template <int NP, int CT, int RW, int SA, int SB>
+hipError_t LaunchWideStreamT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
+                             const int8_t* btiles, u64* partial, hipStream_t s) {
+  using W = StreamShape<NP, CT, RW, SA, SB>;
+  static_assert(W::kLdsBytes <= 160 * 1024, "WideScanStream LDS");
+  const long long ntiles = static_cast<long long>((L.strips + W::RS - 1) / W::RS) * P.ksplit;
+  const long long blocks = (ntiles + 7) / 8 * 8 * (P.ngp / 4);
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((WideScanStreamKernel<NP, CT, RW, SA, SB>),
+                     dim3(static_cast<unsigned>(blocks)), dim3(512), W::kLdsBytes, s, tiles,
+                     L.strips, L.rows, L.kblocks, P.kb_per_split, P.ksplit, btiles, P.ngp, P.nq,
+                     partial);
+  return hipGetLastError();
+}
+
 template <int NP>
 hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                         const int8_t* btiles, u64* partial, hipStream_t s) {
@@ -1198,6 +1274,10 @@ hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan
       if (!P.digit_major) return LaunchWideT<NP, 7, false>(tiles, L, P, btiles, partial, s);
       if constexpr (NP == 2) {
         if (P.tall) return LaunchWideTallT<7>(tiles, L, P, btiles, partial, s);
+      }
+      if constexpr (NP == 4) {
+        if (P.tall == 1) return LaunchWideStreamT<4, 7, 5, 2, 2>(tiles, L, P, btiles, partial, s);
+        if (P.tall == 2) return LaunchWideStreamT<4, 7, 4, 3, 2>(tiles, L, P, btiles, partial, s);
       }
       // Two register sets of fragments fit beside the accumulators for NP <= 2.
       if constexpr (NP <= 2) {
@@ -1289,6 +1369,14 @@ hipError_t ConfigureWideTall() {
                              static_cast<int>(TallShape<CT>::kLdsBytes));
 }
 
+template <int NP, int CT, int RW, int SA, int SB>
+hipError_t ConfigureWideStream() {
+  return hipFuncSetAttribute(
+      reinterpret_cast<const void*>(WideScanStreamKernel<NP, CT, RW, SA, SB>),
+      hipFuncAttributeMaxDynamicSharedMemorySize,
+      static_cast<int>(StreamShape<NP, CT, RW, SA, SB>::kLdsBytes));
+}
+
 template <int NP, int CT>
 hipError_t ConfigureWidePipe() {
   return hipFuncSetAttribute(reinterpret_cast<const void*>(WideScanPipeKernel<NP, CT>),
@@ -1305,7 +1393,8 @@ hipError_t ConfigureScanKernels() {
                            ConfigureWide<4, 7, true>(), ConfigureWidePipe<1, 7>(),
                            ConfigureWidePipe<2, 7>(), ConfigureWide<4, 4, true>(),
                            ConfigureWidePipe<1, 4>(), ConfigureWidePipe<2, 4>(),
-                           ConfigureWideTall<7>(),
+                           ConfigureWideTall<7>(), ConfigureWideStream<4, 7, 5, 2, 2>(),
+                           ConfigureWideStream<4, 7, 4, 3, 2>(),
                            hipFuncSetAttribute(reinterpret_cast<const void*>(DbPackScanKernel),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                static_cast<int>(kFusedScanLds))};
@@ -1333,7 +1422,8 @@ void CompletePlan(const LimbLayout& L, int ks_scale, ScanPlan* P) {
   // K-split: exact int32 sums need at most 65536 / planes columns per split;
   // beyond that, enough workgroups to occupy every CU a few times over.
   const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
-  const int wide_rows = P->tall ? 12 : 8;  // strips per wide workgroup
+  // Strips per wide workgroup (the streaming shapes of LaunchScanNP).
+  const int wide_rows = P->tall == 1 ? (L.planes == 4 ? 10 : 12) : 8;
   const int row_wgs = P->wide ? (L.strips + wide_rows - 1) / wide_rows
                               : (L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR);
   const int z = P->wide ? P->ngp / 4 : P->ngp;
@@ -1384,12 +1474,16 @@ std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bi
   if (DigitMajorShifts(ns) && nq > 16) {
     add(1, ns, 1, 0);
     if (L.planes == 2 && ns == 7) add(1, ns, 1, 1);
+    if (L.planes == 4 && ns == 7) {
+      add(1, ns, 1, 1);
+      add(1, ns, 1, 2);
+    }
   }
   return out;
 }
 
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries,
-                      bool allow_digit_major, bool allow_tall) {
+                      bool allow_digit_major, bool allow_tall, int four_plane_stream) {
   ScanPlan P{};
   P.nq = nq;
   P.ns = (out_bits + 7) / 8;
@@ -1412,7 +1506,9 @@ ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_qu
     P.wide = 1;
     P.ctg = ns;
     P.digit_major = 1;
-    P.tall = allow_tall && L.planes == 2 && ns == 7 ? 1 : 0;
+    P.tall = ns != 7 ? 0
+             : L.planes == 2 ? (allow_tall ? 1 : 0)
+             : L.planes == 4 ? four_plane_stream : 0;
   } else {
     P.wide = 1;
     P.ctg = kWideTiles[2];
@@ -1528,7 +1624,8 @@ hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& 
 
 bool ScanPairFusable(const LimbLayout& Ld, const ScanPlan& Pd, const LimbLayout& Lp,
                      const ScanPlan& Pp) {
-  return Pd.nq > 0 && Pp.nq > 0 && Pd.wide && Pd.digit_major && Pd.tall && Pd.ctg == 7 &&
+  return Pd.nq > 0 && Pp.nq > 0 && Pd.wide && Pd.digit_major && Pd.tall == 1 && Pp.tall == 0 &&
+         Pd.ctg == 7 &&
          Pd.ns == 7 && Pd.qpg == 16 && Pd.ngp % 4 == 0 && Pd.group_rows == 0 &&
          Ld.planes == 2 && Pp.wide && Pp.digit_major && Pp.ctg == 7 && Pp.ns == 7 &&
          Pp.qpg == 16 && Pp.ngp % 4 == 0 && Pp.group_rows == 0 && Lp.planes == 4;

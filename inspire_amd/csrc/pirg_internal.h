@@ -139,8 +139,12 @@ struct ScanPlan {
   // digit s of its 16 queries, so plane p's operand for output tile s is
   // plane 0's tile s - p, unchanged.
   int digit_major;
-  // Digit-major with two A planes and 7 digits only: 1 runs the tall kernel
-  // (6 strips per wave, 12 per workgroup) instead of the pipelined one (4 and 8).
+  // Digit-major with 7 digits: 1 runs the streaming schedule with its widest
+  // tile (two A planes: WideScanTallKernel, 6 strips per wave, 12 per
+  // workgroup; four: WideScanStreamKernel, 5 and 10, two-stage rings) and 2,
+  // for four planes, WideScanStreamKernel with 4 strips per wave and a
+  // three-stage A ring; 0 the pipelined (two planes) or plain WideScan kernel
+  // with 4 and 8.
   int tall;
   int ngp;           // groups in the B layout (padded for WideScan)
   int ksplit;        // k-ranges (exact int32 sums and occupancy)
@@ -153,9 +157,12 @@ struct ScanPlan {
 };
 
 // wide_min_queries: batches of at least this many queries use WideScan even
-// when they would fit NarrowScan.
+// when they would fit NarrowScan. allow_tall: digit-major scans of two-plane
+// matrices use the tall kernel; four_plane_stream: the `tall` value of
+// digit-major scans of four-plane matrices.
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries,
-                      bool allow_digit_major = true, bool allow_tall = true);
+                      bool allow_digit_major = true, bool allow_tall = true,
+                      int four_plane_stream = 2);
 // The configurations an autotuner may time for nq queries (kind, tile width,
 // K-split).
 std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits);

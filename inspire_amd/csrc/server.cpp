@@ -458,7 +458,8 @@ ScanPlan PlanFor(const pirg_server* S, bool db, int nq) {
   const auto& tuned = db ? S->tuned_db : S->tuned_pk;
   auto it = tuned.find(nq);
   if (it != tuned.end()) return it->second;
-  return MakeScanPlan(db ? S->dbL : S->pkL, nq, S->log_q, S->wide_min, S->digit_major, S->tall);
+  return MakeScanPlan(db ? S->dbL : S->pkL, nq, S->log_q, S->wide_min, S->digit_major, S->tall,
+                      S->pack_stream);
 }
 
 namespace {
@@ -1208,6 +1209,10 @@ void ApplyPlanOverrides(pirg_server* S) {
   if (const char* env = std::getenv("PIRG_TALL")) S->tall = std::atoi(env) != 0;
   if (const char* env = std::getenv("PIRG_HORNER_SPLIT")) S->horner_split = std::atoi(env) != 0;
   if (const char* env = std::getenv("PIRG_FUSE_SCANS")) S->fuse_scans = std::atoi(env) != 0;
+  if (const char* env = std::getenv("PIRG_PACK_STREAM")) {
+    const int v = std::atoi(env);
+    if (v >= 0 && v <= 2) S->pack_stream = v;
+  }
 }
 
 // Matrix<uint64_t> pack blocks (PirServer<Db, uint64_t, uint64_t>,
@@ -1435,7 +1440,8 @@ namespace {
 std::string DescribePlan(const ScanPlan& P) {
   char buf[128];
   std::snprintf(buf, sizeof(buf), "%s tiles=%d ksplit=%d",
-                P.digit_major ? (P.tall ? "wide-dm-tall" : "wide-dm") : (P.wide ? "wide" : "narrow"),
+                P.digit_major ? (P.tall == 1 ? "wide-dm-tall" : (P.tall == 2 ? "wide-dm-stream" : "wide-dm"))
+                              : (P.wide ? "wide" : "narrow"),
                 P.ctg, P.ksplit);
   return buf;
 }

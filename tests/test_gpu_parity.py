@@ -170,13 +170,16 @@ def test_batch_equals_sequential_with_invalid_request():
                 np.testing.assert_array_equal(single, want)
 
 
-@pytest.mark.parametrize("nq,wide_min,digit_major,tall,fuse", [
-    (1, 10, 1, 1, 1), (2, 10, 1, 1, 1), (3, 10, 1, 1, 1), (9, 10, 1, 1, 1), (1, 1, 1, 1, 1),
-    (3, 1, 1, 1, 1), (16, 10, 1, 1, 1), (33, 10, 0, 1, 1), (33, 10, 1, 1, 1), (33, 10, 1, 0, 1),
-    (64, 10, 0, 1, 1), (64, 10, 1, 1, 1), (64, 10, 1, 1, 0), (64, 10, 1, 0, 1), (70, 10, 0, 1, 1),
-    (70, 10, 1, 1, 1), (70, 10, 1, 0, 1), (129, 10, 1, 1, 1), (129, 10, 1, 1, 0),
-    (129, 10, 1, 0, 1)])
-def test_wide_batches_match_oracle(nq, wide_min, digit_major, tall, fuse, monkeypatch):
+@pytest.mark.parametrize("nq,wide_min,digit_major,tall,fuse,pack_stream", [
+    (1, 10, 1, 1, 1, 0), (2, 10, 1, 1, 1, 0), (3, 10, 1, 1, 1, 0), (9, 10, 1, 1, 1, 0),
+    (1, 1, 1, 1, 1, 0), (3, 1, 1, 1, 1, 0), (16, 10, 1, 1, 1, 0), (33, 10, 0, 1, 1, 0),
+    (33, 10, 1, 1, 1, 0), (33, 10, 1, 0, 1, 0), (33, 10, 1, 1, 1, 1), (33, 10, 1, 1, 1, 2),
+    (64, 10, 0, 1, 1, 0), (64, 10, 1, 1, 1, 0), (64, 10, 1, 1, 0, 0), (64, 10, 1, 0, 1, 0),
+    (64, 10, 1, 1, 1, 1), (64, 10, 1, 1, 1, 2), (70, 10, 0, 1, 1, 0), (70, 10, 1, 1, 1, 0),
+    (70, 10, 1, 0, 1, 0), (70, 10, 1, 1, 1, 1), (70, 10, 1, 1, 1, 2), (129, 10, 1, 1, 1, 0),
+    (129, 10, 1, 1, 0, 0), (129, 10, 1, 0, 1, 0), (129, 10, 1, 1, 1, 1), (129, 10, 1, 1, 1, 2)])
+def test_wide_batches_match_oracle(nq, wide_min, digit_major, tall, fuse, pack_stream,
+                                   monkeypatch):
     # Every scan configuration: NarrowScan with 1, 2 and 4 tiles per group,
     # WideScan with 2, 4 and 7 tiles per wave in the query-major B layout and
     # with 7 digit tiles in the digit-major one (8 strips per workgroup, or
@@ -184,18 +187,22 @@ def test_wide_batches_match_oracle(nq, wide_min, digit_major, tall, fuse, monkey
     # full: 1024 rows = 5 1/3 blocks), several workgroup column blocks (70 and
     # 129 queries) and partly filled groups. With the tall kernel the DB and
     # pack scans run as one launch (fuse = 1) or as two; from 22 requests on,
-    # EvalPoly runs in two halves.
+    # EvalPoly runs in two halves. The four-plane pack product runs the plain
+    # WideScan (pack_stream = 0) or the streaming schedule with 10 strips per
+    # workgroup (1; the pack matrix's 1024 rows leave the last block 2/5
+    # full) or 8 (2).
     monkeypatch.setenv("PIRG_WIDE_MIN", str(wide_min))
     monkeypatch.setenv("PIRG_DIGIT_MAJOR", str(digit_major))
     monkeypatch.setenv("PIRG_TALL", str(tall))
     monkeypatch.setenv("PIRG_FUSE_SCANS", str(fuse))
+    monkeypatch.setenv("PIRG_PACK_STREAM", str(pack_stream))
     P = O.metrics_tool_params(n=4096, log_d=9, t=2)
     mat = O.synthetic_material(P, seed=nq)
     reqs = [O.synthetic_request(P, seed=1000 * nq + i) for i in range(nq)]
     with server_from_material(P, mat) as srv:
         batch, statuses, _ = srv.process_batch([gpu_request(r) for r in reqs])
         if nq >= 33:
-            assert srv.scans_fused(nq) == bool(fuse and digit_major and tall)
+            assert srv.scans_fused(nq) == bool(fuse and digit_major and tall and not pack_stream)
     assert statuses == [0] * nq
     for i in range(0, nq, max(1, nq // 5)):
         want, _, _ = O.process_response(P, mat, reqs[i])
