@@ -334,6 +334,8 @@ def test_entry_points_reject_null_and_negative_arguments():
     assert lib.pirg_batch_stage(None, -1, None, err, n) == ia.INVALID_ARGUMENT
     assert lib.pirg_batch_run(None, 1, err, n) == ia.INVALID_ARGUMENT
     assert lib.pirg_batch_fetch(None, 1, None, err, n) == ia.INVALID_ARGUMENT
+    fused = ctypes.c_int()
+    assert lib.pirg_server_scans_fused(None, 64, ctypes.byref(fused)) == ia.INVALID_ARGUMENT
     P = pir_server_test_params(*PIR_SERVER_TEST_TYPES["Uint64Mod62Interp1"])
     gp = ia.PirParams(**{**{f: getattr(to_gpu_params(P), f)
                             for f in ia.PirParams.__dataclass_fields__}, "log_degree": 40})
