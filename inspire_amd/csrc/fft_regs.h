@@ -117,9 +117,25 @@ __device__ __forceinline__ void Butterfly(double2& x, double2& y, const double2 
   y = make_double2(ux - vr, uy - vi);
 }
 
+// The butterfly at offset 0 of a stage, whose twiddle is tw[0] =
+// (cos 0, sin ±0) = (1, ±0): y * tw[0] computes (y.x - y.y * ±0,
+// y.x * ±0 + y.y), which equals y except possibly in the sign of a zero
+// component. No later operation turns two zeros of opposite sign into
+// different nonzero values (sums, differences and products by twiddles of
+// the same real numbers round to the same real numbers), and the final
+// round-to-integer maps both to 0, so every integer result is unchanged.
+__device__ __forceinline__ void ButterflyUnit(double2& x, double2& y) {
+  const double ux = x.x, uy = x.y;
+  x = make_double2(ux + y.x, uy + y.y);
+  y = make_double2(ux - y.x, uy - y.y);
+}
+
 // Stages 4K+1 .. 4K+P on every group the thread holds. H (two transforms
 // per CU): a stage's twiddles are loaded just before its butterflies, to
-// keep few registers live; otherwise all of them up front.
+// keep few registers live; otherwise all of them up front. In super-stage 0
+// every thread's offset-0 butterflies (twiddle tw[0]) are known at compile
+// time and skip the complex product: 15 of each thread's 104 butterflies at
+// N = 8192.
 template <int L, int K, bool H>
 __device__ __forceinline__ void Compute(double2 (&v)[16], int t,
                                         const double2* __restrict__ st) {
@@ -134,14 +150,18 @@ __device__ __forceinline__ void Compute(double2 (&v)[16], int t,
       for (int r = 0; r < P; ++r) {
         double2 w[8];
 #pragma unroll
-        for (int q = 0; q < (1 << r); ++q) {
+        for (int q = (K == 0 ? 1 : 0); q < (1 << r); ++q) {
           w[q] = st[(1 << (S0 + r)) - 1 + glo + (q << S0)];
         }
 #pragma unroll
         for (int m = 0; m < (1 << P); ++m) {
           if (m & (1 << r)) continue;
-          Butterfly(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)],
-                    w[m & ((1 << r) - 1)]);
+          const int q = m & ((1 << r) - 1);
+          if (K == 0 && q == 0) {
+            ButterflyUnit(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)]);
+          } else {
+            Butterfly(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)], w[q]);
+          }
         }
       }
     } else {
@@ -149,7 +169,7 @@ __device__ __forceinline__ void Compute(double2 (&v)[16], int t,
 #pragma unroll
       for (int r = 0; r < P; ++r) {
 #pragma unroll
-        for (int q = 0; q < (1 << r); ++q) {
+        for (int q = (K == 0 ? 1 : 0); q < (1 << r); ++q) {
           w[(1 << r) - 1 + q] = st[(1 << (S0 + r)) - 1 + glo + (q << S0)];
         }
       }
@@ -158,8 +178,13 @@ __device__ __forceinline__ void Compute(double2 (&v)[16], int t,
 #pragma unroll
         for (int m = 0; m < (1 << P); ++m) {
           if (m & (1 << r)) continue;
-          Butterfly(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)],
-                    w[(1 << r) - 1 + (m & ((1 << r) - 1))]);
+          const int q = m & ((1 << r) - 1);
+          if (K == 0 && q == 0) {
+            ButterflyUnit(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)]);
+          } else {
+            Butterfly(v[i * (1 << P) + m], v[i * (1 << P) + m + (1 << r)],
+                      w[(1 << r) - 1 + q]);
+          }
         }
       }
     }
