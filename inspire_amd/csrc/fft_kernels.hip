@@ -210,6 +210,29 @@ __device__ __forceinline__ u64 LowBitsD(u64 x, int amount) {
   return amount >= 64 ? x : (x & ((1ULL << amount) - 1));
 }
 
+// Partial sums of the combine kernels: p[0] + p[stride] + ... +
+// p[(n - 1) * stride] mod 2^64 over short runs (K-splits, chunk pairs). A loop
+// with a runtime bound waits for each load before it issues the next, so a
+// run is loaded in two steps: Issue<W> starts the loads of its first W terms
+// (callers issue several runs before adding any), Finish adds them and any
+// terms past W.
+template <int W>
+struct StridedRun {
+  u64 v[W];
+  __device__ __forceinline__ void Issue(const u64* __restrict__ p, long long stride, int n) {
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[j] = j < n ? p[(long long)j * stride] : 0ULL;
+  }
+  __device__ __forceinline__ u64 Finish(const u64* __restrict__ p, long long stride,
+                                        int n) const {
+    u64 s = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) s += v[j];
+    for (int k = W; k < n; ++k) s += p[(long long)k * stride];
+    return s;
+  }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -225,20 +248,21 @@ __global__ void FinalizeCombineKernel(FinalizeArgs A) {
        g += (long long)gridDim.x * blockDim.x) {
     const int q = static_cast<int>(g / A.M);
     const int row = static_cast<int>(g % A.M);  // (s*t + i)*d + m
-    u64 bprime = A.corr_db[q];
-#pragma unroll 8
-    for (int k = 0; k < A.ksplit_db; ++k) {
-      bprime += A.db_partial[((long long)k * A.nq + q) * A.M + row];
-    }
-    u64 bpart = A.corr_pk[A.corr_rows > 0 ? row / A.corr_rows : q];
-#pragma unroll 8
-    for (int k = 0; k < A.ksplit_pk; ++k) {
-      bpart += A.pk_partial[((long long)k * A.nq + q) * A.M + row];
-    }
+    const long long qm = (long long)A.nq * A.M;
+    const u64* dp = A.db_partial + (long long)q * A.M + row;
+    const u64* kp = A.pk_partial + (long long)q * A.M + row;
     const int oi = row / A.d, m = row % A.d;
-    const u64* fp = A.fin_partial + (((long long)q * (A.M / A.d) + oi) * A.npairs) * A.d;
-    u64 inner = 0;
-    for (int pi = 0; pi < A.npairs; ++pi) inner += fp[(long long)pi * A.d + m];
+    const u64* fp = A.fin_partial + (((long long)q * (A.M / A.d) + oi) * A.npairs) * A.d + m;
+    StridedRun<8> rd;
+    StridedRun<4> rk;
+    StridedRun<8> rf;
+    rd.Issue(dp, qm, A.ksplit_db);
+    rk.Issue(kp, qm, A.ksplit_pk);
+    rf.Issue(fp, A.d, A.npairs);
+    const u64 bprime = A.corr_db[q] + rd.Finish(dp, qm, A.ksplit_db);
+    const u64 bpart = A.corr_pk[A.corr_rows > 0 ? row / A.corr_rows : q] +
+                      rk.Finish(kp, qm, A.ksplit_pk);
+    const u64 inner = rf.Finish(fp, A.d, A.npairs);
     const u64 bagg = LowBitsD(LowBitsD(bprime + bpart, A.log_q) + inner, A.log_q);
     A.ct_b[(long long)q * A.M + row] = bagg;
     const int s = oi / A.t, i = oi % A.t;
@@ -439,8 +463,9 @@ __global__ void ExtCombineKernel(ExtCombineArgs A) {
     const int s = static_cast<int>(qs % A.ell);
     const long long q = qs / A.ell;
     const u64* pp = A.ext_partial + ((qs * 2 + which) * A.npairs) * (long long)A.d;
-    u64 v = 0;
-    for (int pi = 0; pi < A.npairs; ++pi) v += pp[(long long)pi * A.d + m];
+    StridedRun<8> rp;
+    rp.Issue(pp + m, A.d, A.npairs);
+    u64 v = rp.Finish(pp + m, A.d, A.npairs);
     const long long oi = (long long)s * A.t + A.step_i;
     const u64 c = (which == 0) ? A.a_tilde[oi * A.d + m]
                                : A.ct_b[(q * A.ell * A.t + oi) * A.d + m];
