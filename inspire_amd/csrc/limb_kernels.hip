@@ -278,11 +278,27 @@ __global__ __launch_bounds__(256) void QueryDigitsSumsKernel(
   u64* const mine = rows[w];
   u64 acc = 0;
   for (int kb = kb_lo + w; kb < kb_hi; kb += kDigitSumWaves) {
-    const int k = kb * 64 + lane;
+    // Two rows per load instruction: lanes 0-31 take row i's 64 values in
+    // pairs, lanes 32-63 row i + 1's (16 bytes per lane).
+    const int k = kb * 64 + 2 * (lane & 31);
+    const int half = lane >> 5;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int q = grp * 16 + i;
-      mine[i * kDigitSumRow + lane] = (q < nq && k < len) ? vec[(long long)q * stride + k] : 0ULL;
+    for (int i = 0; i < 16; i += 2) {
+      const int q = grp * 16 + i + half;
+      const u64* src = vec + (long long)q * stride + k;
+      u64 x0 = 0, x1 = 0;
+      if (q < nq) {
+        if (k + 1 < len && ((reinterpret_cast<uintptr_t>(src) & 15) == 0)) {
+          const ulonglong2 p2 = *reinterpret_cast<const ulonglong2*>(src);
+          x0 = p2.x;
+          x1 = p2.y;
+        } else {
+          x0 = k < len ? src[0] : 0ULL;
+          x1 = k + 1 < len ? src[1] : 0ULL;
+        }
+      }
+      mine[(i + half) * kDigitSumRow + 2 * (lane & 31)] = x0;
+      mine[(i + half) * kDigitSumRow + 2 * (lane & 31) + 1] = x1;
     }
     // The wave's own LDS region: its stores are visible to its loads after
     // the wave-wide fence.
