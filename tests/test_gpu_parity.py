@@ -49,7 +49,10 @@ def test_matrix_multiply_i32_random(rows, cols):
 
 
 @pytest.mark.parametrize("n,d,ub,vb,cb,signed", [
+    (1, 2, 20, 20, 20, False),
     (1, 4, 32, 32, 20, False),
+    (2, 8, 19, 52, 20, True),
+    (2, 16, 19, 52, 19, False),
     (2, 64, 19, 52, 20, True),
     (6, 2048, 19, 52, 19, False),
     (6, 4096, 19, 52, 19, False),
