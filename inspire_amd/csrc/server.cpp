@@ -88,6 +88,7 @@ Slot::~Slot() {
   if (side != nullptr) (void)hipStreamSynchronize(side);
   if (fork_ev != nullptr) (void)hipEventDestroy(fork_ev);
   if (join_ev != nullptr) (void)hipEventDestroy(join_ev);
+  if (pk_join_ev != nullptr) (void)hipEventDestroy(pk_join_ev);
   if (hfork_ev != nullptr) (void)hipEventDestroy(hfork_ev);
   if (hjoin_ev != nullptr) (void)hipEventDestroy(hjoin_ev);
   if (side != nullptr) (void)hipStreamDestroy(side);
@@ -104,6 +105,7 @@ Error Slot::Init() {
   PIRG_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   PIRG_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
   PIRG_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+  PIRG_HIP(hipEventCreateWithFlags(&pk_join_ev, hipEventDisableTiming));
   PIRG_HIP(hipEventCreateWithFlags(&hfork_ev, hipEventDisableTiming));
   PIRG_HIP(hipEventCreateWithFlags(&hjoin_ev, hipEventDisableTiming));
   return Ok();
@@ -877,16 +879,11 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
   // spectra) run beside it on the side stream, forked from and joined back
   // into the slot's stream; the pack scan, bound by HBM too, follows the DB
   // scan. An MFMA-bound scan keeps every CU busy: then all of it stays in line.
+  // The side branch forks here (it needs only the request records) but its
+  // kernels are issued after the DB scan's: issued first, its seven small
+  // launches ran before the DB scan started, ~80 us at one request.
   const bool fork = !pd.wide && S->fork_branches;
-  if (fork) {
-    PIRG_HIP(hipEventRecord(slot.fork_ev, st));
-    PIRG_HIP(hipStreamWaitEvent(slot.side, slot.fork_ev, 0));
-    Error er = pack_digits(slot.side);
-    if (er.ok()) er = finalize_ip(slot.side);
-    if (er.ok() && S->t > 1) er = second_dim_spectra(slot.side);
-    if (!er.ok()) return er;
-    PIRG_HIP(hipEventRecord(slot.join_ev, slot.side));
-  }
+  if (fork) PIRG_HIP(hipEventRecord(slot.fork_ev, st));
   PIRG_HIP(LaunchQueryDigitsSums(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(),
                                  S->dbL.c_off, W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(),
                                  st));
@@ -915,14 +912,25 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
                         W.db_partial.as<u64>(), st));
     PIRG_HIP(kmark(slot.kev[1], st));
   }
+  if (fork) {
+    PIRG_HIP(hipStreamWaitEvent(slot.side, slot.fork_ev, 0));
+    Error er = pack_digits(slot.side);
+    if (!er.ok()) return er;
+    PIRG_HIP(hipEventRecord(slot.pk_join_ev, slot.side));
+    er = finalize_ip(slot.side);
+    if (er.ok() && S->t > 1) er = second_dim_spectra(slot.side);
+    if (!er.ok()) return er;
+    PIRG_HIP(hipEventRecord(slot.join_ev, slot.side));
+  }
   PIRG_HIP(mark(slot.ev[1]));
   {
     Error er;
     if (fused) {
       er = finalize_ip(st);
     } else if (fork) {
-      PIRG_HIP(hipStreamWaitEvent(st, slot.join_ev, 0));
+      PIRG_HIP(hipStreamWaitEvent(st, slot.pk_join_ev, 0));
       er = pack_scan();
+      if (er.ok()) PIRG_HIP(hipStreamWaitEvent(st, slot.join_ev, 0));
     } else {
       er = pack_digits(st);
       if (er.ok()) er = pack_scan();

@@ -125,11 +125,12 @@ struct Slot {
   hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};  // DB / pack scans
   hipEvent_t done = nullptr;                                 // serving: batch finished
   hipEvent_t graph_done = nullptr;  // serving: the batch's kernels finished
-  // The pack branch of a small batch (packing key digits, pack scan,
-  // FinalizeMatrixPack's spectra, second-dimension spectra) runs on `side`
-  // beside the DB scan, forked from and joined back into `stream`.
+  // The request-only kernels of a small batch (packing-key digits,
+  // FinalizeMatrixPack's inner products, second-dimension spectra) run on
+  // `side` beside the DB scan, forked from `stream`; the pack scan waits for
+  // the digits (pk_join_ev), FinalizeCombine for the rest (join_ev).
   hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr, pk_join_ev = nullptr;
   // Larger batches run EvalPoly's Horner chain as two independent halves of
   // their requests, the second half on `side`.
   hipEvent_t hfork_ev = nullptr, hjoin_ev = nullptr;
