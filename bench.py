@@ -623,7 +623,9 @@ def run_vpir(widths, steps: int, warmup: int, cpu: bool, local: bool = False):
                        "int32 pack (8, 4), " + ("local finalize" if local else "FFT finalize"),
            "widths": []}
     try:
-        srv.autotune([w for w in widths])
+        out["autotune"] = srv.autotune([w for w in widths]).splitlines()
+        for line in out["autotune"]:
+            log(f"vPIR autotune {line}")
         for B in widths:
             srv.batch_stage(reqs[:B])
             srv.use_graphs(True)

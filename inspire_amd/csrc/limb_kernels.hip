@@ -927,20 +927,30 @@ __global__ __launch_bounds__(512) void WideScanPipeKernel(
 // leads of SA - 1 and SB - 1 steps. The same schedule serves four-plane
 // matrices (NP = 4, 22 MFMAs per strip and group), whose 4 KiB A strips leave
 // LDS for two or three A stages.
-template <int NP_, int CT, int RW_, int SA_, int SB_>
+//
+// GW query groups per wave: with 4 digit tiles (log q <= 32, 10 MFMAs per
+// strip and group on four planes) one group's B tiles are few for the A
+// bytes a strip brings in, so each wave takes two adjacent groups (a
+// workgroup 8 groups, 128 queries): 4 strips x 2 groups moves 64 KiB per
+// k-block for 640 MFMAs, where 4 strips x 1 group moves 48 KiB for 320.
+template <int NP_, int CT, int RW_, int SA_, int SB_, int GW_ = 1>
 struct StreamShape {
   static constexpr int NP = NP_, RW = RW_, RS = 2 * RW_, AF = RS * NP / 8;
   static_assert((RS * NP) % 8 == 0, "A fragments split evenly over 8 waves");
-  static constexpr int BH0 = (CT + 1) / 2, BH1 = CT - BH0;
+  static constexpr int GW = GW_;       // query groups per wave
+  static constexpr int CW = GW_ * CT;  // B tiles per wave
+  static constexpr int BH0 = (CW + 1) / 2, BH1 = CW - BH0;
   static constexpr int kA = RS * NP * 64;  // v4i of A per stage
-  static constexpr int kB = 4 * CT * 64;   // v4i of B per stage
+  static constexpr int kB = 4 * CW * 64;   // v4i of B per stage
   static constexpr int SA = SA_, SB = SB_;
   // Within a step the ring with the shorter lead issues first, so that the
   // DMAs still allowed in flight at the step's barrier are the latest issued.
   static constexpr bool kBFirst = SB_ < SA_;
   static_assert((kBFirst && SB_ == 2) || (!kBFirst && SA_ == SB_), "in-order vmcnt waits");
   static constexpr size_t kRing = static_cast<size_t>(SA * kA + SB * kB) * 16;
-  static constexpr size_t kRed = static_cast<size_t>(8) * RW * 16 * kQMax * 8;
+  // One epilogue region per (wave, group): RW * 16 rows x kQMax u64.
+  static constexpr int kRedWave = RW * 16 * kQMax;
+  static constexpr size_t kRed = static_cast<size_t>(8) * GW * kRedWave * 8;
   static constexpr size_t kLdsBytes = kRing > kRed ? kRing : kRed;
 };
 
@@ -975,9 +985,10 @@ __device__ __forceinline__ void WideScanTallBody(
     int ksplit, const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial,
     v4i* lds) {
   constexpr int NP = W::NP, RS = W::RS, RW = W::RW, AF = W::AF, SA = W::SA, SB = W::SB;
+  constexpr int GW = W::GW, CW = W::CW;
   const int nx = (strips + RS - 1) / RS;
   WideTile wt;
-  if (!DecodeWideTile(w, nx, nx * ksplit, ngp / 4, &wt)) return;
+  if (!DecodeWideTile(w, nx, nx * ksplit, ngp / (4 * GW), &wt)) return;
   const int lane = threadIdx.x & 63;
   // Wave-uniform, so that addresses derived from it stay in scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -985,7 +996,8 @@ __device__ __forceinline__ void WideScanTallBody(
   const int g = wave & 3;
   const int strip0 = wt.strip_block * RS;
   const int ks = wt.ks;
-  const int grp = wt.zb * 4 + g;
+  // The wave's first group; its GW groups' B tiles are adjacent in btiles.
+  const int grp = (wt.zb * 4 + g) * GW;
   const int kb0 = ks * kbps;
   const int nkb = min(kblocks, kb0 + kbps) - kb0;
   const v4i* A = reinterpret_cast<const v4i*>(tiles);
@@ -999,11 +1011,13 @@ __device__ __forceinline__ void WideScanTallBody(
   const int c0 = h == 0 ? 0 : W::BH0;
   v4i* const ring_a = lds;
   v4i* const ring_b = lds + SA * W::kA;
-  v4i acc[RW][CT];
+  v4i acc[GW][RW][CT];
 #pragma unroll
-  for (int r = 0; r < RW; ++r)
+  for (int gi = 0; gi < GW; ++gi)
 #pragma unroll
-    for (int c = 0; c < CT; ++c) acc[r][c] = v4i{0, 0, 0, 0};
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[gi][r][c] = v4i{0, 0, 0, 0};
 
   // A fragment j of step s (steps past the k-range re-read the last k-block),
   // and this wave's B share cc of step s (past the range: the zero block).
@@ -1019,7 +1033,7 @@ __device__ __forceinline__ void WideScanTallBody(
     if (h == 0 || cc < W::BH1) {
       const long long kbb = s < nkb ? kb0 + s : kblocks;
       const int c = c0 + cc;
-      DmaLoad16(Bg + (kbb * ngp * CT + c) * 64, ring_b + (s % SB) * W::kB + (g * CT + c) * 64);
+      DmaLoad16(Bg + (kbb * ngp * CT + c) * 64, ring_b + (s % SB) * W::kB + (g * CW + c) * 64);
     }
   };
   auto read_a = [&](int s, int blk) -> v4i {
@@ -1027,11 +1041,11 @@ __device__ __forceinline__ void WideScanTallBody(
     return ring_a[(s % SA) * W::kA + ((h * RW + r) * NP + p) * 64 + lane];
   };
   auto read_b = [&](int s, int c) -> v4i {
-    return ring_b[(s % SB) * W::kB + (g * CT + c) * 64 + lane];
+    return ring_b[(s % SB) * W::kB + (g * CW + c) * 64 + lane];
   };
   constexpr int kBlk = RW * NP;      // MFMA blocks per step, one per (strip, plane)
   constexpr int kDma = AF + W::BH0;  // DMA slots per step (an h = 1 wave skips the last)
-  static_assert(kDma <= kBlk && CT <= kBlk, "one DMA slot and one B read per block at most");
+  static_assert(kDma <= kBlk && CW <= kBlk, "one DMA slot and one B read per block at most");
   // Own DMAs allowed in flight: na steps' A issues and nb steps' B issues.
   auto wait_own = [&](auto na_c, auto nb_c) {
     constexpr int na = decltype(na_c)::value, nb = decltype(nb_c)::value;
@@ -1057,9 +1071,9 @@ __device__ __forceinline__ void WideScanTallBody(
   // identical partial sums, and the query tiles' HBM traffic fell from about
   // 20 to 15 GB per dispatch.
   constexpr int kBar = kBlk - 2;
-  static_assert(kBar >= kDma - 1 && kBar >= CT - 1,
+  static_assert(kBar >= kDma - 1 && kBar >= CW - 1,
                 "the barrier follows the step's DMA issues and B reads");
-  auto step = [&](int i, v4i (&cb)[CT], v4i (&nb)[CT], v4i& a_in) {
+  auto step = [&](int i, v4i (&cb)[CW], v4i (&nb)[CW], v4i& a_in) {
     v4i ab[2];
     ab[0] = a_in;
 #pragma unroll
@@ -1069,8 +1083,11 @@ __device__ __forceinline__ void WideScanTallBody(
       if (blk + 1 < kBlk) ab[(blk + 1) & 1] = read_a(i, blk + 1);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int c = p; c < CT; ++c)
-        acc[r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur, cb[c - p], acc[r][c], 0, 0, 0);
+      for (int gi = 0; gi < GW; ++gi)
+#pragma unroll
+        for (int c = p; c < CT; ++c)
+          acc[gi][r][c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur, cb[gi * CT + c - p],
+                                                                acc[gi][r][c], 0, 0, 0);
       __builtin_amdgcn_sched_barrier(0);
       if (blk < kDma) {
         if constexpr (W::kBFirst) {
@@ -1087,7 +1104,7 @@ __device__ __forceinline__ void WideScanTallBody(
           }
         }
       }
-      if (blk < CT) nb[blk] = read_b(i + 1, blk);
+      if (blk < CW) nb[blk] = read_b(i + 1, blk);
       if (blk == kBar) {
         wait_own(std::integral_constant<int, SA - 2>{}, std::integral_constant<int, SB - 2>{});
         LdsBarrier();
@@ -1096,7 +1113,7 @@ __device__ __forceinline__ void WideScanTallBody(
       __builtin_amdgcn_sched_barrier(0);
     }
   };
-  v4i cb0[CT], cb1[CT], a_in;
+  v4i cb0[CW], cb1[CW], a_in;
   constexpr int kPro = (SA - 1 > SB) ? SA - 1 : SB;
 #pragma unroll
   for (int st = 0; st < kPro; ++st) {
@@ -1120,7 +1137,7 @@ __device__ __forceinline__ void WideScanTallBody(
   }
   LdsBarrier();
 #pragma unroll
-  for (int c = 0; c < CT; ++c) cb0[c] = read_b(0, c);
+  for (int c = 0; c < CW; ++c) cb0[c] = read_b(0, c);
   a_in = read_a(0, 0);
   // Every wave has read B(0) before step 0's B(SB) DMA refills its slot.
   LdsBarrier();
@@ -1131,9 +1148,12 @@ __device__ __forceinline__ void WideScanTallBody(
   }
   WaitVmcnt<0>();
   LdsBarrier();
-  u64* red = reinterpret_cast<u64*>(lds) + wave * (RW * 16 * kQMax);
-  WaveEpilogueDigitMajor<RW, CT>(acc, red, lane, grp, nq, rows,
-                                 (long long)(strip0 + h * RW) * 16, ks, partial);
+#pragma unroll
+  for (int gi = 0; gi < GW; ++gi) {
+    u64* red = reinterpret_cast<u64*>(lds) + (wave * GW + gi) * W::kRedWave;
+    WaveEpilogueDigitMajor<RW, CT>(acc[gi], red, lane, grp + gi, nq, rows,
+                                   (long long)(strip0 + h * RW) * 16, ks, partial);
+  }
 }
 
 template <int CT>
@@ -1146,14 +1166,14 @@ __global__ __launch_bounds__(512) void WideScanTallKernel(
 }
 
 // The streaming schedule on four-plane matrices (shape W).
-template <int NP, int CT, int RW, int SA, int SB>
+template <int NP, int CT, int RW, int SA, int SB, int GW = 1>
 __global__ __launch_bounds__(512) void WideScanStreamKernel(
     const int8_t* __restrict__ tiles, int strips, int rows, int kblocks, int kbps, int ksplit,
     const int8_t* __restrict__ btiles, int ngp, int nq, u64* __restrict__ partial) {
   extern __shared__ v4i lds[];
-  WideScanTallBody<CT, StreamShape<NP, CT, RW, SA, SB>>(blockIdx.x, tiles, strips, rows, kblocks,
-                                                        kbps, ksplit, btiles, ngp, nq, partial,
-                                                        lds);
+  WideScanTallBody<CT, StreamShape<NP, CT, RW, SA, SB, GW>>(blockIdx.x, tiles, strips, rows,
+                                                            kblocks, kbps, ksplit, btiles, ngp,
+                                                            nq, partial, lds);
 }
 
 // One operand of a fused scan launch.
@@ -1251,15 +1271,16 @@ hipError_t LaunchWideTallT(const int8_t* tiles, const LimbLayout& L, const ScanP
   return hipGetLastError();
 }
 
-template <int NP, int CT, int RW, int SA, int SB>
+template <int NP, int CT, int RW, int SA, int SB, int GW = 1>
 hipError_t LaunchWideStreamT(const int8_t* tiles, const LimbLayout& L, const ScanPlan& P,
                              const int8_t* btiles, u64* partial, hipStream_t s) {
-  using W = StreamShape<NP, CT, RW, SA, SB>;
+  using W = StreamShape<NP, CT, RW, SA, SB, GW>;
   static_assert(W::kLdsBytes <= 160 * 1024, "WideScanStream LDS");
+  if (P.ngp % (4 * GW) != 0) return hipErrorInvalidValue;
   const long long ntiles = static_cast<long long>((L.strips + W::RS - 1) / W::RS) * P.ksplit;
-  const long long blocks = (ntiles + 7) / 8 * 8 * (P.ngp / 4);
+  const long long blocks = (ntiles + 7) / 8 * 8 * (P.ngp / (4 * GW));
   if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((WideScanStreamKernel<NP, CT, RW, SA, SB>),
+  hipLaunchKernelGGL((WideScanStreamKernel<NP, CT, RW, SA, SB, GW>),
                      dim3(static_cast<unsigned>(blocks)), dim3(512), W::kLdsBytes, s, tiles,
                      L.strips, L.rows, L.kblocks, P.kb_per_split, P.ksplit, btiles, P.ngp, P.nq,
                      partial);
@@ -1281,6 +1302,12 @@ hipError_t LaunchScanNP(const int8_t* tiles, const LimbLayout& L, const ScanPlan
     case 2: return LaunchWideT<NP, 2, false>(tiles, L, P, btiles, partial, s);
     case 4:
       if (!P.digit_major) return LaunchWideT<NP, 4, false>(tiles, L, P, btiles, partial, s);
+      if constexpr (NP == 4) {
+        // Two 16-query groups per wave (128 queries per workgroup), 6 or 8
+        // strips per workgroup.
+        if (P.tall == 1) return LaunchWideStreamT<4, 4, 3, 3, 2, 2>(tiles, L, P, btiles, partial, s);
+        if (P.tall == 2) return LaunchWideStreamT<4, 4, 4, 2, 2, 2>(tiles, L, P, btiles, partial, s);
+      }
       if constexpr (NP <= 2) {
         return LaunchWidePipeT<NP, 4>(tiles, L, P, btiles, partial, s);
       } else {
@@ -1385,12 +1412,12 @@ hipError_t ConfigureWideTall() {
                              static_cast<int>(TallShape<CT>::kLdsBytes));
 }
 
-template <int NP, int CT, int RW, int SA, int SB>
+template <int NP, int CT, int RW, int SA, int SB, int GW = 1>
 hipError_t ConfigureWideStream() {
   return hipFuncSetAttribute(
-      reinterpret_cast<const void*>(WideScanStreamKernel<NP, CT, RW, SA, SB>),
+      reinterpret_cast<const void*>(WideScanStreamKernel<NP, CT, RW, SA, SB, GW>),
       hipFuncAttributeMaxDynamicSharedMemorySize,
-      static_cast<int>(StreamShape<NP, CT, RW, SA, SB>::kLdsBytes));
+      static_cast<int>(StreamShape<NP, CT, RW, SA, SB, GW>::kLdsBytes));
 }
 
 template <int NP, int CT>
@@ -1411,6 +1438,8 @@ hipError_t ConfigureScanKernels() {
                            ConfigureWidePipe<1, 4>(), ConfigureWidePipe<2, 4>(),
                            ConfigureWideTall<7>(), ConfigureWideStream<4, 7, 5, 2, 2>(),
                            ConfigureWideStream<4, 7, 4, 3, 2>(),
+                           ConfigureWideStream<4, 4, 4, 2, 2, 2>(),
+                           ConfigureWideStream<4, 4, 3, 3, 2, 2>(),
                            hipFuncSetAttribute(reinterpret_cast<const void*>(DbPackScanKernel),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                                static_cast<int>(kFusedScanLds))};
@@ -1429,20 +1458,30 @@ int QueriesPerGroup(int tiles, int ns) {
 constexpr int kNarrowTiles[] = {1, 2, 4};
 constexpr int kWideTiles[] = {2, 4, 7};
 
+// Query groups per wide workgroup: 8 for the streaming scans of four-plane
+// matrices with 4 digit tiles (two groups per wave), 4 otherwise.
+int WideGroupsPerWorkgroup(const LimbLayout& L, const ScanPlan& P) {
+  return P.wide && P.digit_major && P.ns == 4 && L.planes == 4 && P.tall != 0 ? 8 : 4;
+}
+
 // Finishes a plan whose kind and tile width are set: groups, K-split and
 // B-tile size. ks_scale multiplies the workgroup target of the K-split.
 void CompletePlan(const LimbLayout& L, int ks_scale, ScanPlan* P) {
   P->qpg = P->digit_major ? 16 : QueriesPerGroup(P->ctg, P->ns);
   const int groups = (P->nq + P->qpg - 1) / P->qpg;
-  P->ngp = P->wide ? (groups + 3) / 4 * 4 : groups;
+  const int gpw = WideGroupsPerWorkgroup(L, *P);
+  P->ngp = P->wide ? (groups + gpw - 1) / gpw * gpw : groups;
   // K-split: exact int32 sums need at most 65536 / planes columns per split;
   // beyond that, enough workgroups to occupy every CU a few times over.
   const int max_kb = kMaxColsPerSplitNP1 / L.planes / 64;
   // Strips per wide workgroup (the streaming shapes of LaunchScanNP).
-  const int wide_rows = P->tall == 1 ? (L.planes == 4 ? 10 : 12) : 8;
+  const int wide_rows = !P->digit_major || P->tall == 0 ? 8
+                        : P->ns == 4                   ? (P->tall == 1 ? 6 : 8)
+                        : P->tall == 1                 ? (L.planes == 4 ? 10 : 12)
+                                                       : 8;
   const int row_wgs = P->wide ? (L.strips + wide_rows - 1) / wide_rows
                               : (L.strips + 4 * kNarrowR - 1) / (4 * kNarrowR);
-  const int z = P->wide ? P->ngp / 4 : P->ngp;
+  const int z = P->wide ? P->ngp / gpw : P->ngp;
   const int target = (P->wide ? 512 : 2048) * std::max(ks_scale, 1);
   int ks = (target + row_wgs * z - 1) / (row_wgs * z);
   const int min_split = (L.kblocks + max_kb - 1) / max_kb;
@@ -1490,7 +1529,7 @@ std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bi
   if (DigitMajorShifts(ns) && nq > 16) {
     add(1, ns, 1, 0);
     if (L.planes == 2 && ns == 7) add(1, ns, 1, 1);
-    if (L.planes == 4 && ns == 7) {
+    if (L.planes == 4) {
       add(1, ns, 1, 1);
       add(1, ns, 1, 2);
     }
@@ -1522,9 +1561,16 @@ ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_qu
     P.wide = 1;
     P.ctg = ns;
     P.digit_major = 1;
-    P.tall = ns != 7 ? 0
-             : L.planes == 2 ? (allow_tall ? 1 : 0)
-             : L.planes == 4 ? four_plane_stream : 0;
+    // Four planes, by default: the 8-strip streaming scan with 7 digit
+    // tiles; with 4 its two-groups-per-wave form, whose 128-query workgroups
+    // pay from about 100 queries on (at 64 the plain WideScan took 3.2 ms
+    // against 4.5 ms for the streaming form's half-empty workgroups, at 256
+    // 10.7 against 8.3 ms).
+    const int fps = four_plane_stream >= 0 ? four_plane_stream
+                    : ns == 7              ? 2
+                    : nq > 96              ? 2
+                                           : 0;
+    P.tall = L.planes == 4 ? fps : ns == 7 && L.planes == 2 ? (allow_tall ? 1 : 0) : 0;
   } else {
     P.wide = 1;
     P.ctg = kWideTiles[2];
@@ -1623,7 +1669,7 @@ hipError_t LaunchScan(const int8_t* tiles, const LimbLayout& L, const ScanPlan& 
   if (P.nq <= 0) return hipSuccess;
   if (P.qpg < 1 || P.ngp < 1) return hipErrorInvalidValue;
   if (P.digit_major && (!P.wide || !DigitMajorShifts(P.ctg) || P.ns != P.ctg ||
-                        P.qpg != 16 || P.ngp % 4 != 0)) {
+                        P.qpg != 16 || P.ngp % WideGroupsPerWorkgroup(L, P) != 0)) {
     return hipErrorInvalidValue;
   }
   if (P.group_rows > 0 && (P.wide || P.group_rows % (16 * kNarrowR) != 0 ||

@@ -159,13 +159,18 @@ struct ScanPlan {
 // wide_min_queries: batches of at least this many queries use WideScan even
 // when they would fit NarrowScan. allow_tall: digit-major scans of two-plane
 // matrices use the tall kernel; four_plane_stream: the `tall` value of
-// digit-major scans of four-plane matrices.
+// digit-major scans of four-plane matrices (0: the plain WideScan; 7 digit
+// tiles: 1 and 2 the streaming scans with 10 and 8 strips per workgroup; 4
+// digit tiles: 1 and 2 those with 6 and 8 strips and two query groups per
+// wave; -1: the default for the shape and batch size).
 ScanPlan MakeScanPlan(const LimbLayout& L, int nq, int out_bits, int wide_min_queries,
                       bool allow_digit_major = true, bool allow_tall = true,
-                      int four_plane_stream = 2);
+                      int four_plane_stream = -1);
 // The configurations an autotuner may time for nq queries (kind, tile width,
 // K-split).
 std::vector<ScanPlan> ScanPlanCandidates(const LimbLayout& L, int nq, int out_bits);
+// Query groups of 16 that one wide workgroup covers (ngp is a multiple of it).
+int WideGroupsPerWorkgroup(const LimbLayout& L, const ScanPlan& P);
 // Grouped plan: ngroups blocks of group_rows rows (a multiple of 64), block b
 // times vector b.
 ScanPlan MakeGroupedScanPlan(const LimbLayout& L, int ngroups, int group_rows, int out_bits);

@@ -1448,8 +1448,11 @@ namespace {
 std::string DescribePlan(const ScanPlan& P) {
   char buf[128];
   std::snprintf(buf, sizeof(buf), "%s tiles=%d ksplit=%d",
-                P.digit_major ? (P.tall == 1 ? "wide-dm-tall" : (P.tall == 2 ? "wide-dm-stream" : "wide-dm"))
-                              : (P.wide ? "wide" : "narrow"),
+                P.digit_major
+                    ? (P.ns == 4 && P.tall != 0
+                           ? (P.tall == 1 ? "wide-dm-stream-2g6" : "wide-dm-stream-2g8")
+                           : (P.tall == 1 ? "wide-dm-tall" : (P.tall == 2 ? "wide-dm-stream" : "wide-dm")))
+                    : (P.wide ? "wide" : "narrow"),
                 P.ctg, P.ksplit);
   return buf;
 }

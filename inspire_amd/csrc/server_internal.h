@@ -178,8 +178,9 @@ struct pirg_server {
   // matrices its tall kernel.
   bool digit_major = true;
   bool tall = true;
-  // Digit-major scans of four-plane (int32 pack) matrices: ScanPlan::tall.
-  int pack_stream = 2;
+  // Digit-major scans of four-plane (int32 pack) matrices: ScanPlan::tall,
+  // or -1 for MakeScanPlan's default.
+  int pack_stream = -1;
   // Batches with enough requests split EvalPoly's Horner chain into two
   // concurrent halves.
   bool horner_split = true;

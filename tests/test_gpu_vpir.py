@@ -52,11 +52,19 @@ def test_vpir_answers_match_oracle(name, local):
                                       err_msg=f"request {i}")
 
 
-@pytest.mark.parametrize("cfg,nq", [("u8", 1), ("u8", 20), ("u8", 70), ("u16", 40),
-                                    ("u16", 129)])
-def test_vpir_device_batches_and_replays(cfg, nq):
+@pytest.mark.parametrize("cfg,nq,pack_stream", [
+    ("u8", 1, -1), ("u8", 20, -1), ("u8", 70, 0), ("u8", 70, 1), ("u8", 70, 2), ("u16", 40, -1),
+    ("u16", 129, 0), ("u16", 129, 1), ("u16", 129, 2), ("u16", 200, -1)])
+def test_vpir_device_batches_and_replays(cfg, nq, pack_stream, monkeypatch):
     # log q = 32: 4-digit queries, digit-major wide scans above 32 queries
-    # (uint8 DB: 1 plane, uint16: 2 planes, int32 pack: 4 planes).
+    # (uint8 DB: 1 plane, uint16: 2 planes, int32 pack: 4 planes). The pack
+    # product runs the plain digit-major WideScan (pack_stream = 0) or the
+    # streaming scans with two query groups per wave and 6 (1) or 8 (2)
+    # strips per workgroup (70 queries fill 5 of a workgroup's 8 groups, 129
+    # queries 9 of two workgroups' 16); -1 keeps the default plan (from 97
+    # queries on the 8-strip streaming scan).
+    if pack_stream >= 0:
+        monkeypatch.setenv("PIRG_PACK_STREAM", str(pack_stream))
     if cfg == "u8":
         P = metrics_like(ell=2)
     else:
