@@ -14,6 +14,7 @@
 //
 // A workgroup of N/16 threads owns one N-point transform (N = 2d); the kernels
 // are instantiated per log2 N and dispatched at launch.
+#include <algorithm>
 #include "fft_regs.h"
 #include "preprocess_internal.h"
 
@@ -60,6 +61,12 @@ struct IpJob {
   u64* const* optr;            // [n_out] base of the pair partials (npairs x d)
 };
 
+// Job placement of IpKernel: outs outputs per request; r > 1 places tiles of
+// r requests x s outputs on one XCD.
+struct IpTile {
+  int outs, r, s;
+};
+
 // Raises *margin to bits unless it already holds as much. The margin settles
 // within the first workgroups of a launch, so a plain load turns nearly every
 // later atomic into none: tens of thousands of same-address atomics per
@@ -98,17 +105,38 @@ __device__ __forceinline__ void MarginMax(double local, unsigned long long* marg
 // on XCD w % 8), so that group's spectra are fetched into one L2 rather than
 // into all eight; the grid is padded to whole rounds of 8 groups.
 //
+// With many outputs per request (tile.r > 1) the jobs are placed in 2-D
+// tiles instead: tile.r requests x tile.s of each request's tile.outs outputs
+// (outputs q * tile.outs + oi), all pairs, on one XCD. A tile's u spectra
+// (tile.s outputs) and v spectra (tile.r requests) then fit that XCD's L2
+// together, where one request's group would stream every output's u spectra
+// through it.
+//
 // FA: the accumulated spectra come ready from MacKernel (acc, one N-point
 // array per job) and the workgroup only inverts and rounds.
 template <int L, bool H, bool FA>
 __global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 4 : 1))) void IpKernel(
-    IpJob jobs, int n_jobs, int group_jobs, int nterms, int nu, int nv, PairList pairs,
-    const double2* __restrict__ st, unsigned long long* margin, const double2* __restrict__ acc) {
+    IpJob jobs, int n_jobs, int group_jobs, IpTile tile, int nterms, int nu, int nv,
+    PairList pairs, const double2* __restrict__ st, unsigned long long* margin,
+    const double2* __restrict__ acc) {
   extern __shared__ __align__(16) double lds[];
   constexpr int N = Geo<L>::N, E = Geo<L>::E, NT = Geo<L>::NT, d = N / 2;
   const int t = threadIdx.x;
   const int w = blockIdx.x;
-  const int job = ((w >> 3) / group_jobs * 8 + (w & 7)) * group_jobs + (w >> 3) % group_jobs;
+  int job;
+  if (tile.r <= 1) {
+    job = ((w >> 3) / group_jobs * 8 + (w & 7)) * group_jobs + (w >> 3) % group_jobs;
+  } else {
+    const int slot = w >> 3;
+    const int tix = slot / group_jobs * 8 + (w & 7);
+    const int jt = slot % group_jobs;
+    const int tiles_s = (tile.outs + tile.s - 1) / tile.s;
+    const int rest = jt / pairs.count;
+    const int q = tix / tiles_s * tile.r + rest / tile.s;
+    const int oi = tix % tiles_s * tile.s + rest % tile.s;
+    if (oi >= tile.outs) return;
+    job = (q * tile.outs + oi) * pairs.count + jt % pairs.count;
+  }
   if (job >= n_jobs) return;
   const int o = job / pairs.count;
   const int pi = job % pairs.count;
@@ -242,16 +270,16 @@ struct StridedRun {
 // FinalizeMatrixPack's integer part (crypto/lwes_to_rlwe.cc:462-476):
 // b_agg = LowBits(LowBits(b + b_part) + inner_h); then the Horner start
 // (pir_server.cc:137) or, when t == 1, the modulus switch (:288-297).
+// One request per blockIdx.y; d is a power of two.
 __global__ void FinalizeCombineKernel(FinalizeArgs A) {
-  const long long total = (long long)A.nq * A.M;
-  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < total;
-       g += (long long)gridDim.x * blockDim.x) {
-    const int q = static_cast<int>(g / A.M);
-    const int row = static_cast<int>(g % A.M);  // (s*t + i)*d + m
-    const long long qm = (long long)A.nq * A.M;
+  const int q = blockIdx.y;
+  const int log_d = __builtin_ctz(static_cast<unsigned>(A.d));
+  const long long qm = (long long)A.nq * A.M;
+  for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < A.M;  // (s*t + i)*d + m
+       row += gridDim.x * blockDim.x) {
     const u64* dp = A.db_partial + (long long)q * A.M + row;
     const u64* kp = A.pk_partial + (long long)q * A.M + row;
-    const int oi = row / A.d, m = row % A.d;
+    const int oi = row >> log_d, m = row & (A.d - 1);
     const u64* fp = A.fin_partial + (((long long)q * (A.M / A.d) + oi) * A.npairs) * A.d + m;
     StridedRun<8> rd;
     StridedRun<4> rk;
@@ -264,7 +292,9 @@ __global__ void FinalizeCombineKernel(FinalizeArgs A) {
                       rk.Finish(kp, qm, A.ksplit_pk);
     const u64 inner = rf.Finish(fp, A.d, A.npairs);
     const u64 bagg = LowBitsD(LowBitsD(bprime + bpart, A.log_q) + inner, A.log_q);
-    A.ct_b[(long long)q * A.M + row] = bagg;
+    // The Horner steps read the packed ciphertexts' b; with t == 1 there are
+    // none (only out is written).
+    if (A.t > 1) A.ct_b[(long long)q * A.M + row] = bagg;
     const int s = oi / A.t, i = oi % A.t;
     if (i == A.t - 1) {
       const u64 a = A.a_tilde[(long long)oi * A.d + m] & A.cmask;
@@ -575,18 +605,24 @@ struct SpectraL {
 
 template <int L>
 struct IpL {
-  static hipError_t Run(const IpJob& jobs, int n_out, int outs_per_group, int nterms, int nu,
-                        int nv, const PairList& pairs, const double2* st,
+  static hipError_t Run(const IpJob& jobs, int n_out, int outs_per_group, const IpTile& tile,
+                        int nterms, int nu, int nv, const PairList& pairs, const double2* st,
                         unsigned long long* margin, double2* acc, hipStream_t s) {
     const long long n_jobs = static_cast<long long>(n_out) * pairs.count;
-    const long long group_jobs = static_cast<long long>(outs_per_group) * pairs.count;
-    const long long groups = (n_out + outs_per_group - 1) / outs_per_group;
+    long long group_jobs = static_cast<long long>(outs_per_group) * pairs.count;
+    long long groups = (n_out + outs_per_group - 1) / outs_per_group;
+    if (tile.r > 1) {
+      const long long nreq = n_out / tile.outs;
+      group_jobs = static_cast<long long>(tile.r) * tile.s * pairs.count;
+      groups = (nreq + tile.r - 1) / tile.r * ((tile.outs + tile.s - 1) / tile.s);
+    }
     const long long blocks = (groups + 7) / 8 * 8 * group_jobs;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     if (n_jobs >= kHalfMinBlocks) {
       hipLaunchKernelGGL((IpKernel<L, true, false>), dim3(static_cast<unsigned>(blocks)),
                          dim3(Geo<L>::NT), Geo<L>::LdsBytes(true), s, jobs,
-                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), nterms, nu, nv,
+                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), tile, nterms,
+                         nu, nv,
                          pairs, st, margin, nullptr);
     } else if (acc != nullptr && n_jobs <= kIpSplitMaxJobs) {
       const long long total = n_jobs << L;
@@ -594,12 +630,14 @@ struct IpL {
                          0, s, jobs, static_cast<int>(n_jobs), nterms, nu, nv, pairs, L, acc);
       hipLaunchKernelGGL((IpKernel<L, false, true>), dim3(static_cast<unsigned>(blocks)),
                          dim3(Geo<L>::NT), Geo<L>::LdsBytes(false), s, jobs,
-                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), nterms, nu, nv,
+                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), tile, nterms,
+                         nu, nv,
                          pairs, st, margin, acc);
     } else {
       hipLaunchKernelGGL((IpKernel<L, false, false>), dim3(static_cast<unsigned>(blocks)),
                          dim3(Geo<L>::NT), Geo<L>::LdsBytes(false), s, jobs,
-                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), nterms, nu, nv,
+                         static_cast<int>(n_jobs), static_cast<int>(group_jobs), tile, nterms,
+                         nu, nv,
                          pairs, st, margin, nullptr);
     }
     return hipGetLastError();
@@ -688,11 +726,16 @@ hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
                     u64* const* optr, int n_out, int outs_per_group, int nterms, int nu,
                     int nv, const PairList& pairs, const FftDims& F,
                     const double2* tw_bwd, unsigned long long* margin,
-                    hipStream_t s, double2* acc) {
+                    hipStream_t s, double2* acc, int tile_requests, int tile_outs) {
   if (n_out <= 0 || pairs.count <= 0) return hipSuccess;
   if (!ValidDims(F) || outs_per_group <= 0) return hipErrorInvalidValue;
+  IpTile tile{outs_per_group, 1, 1};
+  if (tile_requests > 1) {
+    if (tile_outs <= 0 || n_out % outs_per_group != 0) return hipErrorInvalidValue;
+    tile = IpTile{outs_per_group, tile_requests, tile_outs};
+  }
   IpJob jobs{uptr, vptr, optr};
-  return DispatchLogN<IpL>(F.log_N, jobs, n_out, outs_per_group, nterms, nu, nv, pairs,
+  return DispatchLogN<IpL>(F.log_N, jobs, n_out, outs_per_group, tile, nterms, nu, nv, pairs,
                            tw_bwd, margin, acc, s);
 }
 
@@ -718,8 +761,10 @@ hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,
 }
 
 hipError_t LaunchFinalizeCombine(const FinalizeArgs& A, hipStream_t s) {
-  hipLaunchKernelGGL(FinalizeCombineKernel, dim3(GridFor((long long)A.nq * A.M)),
-                     dim3(256), 0, s, A);
+  if (A.nq <= 0 || A.M <= 0) return hipSuccess;
+  if (A.d <= 0 || (A.d & (A.d - 1)) != 0 || A.nq > 65535) return hipErrorInvalidValue;
+  const int bx = static_cast<int>(std::min<long long>((A.M + 255) / 256, 8192));
+  hipLaunchKernelGGL(FinalizeCombineKernel, dim3(bx, A.nq), dim3(256), 0, s, A);
   return hipGetLastError();
 }
 

@@ -278,13 +278,17 @@ struct ExtCombineArgs {
 // request's outputs); their workgroups are placed on one XCD. acc (optional,
 // kIpSplitMaxJobs * N complex values): launches of at most that many
 // (output, pair) jobs first accumulate the spectra across the whole GPU into
-// acc, then invert one job per workgroup.
+// acc, then invert one job per workgroup. tile_requests > 1 (n_out a multiple
+// of outs_per_group): tiles of tile_requests requests x tile_outs of their
+// outputs take one XCD instead, so that the tile's u and v spectra share its
+// L2.
 constexpr int kIpSplitMaxJobs = 64;
 hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
                     u64* const* optr, int n_out, int outs_per_group, int nterms, int nu,
                     int nv, const PairList& pairs, const FftDims& F,
                     const double2* tw_bwd, unsigned long long* margin,
-                    hipStream_t s, double2* acc = nullptr);
+                    hipStream_t s, double2* acc = nullptr, int tile_requests = 1,
+                    int tile_outs = 0);
 hipError_t LaunchFinalizeCombine(const FinalizeArgs& A, hipStream_t s);
 hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,
                                const double2* tw_fwd, hipStream_t s);

@@ -800,6 +800,11 @@ bool FuseScans(const pirg_server* S, int nq, const ScanPlan& pd, const ScanPlan&
   return it == S->tuned_fuse.end() || it->second;
 }
 
+// FinalizeMatrixPack's inner-product tiles when a request has many (shard, i)
+// blocks: requests x blocks per XCD.
+constexpr int kFinTileRequests = 8;
+constexpr int kFinTileBlocks = 8;
+
 Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io) {
   Workspace& W = slot.ws;
   hipStream_t st = slot.stream;
@@ -859,12 +864,17 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
                            s));
     const size_t n_fin_cap = static_cast<size_t>(W.cap) * lt;
     void* const* tab = W.fin_tab.as<void* const>();
+    // Many (shard, i) blocks per request (vPIR's 256 shards): tiles of 8
+    // requests x 8 blocks share their y_vec_h and t_vec_h spectra through one
+    // L2 (6 MiB per 128 jobs at d = 2048, against 16.5 MiB for 128 jobs of
+    // one request).
+    const int tile_r = lt >= 32 && nq >= 2 ? std::min(nq, kFinTileRequests) : 1;
     PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(tab),
                       reinterpret_cast<const double2* const*>(tab + n_fin_cap),
                       reinterpret_cast<u64* const*>(tab + 2 * n_fin_cap),
                       nq * lt, lt, S->k, S->fin_u.num_chunks, S->fin_v.num_chunks,
                       S->fin_pairs, S->F, S->tw_bwd.as<double2>(), margin, s,
-                      W.mac_acc.as<double2>()));
+                      W.mac_acc.as<double2>(), tile_r, kFinTileBlocks));
     return Ok();
   };
   // EvalPoly's RGSW spectra (pir_server.cc:126-150), from the request alone.

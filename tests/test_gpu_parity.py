@@ -212,6 +212,22 @@ def test_wide_batches_match_oracle(nq, wide_min, digit_major, tall, fuse, pack_s
         np.testing.assert_array_equal(batch[i].ciphertexts, want)
 
 
+@pytest.mark.parametrize("nq,ell", [(11, 5), (2, 4), (9, 1)])
+def test_finalize_tiles_match_oracle(nq, ell):
+    # FinalizeMatrixPack's inner products in tiles of 8 requests x 8 (shard,
+    # i) blocks when a request has 32 or more blocks (l * t = 40 or 32 here:
+    # ragged request and block tiles), and one request's group otherwise.
+    P = O.metrics_tool_params(n=8192, log_d=8, t=8, ell=ell)
+    mat = O.synthetic_material(P, seed=40 + nq)
+    reqs = [O.synthetic_request(P, seed=4000 + 50 * ell + i) for i in range(nq)]
+    with server_from_material(P, mat) as srv:
+        batch, statuses, _ = srv.process_batch([gpu_request(r) for r in reqs])
+    assert statuses == [0] * nq
+    for i in sorted({0, nq // 2, nq - 1}):
+        want, _, _ = O.process_response(P, mat, reqs[i])
+        np.testing.assert_array_equal(batch[i].ciphertexts, want, err_msg=f"request {i}")
+
+
 def test_multi_chunk_upload(monkeypatch):
     # The staged host -> HBM conversion in many small chunks (chunk boundaries
     # that split 16-row strips).

@@ -83,6 +83,22 @@ def test_vpir_device_batches_and_replays(cfg, nq, pack_stream, monkeypatch):
         np.testing.assert_array_equal(first[i], oracle_answer(P, False, mat, *reqs[i]))
 
 
+@pytest.mark.parametrize("local", [False, True])
+def test_vpir_many_shards_match_oracle(local):
+    # 37 shards: FinalizeMatrixPack's inner products run in tiles of 8
+    # requests x 8 shards (ragged in both), 11 requests.
+    P = vpir_params(32, 4, ell=37, log_d=8, n=2048, pack_log_digit=8, pack_num_digits=4,
+                    db_bytes=1, p=256)
+    srv, mat = make(P, local, seed=31)
+    reqs = [request(P, 700 + i, local) for i in range(11)]
+    with srv:
+        got, st, _ = srv.process_batch(reqs)
+    assert st == [0] * len(reqs)
+    for i in (0, 5, 10):
+        np.testing.assert_array_equal(got[i], oracle_answer(P, local, mat, *reqs[i]),
+                                      err_msg=f"request {i}")
+
+
 def test_vpir_errors_match_oracle():
     P = vpir_params(ell=2, **VPIR_TEST_TYPES["Uint64Mod62"])
     s = O.sizes(P)
