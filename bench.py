@@ -820,6 +820,8 @@ def main():
     ap.add_argument("--no-preprocess", action="store_true")
     ap.add_argument("--only-preprocess", action="store_true",
                     help="run only the preprocessing section (for rocprofv3)")
+    ap.add_argument("--only-vpir", action="store_true",
+                    help="run only the vPIR leg (graph replays at --widths, no CPU baseline)")
     ap.add_argument("--cpu-queries", type=int, default=3)
     ap.add_argument("--serve-rates", default="100,500,1000,2000,5000,10000,20000")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
@@ -842,6 +844,10 @@ def main():
     P = ia.metrics_tool_params(n=args.num_entries, log_d=args.log_degree, t=args.interp_degree)
     if args.only_preprocess:
         print(json.dumps({"preprocess": run_preprocess(P, cpu=not args.no_cpu)}))
+        return
+    if args.only_vpir:
+        widths = [int(x) for x in args.widths.split(",")]
+        print(json.dumps({"vpir": run_vpir(widths, args.steps, args.warmup, cpu=False)}))
         return
     d, t = P.degree, P.interp_degree
     r = P.num_entries // t

@@ -61,7 +61,7 @@ struct IpJob {
   u64* const* optr;            // [n_out] base of the pair partials (npairs x d)
 };
 
-// Job placement of IpKernel: outs outputs per request; r > 1 places tiles of
+// Job placement of IpKernel: outs outputs per request; s > 0 places tiles of
 // r requests x s outputs on one XCD.
 struct IpTile {
   int outs, r, s;
@@ -105,12 +105,12 @@ __device__ __forceinline__ void MarginMax(double local, unsigned long long* marg
 // on XCD w % 8), so that group's spectra are fetched into one L2 rather than
 // into all eight; the grid is padded to whole rounds of 8 groups.
 //
-// With many outputs per request (tile.r > 1) the jobs are placed in 2-D
+// With many outputs per request (tile.s > 0) the jobs are placed in 2-D
 // tiles instead: tile.r requests x tile.s of each request's tile.outs outputs
 // (outputs q * tile.outs + oi), all pairs, on one XCD. A tile's u spectra
 // (tile.s outputs) and v spectra (tile.r requests) then fit that XCD's L2
 // together, where one request's group would stream every output's u spectra
-// through it.
+// through it (and a single request's jobs would all run on one XCD).
 //
 // FA: the accumulated spectra come ready from MacKernel (acc, one N-point
 // array per job) and the workgroup only inverts and rounds.
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(Geo<L>::NT) __attribute__((amdgpu_waves_per_eu(H ? 
   const int t = threadIdx.x;
   const int w = blockIdx.x;
   int job;
-  if (tile.r <= 1) {
+  if (tile.s <= 0) {
     job = ((w >> 3) / group_jobs * 8 + (w & 7)) * group_jobs + (w >> 3) % group_jobs;
   } else {
     const int slot = w >> 3;
@@ -611,7 +611,7 @@ struct IpL {
     const long long n_jobs = static_cast<long long>(n_out) * pairs.count;
     long long group_jobs = static_cast<long long>(outs_per_group) * pairs.count;
     long long groups = (n_out + outs_per_group - 1) / outs_per_group;
-    if (tile.r > 1) {
+    if (tile.s > 0) {
       const long long nreq = n_out / tile.outs;
       group_jobs = static_cast<long long>(tile.r) * tile.s * pairs.count;
       groups = (nreq + tile.r - 1) / tile.r * ((tile.outs + tile.s - 1) / tile.s);
@@ -729,9 +729,9 @@ hipError_t LaunchIp(const double2* const* uptr, const double2* const* vptr,
                     hipStream_t s, double2* acc, int tile_requests, int tile_outs) {
   if (n_out <= 0 || pairs.count <= 0) return hipSuccess;
   if (!ValidDims(F) || outs_per_group <= 0) return hipErrorInvalidValue;
-  IpTile tile{outs_per_group, 1, 1};
-  if (tile_requests > 1) {
-    if (tile_outs <= 0 || n_out % outs_per_group != 0) return hipErrorInvalidValue;
+  IpTile tile{outs_per_group, 1, 0};
+  if (tile_outs > 0) {
+    if (tile_requests < 1 || n_out % outs_per_group != 0) return hipErrorInvalidValue;
     tile = IpTile{outs_per_group, tile_requests, tile_outs};
   }
   IpJob jobs{uptr, vptr, optr};

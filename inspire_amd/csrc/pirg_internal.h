@@ -278,8 +278,8 @@ struct ExtCombineArgs {
 // request's outputs); their workgroups are placed on one XCD. acc (optional,
 // kIpSplitMaxJobs * N complex values): launches of at most that many
 // (output, pair) jobs first accumulate the spectra across the whole GPU into
-// acc, then invert one job per workgroup. tile_requests > 1 (n_out a multiple
-// of outs_per_group): tiles of tile_requests requests x tile_outs of their
+// acc, then invert one job per workgroup. tile_outs > 0 (n_out a multiple of
+// outs_per_group): tiles of tile_requests requests x tile_outs of their
 // outputs take one XCD instead, so that the tile's u and v spectra share its
 // L2.
 constexpr int kIpSplitMaxJobs = 64;

@@ -867,14 +867,15 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
     // Many (shard, i) blocks per request (vPIR's 256 shards): tiles of 8
     // requests x 8 blocks share their y_vec_h and t_vec_h spectra through one
     // L2 (6 MiB per 128 jobs at d = 2048, against 16.5 MiB for 128 jobs of
-    // one request).
-    const int tile_r = lt >= 32 && nq >= 2 ? std::min(nq, kFinTileRequests) : 1;
+    // one request), and a single request's jobs spread over every XCD.
+    const int tile_r = std::min(nq, kFinTileRequests);
+    const int tile_s = lt >= 32 ? kFinTileBlocks : 0;
     PIRG_HIP(LaunchIp(reinterpret_cast<const double2* const*>(tab),
                       reinterpret_cast<const double2* const*>(tab + n_fin_cap),
                       reinterpret_cast<u64* const*>(tab + 2 * n_fin_cap),
                       nq * lt, lt, S->k, S->fin_u.num_chunks, S->fin_v.num_chunks,
                       S->fin_pairs, S->F, S->tw_bwd.as<double2>(), margin, s,
-                      W.mac_acc.as<double2>(), tile_r, kFinTileBlocks));
+                      W.mac_acc.as<double2>(), tile_r, tile_s));
     return Ok();
   };
   // EvalPoly's RGSW spectra (pir_server.cc:126-150), from the request alone.

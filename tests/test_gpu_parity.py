@@ -212,7 +212,7 @@ def test_wide_batches_match_oracle(nq, wide_min, digit_major, tall, fuse, pack_s
         np.testing.assert_array_equal(batch[i].ciphertexts, want)
 
 
-@pytest.mark.parametrize("nq,ell", [(11, 5), (2, 4), (9, 1)])
+@pytest.mark.parametrize("nq,ell", [(11, 5), (2, 4), (1, 5), (9, 1)])
 def test_finalize_tiles_match_oracle(nq, ell):
     # FinalizeMatrixPack's inner products in tiles of 8 requests x 8 (shard,
     # i) blocks when a request has 32 or more blocks (l * t = 40 or 32 here:
