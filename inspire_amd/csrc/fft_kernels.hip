@@ -270,11 +270,11 @@ struct StridedRun {
 // FinalizeMatrixPack's integer part (crypto/lwes_to_rlwe.cc:462-476):
 // b_agg = LowBits(LowBits(b + b_part) + inner_h); then the Horner start
 // (pir_server.cc:137) or, when t == 1, the modulus switch (:288-297).
-// One request per blockIdx.y; d is a power of two.
+// Requests along blockIdx.y (strided by gridDim.y); d is a power of two.
 __global__ void FinalizeCombineKernel(FinalizeArgs A) {
-  const int q = blockIdx.y;
   const int log_d = __builtin_ctz(static_cast<unsigned>(A.d));
   const long long qm = (long long)A.nq * A.M;
+  for (int q = blockIdx.y; q < A.nq; q += gridDim.y)
   for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < A.M;  // (s*t + i)*d + m
        row += gridDim.x * blockDim.x) {
     const u64* dp = A.db_partial + (long long)q * A.M + row;
@@ -762,9 +762,9 @@ hipError_t LaunchDigitsSpectra(const DigitsArgs& A, const FftDims& F,
 
 hipError_t LaunchFinalizeCombine(const FinalizeArgs& A, hipStream_t s) {
   if (A.nq <= 0 || A.M <= 0) return hipSuccess;
-  if (A.d <= 0 || (A.d & (A.d - 1)) != 0 || A.nq > 65535) return hipErrorInvalidValue;
+  if (A.d <= 0 || (A.d & (A.d - 1)) != 0) return hipErrorInvalidValue;
   const int bx = static_cast<int>(std::min<long long>((A.M + 255) / 256, 8192));
-  hipLaunchKernelGGL(FinalizeCombineKernel, dim3(bx, A.nq), dim3(256), 0, s, A);
+  hipLaunchKernelGGL(FinalizeCombineKernel, dim3(bx, std::min(A.nq, 65535)), dim3(256), 0, s, A);
   return hipGetLastError();
 }
 

@@ -895,9 +895,17 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
   // launches ran before the DB scan started, ~80 us at one request.
   const bool fork = !pd.wide && S->fork_branches;
   if (fork) PIRG_HIP(hipEventRecord(slot.fork_ev, st));
-  PIRG_HIP(LaunchQueryDigitsSums(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(),
-                                 S->dbL.c_off, W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(),
-                                 st));
+  // The DB product's offset correction (the query's column sums) is read only
+  // by FinalizeCombine: with a side branch it is computed there, and the DB
+  // scan follows its query digits at once.
+  const bool sums_on_side = fork && !pd.digit_major;
+  if (sums_on_side) {
+    PIRG_HIP(LaunchQueryDigits(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(), st));
+  } else {
+    PIRG_HIP(LaunchQueryDigitsSums(rec_fd, rw, S->K, S->dbL, pd, W.bt_db.as<int8_t>(),
+                                   S->dbL.c_off, W.corr_db.as<u64>() + nq, W.corr_db.as<u64>(),
+                                   st));
+  }
   // Two MFMA-bound scans of the kinds LaunchScanPair fuses run as one launch
   // (the pack product's query digits first) when the DB product takes few
   // rounds of workgroups: the pack product's then fill the CUs that its last,
@@ -928,6 +936,10 @@ Error EnqueueBatch(pirg_server* S, Slot& slot, int nq, bool timed, bool host_io)
     Error er = pack_digits(slot.side);
     if (!er.ok()) return er;
     PIRG_HIP(hipEventRecord(slot.pk_join_ev, slot.side));
+    if (sums_on_side) {
+      PIRG_HIP(LaunchVectorSums(rec_fd, nq, rw, S->K, S->dbL.c_off, W.corr_db.as<u64>() + nq,
+                                W.corr_db.as<u64>(), slot.side));
+    }
     er = finalize_ip(slot.side);
     if (er.ok() && S->t > 1) er = second_dim_spectra(slot.side);
     if (!er.ok()) return er;
