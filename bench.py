@@ -648,6 +648,13 @@ def run_vpir(widths, steps: int, warmup: int, cpu: bool, local: bool = False):
                 line["pack_scan_roofline"] = {"bound": "hbm", "achieved": gbs,
                                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                               "frac": gbs / HBM_PEAK_GBS}
+            elif B >= 64:
+                # Four planes x 4 digits (log q = 32): 10 (plane, digit)
+                # products with 8 * (p + s) < 32 per pack-matrix element.
+                tops = 2.0 * M * k * d * B * 10 / (line["pack_scan_ms"] * 1e-3) / 1e12
+                line["pack_scan_roofline"] = {"bound": "mfma", "achieved": tops,
+                                              "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                                              "frac": tops / INT8_PEAK_TOPS}
             out["widths"].append(line)
             log(f"vPIR width {B}: {line['qps']:.1f} q/s, {line['ms_per_batch']:.3f} ms/batch, "
                 f"pack scan {line['pack_scan_ms']:.3f} ms")
