@@ -11,6 +11,12 @@ the GPU, against the oracle.
   slots; sampled answers must equal the oracle's ProcessResponse
   (crypto/pir/server/pir_server.cc:152-306) and the same request answered
   alone.
+- The vPIR online answer at vpir_metrics_tool's parameters (d = 2048, 256
+  shards, q = 2^32, uint8 DB, int32 pack matrix with gadget (8, 4): 16 GiB):
+  a batch of 256 requests (the two-groups-per-wave pack scan, the tiled
+  FinalizeMatrixPack inner products) and one request alone, sampled against
+  the oracle's VpirServer::ProcessResponse (crypto/vpir/server/
+  vpir_server.cc:101-195).
 """
 import threading
 
@@ -19,7 +25,8 @@ import pytest
 
 import inspire_amd as ia
 import oracle as O
-from helpers import gpu_request, server_from_material
+from helpers import gpu_request, server_from_material, to_gpu_params
+from vpir_helpers import metrics_like, request
 
 pytestmark = pytest.mark.gpu
 
@@ -92,3 +99,27 @@ def test_serving_at_full_shape_matches_oracle_and_alone():
         np.testing.assert_array_equal(served[i], alone[i], err_msg=f"request {i}")
         want, _, _ = O.process_response(P, mat, reqs[i])
         np.testing.assert_array_equal(served[i], want, err_msg=f"request {i}")
+
+
+def test_vpir_metrics_shape_256_requests_match_oracle():
+    P = metrics_like(ell=256, n=2048, log_d=11)
+    s = O.sizes(P)
+    d, ell, k, n = s.d, s.ell, s.k, P.num_entries
+    M = ell * d
+    rng = np.random.default_rng(91)
+    db = np.frombuffer(rng.bytes(M * n), dtype=np.uint64).reshape(M, n // 8)
+    # int32 entries inside the preprocessing bound |x| < 2^(log_digit + log d - 1).
+    pack32 = (np.frombuffer(rng.bytes(M * k * d * 4), dtype=np.int32) >> 13).reshape(M, k * d)
+    pack = pack32.view(np.uint64).reshape(M, k * d // 2)
+    a_tilde = rng.integers(0, 1 << 32, size=(ell, d), dtype=np.uint64)
+    t_vec_h = rng.integers(-128, 128, size=(ell, k, d)).astype(np.uint64)
+    reqs = [request(P, 9000 + i, False) for i in range(256)]
+    srv = ia.VpirServer.create(to_gpu_params(P), False, db, M, n, pack, a_tilde, t_vec_h)
+    with srv:
+        got, st, _ = srv.process_batch(reqs)
+        one = srv.process_response(*reqs[200])
+    assert st == [0] * len(reqs)
+    np.testing.assert_array_equal(one, got[200])
+    for i in (0, 131, 255):
+        want = O.vpir_process(P, False, db, n, pack, a_tilde, t_vec_h, *reqs[i])
+        np.testing.assert_array_equal(got[i], want, err_msg=f"request {i}")
